@@ -1,0 +1,241 @@
+/*
+ * mf.h -- C-ABI of the MI355X (gfx950) Mille-feuille solver library
+ * (libmfamd.so).  Plain pointers and sizes only.
+ *
+ * Two groups of entry points:
+ *  1. Drop-in replacements for the reference's host-visible functions, same
+ *     names, argument meaning and side effects (cited per declaration).
+ *  2. The library's own API (mf_*): device-resident matrices whose
+ *     reference tile layout is built on the GPU, tiled SpMV, persistent
+ *     CG/BiCGSTAB (FP64 or per-tile mixed precision with FP64 iterative
+ *     refinement), batched solves.
+ *
+ * Reference citations use PC/ = Performance-Comparison/, MP/ = Mixed-Precision/,
+ * MC/ = Memory-Cost/.
+ */
+#ifndef MF_H
+#define MF_H
+#include <stddef.h>
+#include "mf_tile_matrix.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+typedef enum {
+    MF_OK = 0,
+    MF_E_IO = -1,         /* fopen failed (mmio_info/mmio_data return -1) */
+    MF_E_BANNER = -2,     /* Matrix Market banner (mmio returns -2) */
+    MF_E_SIZE = -4,       /* size line (mmio returns -4) */
+    MF_E_ARG = -10,       /* invalid argument or shape */
+    MF_E_HIP = -11,       /* HIP runtime error (message in mf_last_error) */
+    MF_E_NOMEM = -12,
+    MF_E_FORMAT = -13,    /* matrix not representable in the device layout */
+    MF_E_NODEV = -14,     /* no usable gfx950 device */
+    MF_E_UNSUPPORTED = -15,
+    MF_E_BREAKDOWN = -20, /* Krylov breakdown (zero/negative curvature, rho=0) */
+    MF_E_MAXIT = -21,     /* iteration limit before tolerance */
+    MF_E_TIMEOUT = -22,   /* grid barrier spin limit reached; kernel aborted */
+    MF_E_NONFINITE = -23  /* NaN/Inf in a reduction */
+} mf_status;
+
+const char *mf_status_str(int status);
+const char *mf_last_error(void);
+
+/* ------------------------------------------- 1. reference drop-ins ------- */
+
+/* PC/mmio_highlevel.h:17 -- header pass: sizes, nnz after symmetric
+ * expansion.  Returns 0, -1 (fopen), -2 (banner), -4 (size line). */
+int mmio_info(int *m, int *n, int *nnz, int *isSymmetric, char *filename);
+/* PC/mmio_highlevel.h:275 -- CSR in file order (no column sort, duplicates
+ * kept), symmetric/hermitian off-diagonals mirrored. */
+int mmio_data(int *csrRowPtr, int *csrColIdx, double *csrVal, char *filename);
+/* PC/encode.h:29 / :52 -- 4-bit column packing, two per byte. */
+void encode(unsigned char *str, unsigned char *res, int str_len, int offset);
+void decode(unsigned char *str, unsigned char *res, int res_len);
+/* PC/utils.h:34 / :50 -- in-place exclusive scans (char form wraps mod 256). */
+void exclusive_scan(int *input, int length);
+void exclusive_scan_char(unsigned char *input, int length);
+/* PC/csr2block.h:267 -- builds the 16x16 (MF_TILE_BLOCK_SIZE) tile layout.
+ * Here the build runs on the GPU and the CSR fields of Tile_matrix are filled
+ * with host copies (malloc'ed, release with Tile_destroy).  On error the
+ * struct is zeroed and mf_last_error() says why. */
+void Tile_create(Tile_matrix *matrix, int rowA, int colA, int nnzA, int *csrRowPtrA,
+                 int *csrColIdxA, double *csrValA, float *csrValA_Low);
+/* PC/format.h:88 -- frees what Tile_create allocated. */
+void Tile_destroy(Tile_matrix *matrix);
+/* PC/Mille-feuille_CG_AMD.cu:1240 -- FP64 CG.  Rows are truncated to a
+ * multiple of 16 as in the reference (:1246-1248).  Solves to true relative
+ * residual MF_DROPIN_TOL (1e-10) with at most max(maxiter, 10000) iterations
+ * on the GPU persistent kernel, zeroes then writes x, stores the iteration
+ * count in *iter and appends one row to cg_syncfree_amd_mi200.csv in the
+ * reference format (:2021-2030). threshold is ignored as in the reference. */
+void cg_solve_inc(int *RowPtr, int *ColIdx, double *Val, float *Val_Low, double *x, double *b,
+                  int n, int *iter, int maxiter, double threshold, char *filename, int nnzR,
+                  int ori, int block_nnz);
+/* PC/Mille-feuille_BiCGSTAB_AMD.cu:837 -- FP64 BiCGSTAB; overwrites rhs with
+ * A*1 as the reference does (:1343-1350), solves to 1e-10, appends one row
+ * to bicg_syncfree_amd_mi200.csv (:1606-1615).  Returns 1 if not converged. */
+int bicgstab(int *RowPtr, int *ColIdx, double *Val, double *rhs, double *x, int nnzR,
+             char *filename, int n, int block_nnz);
+
+/* ------------------------------------------------- 2. library API -------- */
+
+/* Host copy of the reference tile layout (field meaning as Tile_matrix,
+ * PC/format.h:14-86, CSR-tile fields only). */
+typedef struct {
+    int B, rowA, colA, tilem, tilen, tilenum, csrsize, csrptrlen;
+    int *tile_ptr;          /* tilem+1 */
+    int *tile_columnidx;    /* tilenum */
+    int *tile_nnz;          /* tilenum+1, scanned */
+    char *Format;           /* tilenum, all 0 (CSR tiles) */
+    int *blknnz;            /* tilenum+1, scanned */
+    unsigned char *blknnznnz; /* tilenum+1, per-tile nnz mod 256 */
+    int *csr_offset;        /* tilenum+1 */
+    int *csrptr_offset;     /* tilenum+1 */
+    double *Blockcsr_Val;   /* csrsize */
+    float *Blockcsr_Val_Low;/* csrsize */
+    unsigned char *Blockcsr_Ptr; /* csrptrlen, uint8 row starts */
+    unsigned char *Tile_csr_Col; /* csrsize, tile-local column */
+    unsigned char *csr_compressedIdx; /* ceil(csrsize/2); zero, as the reference leaves it */
+} mf_ref_tiles;
+
+typedef struct mf_matrix_s *mf_matrix;
+
+/* Library set-up on a device (also done lazily by every entry point). */
+int mf_init(int device);
+
+/* Uploads a CSR matrix and builds the reference tile layout on the GPU
+ * (PC/csr2block.h:267 semantics, bit-exact).  B in {8, 16, 32}.  val_low may
+ * be NULL ((float)val is used, as the reference's callers pass).  The caller's
+ * arrays are not retained. */
+int mf_matrix_create(int rowA, int colA, int nnz, const int *rowptr, const int *colidx,
+                     const double *val, const float *val_low, int B, mf_matrix *out);
+void mf_matrix_destroy(mf_matrix m);
+/* Copies the reference-layout arrays to host (malloc'ed; mf_ref_tiles_free). */
+int mf_matrix_export(mf_matrix m, mf_ref_tiles *out);
+void mf_ref_tiles_free(mf_ref_tiles *t);
+/* PC/encode.h:29 over the whole Tile_csr_Col array (pairing follows the
+ * global nonzero index as PC/blockspmv_cpu.h:104 decodes it), computed on the
+ * GPU; packed has ceil(csrsize/2) bytes. */
+int mf_matrix_encode_cols(mf_matrix m, unsigned char *packed);
+
+/* Per-tile storage class, computed on the GPU.  Codes follow the reference
+ * enumeration of MP/Mille-feuille_CG.cu:257-352 (0=FP64, 4=FP32, 3=FP16,
+ * 2=FP8 E4M3).  A tile takes the narrowest enabled format whose normal range
+ * covers all its values and whose round-to-nearest error is <= tau*|v| for
+ * every value (tau = 0: lossless only).  allow_mask bit k enables code k. */
+int mf_matrix_classes(mf_matrix m, double tau, int allow_mask, unsigned char *codes);
+
+typedef struct {
+    int rowA, colA, nnz, B, tilem, tilen, tilenum;
+    long long tiles_per_code[5];  /* index = code */
+    long long nnz_per_code[5];
+    double bytes_alg_fp64;        /* algorithmic bytes of one FP64 SpMV */
+    double bytes_alg_mixed;       /* same for the last mixed layout built */
+    double bytes_layout_mixed;    /* actual bytes of the mixed device layout */
+} mf_matrix_info;
+int mf_matrix_get_info(mf_matrix m, mf_matrix_info *info);
+
+/* Precision / solver options. */
+enum { MF_CG = 0, MF_BICGSTAB = 1 };
+enum { MF_FP64 = 0, MF_MIXED = 1 };
+typedef struct {
+    int method;         /* MF_CG or MF_BICGSTAB */
+    int precision;      /* MF_FP64 or MF_MIXED */
+    double tol;         /* true relative residual target (default 1e-10) */
+    int maxit;          /* per inner solve (default 10000) */
+    double tau;         /* mixed: per-tile rounding tolerance (default 0 = lossless) */
+    int allow_mask;     /* mixed: enabled codes (default FP32|FP16|FP8) */
+    double rr_factor;   /* residual replacement when ||r|| drops by this factor
+                           since the last replacement (0: only at convergence) */
+    int ir_max_outer;   /* FP64 iterative-refinement steps (default 8) */
+    double ir_inner_tol;/* relative tolerance of each inner solve when tiles are lossy */
+    int grid;           /* workgroups (0: one per CU) */
+    int block;          /* threads per workgroup (0: auto, 512 or 1024) */
+    long long spin_limit; /* grid-barrier polls before MF_E_TIMEOUT */
+} mf_opts;
+void mf_opts_default(mf_opts *o);
+
+typedef struct {
+    int status;          /* mf_status of the solve */
+    int iterations;      /* Krylov iterations, all inner solves */
+    int outer;           /* IR steps */
+    int replacements;    /* true-residual replacements inside the kernel */
+    double relres_true;  /* ||b - A x|| / ||b||, FP64, computed on the GPU */
+    double ms_solve;     /* time to tolerance (device events, inputs resident) */
+    double ms_setup;     /* layout build */
+    int grid, block, rows_per_thread;
+} mf_report;
+
+/* Host buffers in/out (copies included; the timed ms_solve excludes them). */
+int mf_solve(mf_matrix m, const double *b, double *x, const mf_opts *o, mf_report *rep);
+/* Device pointers (n = tilem*B entries, x zero-initialised by the call),
+ * on the library stream. */
+int mf_solve_device(mf_matrix m, const double *d_b, double *d_x, const mf_opts *o,
+                    mf_report *rep);
+
+/* Tiled SpMV y = A x.  mixed: 0 = FP64 layout, 1 = mixed layout of the last
+ * mf_solve/mf_matrix_layout call.  Host arrays: x has tilen*B entries (zero
+ * padded by the call from colA), y receives tilem*B. */
+int mf_matrix_layout(mf_matrix m, double tau, int allow_mask);
+int mf_spmv(mf_matrix m, int mixed, const double *x, double *y);
+/* Times reps launches of the standalone SpMV kernel (inputs resident) and
+ * returns the mean kernel time in ms (device events). */
+int mf_spmv_bench(mf_matrix m, int mixed, int reps, double *ms);
+
+/* Batched: nsys independent square systems, each with nrhs right-hand sides
+ * (column-major: b[s][k*n_s + i]), solved by one persistent launch captured
+ * in a hipGraph.  Matrices are given as CSR.  x likewise.  Per-system
+ * iterations and true relative residuals (max over RHS) are returned. */
+int mf_solve_batched(int nsys, const int *n, const int *const *rowptr, const int *const *colidx,
+                     const double *const *val, int nrhs, const double *const *b, double *const *x,
+                     const mf_opts *o, int *iters, double *relres, mf_report *rep);
+
+/* Memory-Cost classifier (MC/memory_Mille_feillue.cu:866-918) over the built
+ * tiles: counts[0..3] = fp8, fp16, fp32, fp64 nonzeros; returns memory_MB of
+ * the values. */
+double mf_mc_classify(mf_matrix m, long long counts[4]);
+
+/* Host preprocessing of the reference schedule, exported for the CSV rows:
+ * compact row lists (PC/Mille-feuille_CG_AMD.cu:1421-1480) and the two-pointer
+ * warp partition (:1483-1735; mode 1 = AMD autotune, mode 0 = fixed
+ * each_block_nnz as PC/Mille-feuille_CG_NVIDIA.cu:4328-4420).  Arrays are
+ * malloc'ed; release with mf_free. */
+typedef struct {
+    int cnt_non_new;
+    int *nonzero_row_new;
+    unsigned char *blockrowid_new;
+    unsigned char *blockcsr_ptr_new;
+    int *block_signal;
+    int *tile_rowidx;
+} mf_compact;
+int mf_compact_rows(const mf_ref_tiles *t, mf_compact *out);
+typedef struct {
+    int index, each_block_nnz, block_per_warp, vector_each_warp_32, vector_total_32;
+    int *balance_tile_ptr, *index_each_block, *row_each_block, *non_each_block_offset,
+        *balance_tile_ptr_shared_end;
+} mf_partition;
+int mf_partition_build(const mf_ref_tiles *t, int nnzR, int block_nnz, int sm_num, int mode,
+                       mf_partition *out);
+void mf_free(void *p);
+
+/* Synthetic matrices of the benchmark configurations, as host CSR
+ * (malloc'ed; mf_free).  Rows in lexicographic order, x fastest. */
+int mf_gen_laplace2d(int nx, int ny, int *n, int **rowptr, int **colidx, double **val);
+int mf_gen_poisson3d(int nx, int ny, int nz, int *n, int **rowptr, int **colidx, double **val);
+int mf_gen_convdiff3d(int m, double peclet, double sigma, unsigned long long seed, int *n,
+                      int **rowptr, int **colidx, double **val);
+int mf_gen_random_spd(int n, double zipf_a, int lmin, int lmax, unsigned long long seed, int *nnz,
+                      int **rowptr, int **colidx, double **val);
+int mf_gen_stencil_lognormal(int dim, int n_target, double sigma, unsigned long long seed, int *n,
+                             int **rowptr, int **colidx, double **val);
+/* Portable generator (mt19937_64 with explicit transforms). */
+int mf_gen_uniform(unsigned long long seed, int count, double lo, double hi, double *out);
+int mf_gen_normal(unsigned long long seed, int count, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,395 @@
+"""ctypes binding of libmfamd.so, the MI355X Mille-feuille solver library.
+
+The library itself (C-ABI in include/mf.h) is the product: HIP kernels for
+gfx950 and C++ host code.  This module only moves numpy arrays across that
+boundary for the tests and the benchmark.  It raises if the library is
+missing or if a call returns an error -- there is no fallback path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmfamd.so")
+
+MF_CG, MF_BICGSTAB = 0, 1
+MF_FP64, MF_MIXED = 0, 1
+CODE_FP64, CODE_FP8, CODE_FP16, CODE_FP32 = 0, 2, 3, 4
+ALLOW_ALL_LOW = (1 << 2) | (1 << 3) | (1 << 4)
+
+_i = C.POINTER(C.c_int)
+_u8 = C.POINTER(C.c_ubyte)
+_d = C.POINTER(C.c_double)
+_f = C.POINTER(C.c_float)
+
+
+class MFError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"status {status}: {msg}")
+        self.status = status
+
+
+class RefTiles(C.Structure):
+    _fields_ = [
+        ("B", C.c_int), ("rowA", C.c_int), ("colA", C.c_int), ("tilem", C.c_int),
+        ("tilen", C.c_int), ("tilenum", C.c_int), ("csrsize", C.c_int), ("csrptrlen", C.c_int),
+        ("tile_ptr", _i), ("tile_columnidx", _i), ("tile_nnz", _i), ("Format", C.POINTER(C.c_char)),
+        ("blknnz", _i), ("blknnznnz", _u8), ("csr_offset", _i), ("csrptr_offset", _i),
+        ("Blockcsr_Val", _d), ("Blockcsr_Val_Low", _f), ("Blockcsr_Ptr", _u8),
+        ("Tile_csr_Col", _u8), ("csr_compressedIdx", _u8),
+    ]
+
+
+class Info(C.Structure):
+    _fields_ = [("rowA", C.c_int), ("colA", C.c_int), ("nnz", C.c_int), ("B", C.c_int),
+                ("tilem", C.c_int), ("tilen", C.c_int), ("tilenum", C.c_int),
+                ("tiles_per_code", C.c_longlong * 5), ("nnz_per_code", C.c_longlong * 5),
+                ("bytes_alg_fp64", C.c_double), ("bytes_alg_mixed", C.c_double),
+                ("bytes_layout_mixed", C.c_double)]
+
+
+class Opts(C.Structure):
+    _fields_ = [("method", C.c_int), ("precision", C.c_int), ("tol", C.c_double), ("maxit", C.c_int),
+                ("tau", C.c_double), ("allow_mask", C.c_int), ("rr_factor", C.c_double),
+                ("ir_max_outer", C.c_int), ("ir_inner_tol", C.c_double), ("grid", C.c_int),
+                ("block", C.c_int), ("spin_limit", C.c_longlong)]
+
+
+class Report(C.Structure):
+    _fields_ = [("status", C.c_int), ("iterations", C.c_int), ("outer", C.c_int),
+                ("replacements", C.c_int), ("relres_true", C.c_double), ("ms_solve", C.c_double),
+                ("ms_setup", C.c_double), ("grid", C.c_int), ("block", C.c_int),
+                ("rows_per_thread", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Compact(C.Structure):
+    _fields_ = [("cnt_non_new", C.c_int), ("nonzero_row_new", _i), ("blockrowid_new", _u8),
+                ("blockcsr_ptr_new", _u8), ("block_signal", _i), ("tile_rowidx", _i)]
+
+
+class Partition(C.Structure):
+    _fields_ = [("index", C.c_int), ("each_block_nnz", C.c_int), ("block_per_warp", C.c_int),
+                ("vector_each_warp_32", C.c_int), ("vector_total_32", C.c_int),
+                ("balance_tile_ptr", _i), ("index_each_block", _i), ("row_each_block", _i),
+                ("non_each_block_offset", _i), ("balance_tile_ptr_shared_end", _i)]
+
+
+# symbols declared in include/mf.h (checked by the CPU tests)
+EXPORTS = (
+    "mmio_info", "mmio_data", "encode", "decode", "exclusive_scan", "exclusive_scan_char",
+    "Tile_create", "Tile_destroy", "cg_solve_inc", "bicgstab", "mf_status_str", "mf_last_error",
+    "mf_init", "mf_matrix_create", "mf_matrix_destroy", "mf_matrix_export", "mf_ref_tiles_free",
+    "mf_matrix_encode_cols", "mf_matrix_classes", "mf_matrix_get_info", "mf_opts_default",
+    "mf_solve", "mf_solve_device", "mf_matrix_layout", "mf_spmv", "mf_spmv_bench",
+    "mf_solve_batched", "mf_mc_classify", "mf_compact_rows", "mf_partition_build", "mf_free",
+    "mf_gen_laplace2d", "mf_gen_poisson3d", "mf_gen_convdiff3d", "mf_gen_random_spd",
+    "mf_gen_stencil_lognormal", "mf_gen_uniform", "mf_gen_normal",
+)
+
+_L = None
+
+
+def lib():
+    """Loads libmfamd.so (raises when it was not built)."""
+    global _L
+    if _L is not None:
+        return _L
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(f"{LIB_PATH} not built (run __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    L.mf_status_str.restype = C.c_char_p
+    L.mf_last_error.restype = C.c_char_p
+    L.mf_matrix_create.argtypes = [C.c_int, C.c_int, C.c_int, _i, _i, _d, _f, C.c_int, C.POINTER(C.c_void_p)]
+    L.mf_matrix_destroy.argtypes = [C.c_void_p]
+    L.mf_matrix_export.argtypes = [C.c_void_p, C.POINTER(RefTiles)]
+    L.mf_ref_tiles_free.argtypes = [C.POINTER(RefTiles)]
+    L.mf_matrix_encode_cols.argtypes = [C.c_void_p, _u8]
+    L.mf_matrix_classes.argtypes = [C.c_void_p, C.c_double, C.c_int, _u8]
+    L.mf_matrix_get_info.argtypes = [C.c_void_p, C.POINTER(Info)]
+    L.mf_matrix_layout.argtypes = [C.c_void_p, C.c_double, C.c_int]
+    L.mf_spmv.argtypes = [C.c_void_p, C.c_int, _d, _d]
+    L.mf_spmv_bench.argtypes = [C.c_void_p, C.c_int, C.c_int, _d]
+    L.mf_opts_default.argtypes = [C.POINTER(Opts)]
+    L.mf_solve.argtypes = [C.c_void_p, _d, _d, C.POINTER(Opts), C.POINTER(Report)]
+    L.mf_mc_classify.argtypes = [C.c_void_p, C.POINTER(C.c_longlong)]
+    L.mf_mc_classify.restype = C.c_double
+    L.mf_compact_rows.argtypes = [C.POINTER(RefTiles), C.POINTER(Compact)]
+    L.mf_partition_build.argtypes = [C.POINTER(RefTiles), C.c_int, C.c_int, C.c_int, C.c_int,
+                                     C.POINTER(Partition)]
+    L.mf_free.argtypes = [C.c_void_p]
+    L.mmio_info.argtypes = [_i, _i, _i, _i, C.c_char_p]
+    L.mmio_data.argtypes = [_i, _i, _d, C.c_char_p]
+    L.encode.argtypes = [_u8, _u8, C.c_int, C.c_int]
+    L.decode.argtypes = [_u8, _u8, C.c_int]
+    L.exclusive_scan.argtypes = [_i, C.c_int]
+    L.exclusive_scan_char.argtypes = [_u8, C.c_int]
+    L.mf_gen_laplace2d.argtypes = [C.c_int, C.c_int, _i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_d)]
+    L.mf_gen_poisson3d.argtypes = [C.c_int, C.c_int, C.c_int, _i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_d)]
+    L.mf_gen_convdiff3d.argtypes = [C.c_int, C.c_double, C.c_double, C.c_ulonglong, _i, C.POINTER(_i),
+                                    C.POINTER(_i), C.POINTER(_d)]
+    L.mf_gen_random_spd.argtypes = [C.c_int, C.c_double, C.c_int, C.c_int, C.c_ulonglong, _i,
+                                    C.POINTER(_i), C.POINTER(_i), C.POINTER(_d)]
+    L.mf_gen_stencil_lognormal.argtypes = [C.c_int, C.c_int, C.c_double, C.c_ulonglong, _i, C.POINTER(_i),
+                                           C.POINTER(_i), C.POINTER(_d)]
+    L.mf_gen_uniform.argtypes = [C.c_ulonglong, C.c_int, C.c_double, C.c_double, _d]
+    L.mf_gen_normal.argtypes = [C.c_ulonglong, C.c_int, _d]
+    L.mf_solve_batched.argtypes = [C.c_int, _i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_d), C.c_int,
+                                   C.POINTER(_d), C.POINTER(_d), C.POINTER(Opts), _i, _d, C.POINTER(Report)]
+    _L = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        L = lib()
+        raise MFError(rc, f"{L.mf_status_str(rc).decode()}: {L.mf_last_error().decode()}")
+
+
+def _p(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def _copy(ptr, n, dtype):
+    if n <= 0:
+        return np.zeros(0, dtype=dtype)
+    return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
+
+
+def ref_tiles_to_dict(t):
+    n1, m1 = t.tilenum + 1, t.tilem + 1
+    csz = t.csrsize // 2 + (t.csrsize % 2)
+    fmt = np.ctypeslib.as_array(C.cast(t.Format, _u8), shape=(max(t.tilenum, 1),))[: t.tilenum]
+    return {
+        "tilem": t.tilem, "tilen": t.tilen, "tilenum": t.tilenum, "csrsize": t.csrsize,
+        "csrptrlen": t.csrptrlen,
+        "tile_ptr": _copy(t.tile_ptr, m1, np.int32),
+        "tile_columnidx": _copy(t.tile_columnidx, t.tilenum, np.int32),
+        "tile_nnz": _copy(t.tile_nnz, n1, np.int32),
+        "Format": fmt.astype(np.uint8),
+        "blknnz": _copy(t.blknnz, n1, np.int32),
+        "blknnznnz": _copy(t.blknnznnz, n1, np.uint8),
+        "csr_offset": _copy(t.csr_offset, n1, np.int32),
+        "csrptr_offset": _copy(t.csrptr_offset, n1, np.int32),
+        "Blockcsr_Val": _copy(t.Blockcsr_Val, t.csrsize, np.float64),
+        "Blockcsr_Val_Low": _copy(t.Blockcsr_Val_Low, t.csrsize, np.float32),
+        "Blockcsr_Ptr": _copy(t.Blockcsr_Ptr, t.csrptrlen, np.uint8),
+        "Tile_csr_Col": _copy(t.Tile_csr_Col, t.csrsize, np.uint8),
+        "csr_compressedIdx": _copy(t.csr_compressedIdx, csz, np.uint8),
+    }
+
+
+def default_opts(**kw):
+    o = Opts()
+    lib().mf_opts_default(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class Matrix:
+    """A device-resident matrix; its reference tile layout is built on the GPU."""
+
+    def __init__(self, rowptr, col, val, rowA=None, colA=None, B=16, val_low=None):
+        L = lib()
+        self.rowptr = np.ascontiguousarray(rowptr, np.int32)
+        self.col = np.ascontiguousarray(col, np.int32)
+        self.val = np.ascontiguousarray(val, np.float64)
+        self.rowA = len(self.rowptr) - 1 if rowA is None else int(rowA)
+        self.colA = self.rowA if colA is None else int(colA)
+        low = None if val_low is None else np.ascontiguousarray(val_low, np.float32)
+        h = C.c_void_p()
+        _check(L.mf_matrix_create(self.rowA, self.colA, len(self.col), _p(self.rowptr, C.c_int),
+                                  _p(self.col, C.c_int), _p(self.val, C.c_double),
+                                  _p(low, C.c_float) if low is not None else None, B, C.byref(h)))
+        self.h = h
+        self.B = B
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mf_matrix_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def export(self):
+        t = RefTiles()
+        _check(lib().mf_matrix_export(self.h, C.byref(t)))
+        d = ref_tiles_to_dict(t)
+        self._last_export = t
+        return d
+
+    def export_struct(self):
+        t = RefTiles()
+        _check(lib().mf_matrix_export(self.h, C.byref(t)))
+        return t
+
+    def encode_cols(self):
+        info = self.info()
+        out = np.zeros((info["nnz"] + 1) // 2 + 1, np.uint8)
+        _check(lib().mf_matrix_encode_cols(self.h, _p(out, C.c_ubyte)))
+        return out[: (info["nnz"] + 1) // 2]
+
+    def classes(self, tau=0.0, allow_mask=ALLOW_ALL_LOW):
+        info = self.info()
+        out = np.zeros(max(info["tilenum"], 1), np.uint8)
+        _check(lib().mf_matrix_classes(self.h, tau, allow_mask, _p(out, C.c_ubyte)))
+        return out[: info["tilenum"]]
+
+    def info(self):
+        i = Info()
+        _check(lib().mf_matrix_get_info(self.h, C.byref(i)))
+        d = {k: getattr(i, k) for k, _ in Info._fields_}
+        d["tiles_per_code"] = list(i.tiles_per_code)
+        d["nnz_per_code"] = list(i.nnz_per_code)
+        return d
+
+    def layout(self, tau=0.0, allow_mask=ALLOW_ALL_LOW):
+        _check(lib().mf_matrix_layout(self.h, tau, allow_mask))
+
+    def spmv(self, x, mixed=False):
+        info = self.info()
+        y = np.zeros(info["tilem"] * self.B, np.float64)
+        xx = np.ascontiguousarray(x, np.float64)
+        assert len(xx) >= self.colA
+        _check(lib().mf_spmv(self.h, 1 if mixed else 0, _p(xx, C.c_double), _p(y, C.c_double)))
+        return y[: self.rowA]
+
+    def spmv_bench(self, reps=20, mixed=False):
+        ms = C.c_double()
+        _check(lib().mf_spmv_bench(self.h, 1 if mixed else 0, reps, C.byref(ms)))
+        return ms.value
+
+    def solve(self, b, **opts):
+        o = default_opts(**opts)
+        rep = Report()
+        bb = np.ascontiguousarray(b, np.float64)
+        x = np.zeros(self.rowA, np.float64)
+        _check(lib().mf_solve(self.h, _p(bb, C.c_double), _p(x, C.c_double), C.byref(o), C.byref(rep)))
+        return x, rep.as_dict()
+
+    def mc_classify(self):
+        cnt = (C.c_longlong * 4)()
+        mb = lib().mf_mc_classify(self.h, cnt)
+        return mb, [cnt[i] for i in range(4)]
+
+
+def _take_csr(n, rp, ci, va, nnz=None):
+    nrows = n
+    m = _copy(rp, nrows + 1, np.int32)
+    nz = int(m[-1])
+    out = (m, _copy(ci, nz, np.int32), _copy(va, nz, np.float64))
+    for q in (rp, ci, va):
+        lib().mf_free(C.cast(q, C.c_void_p))
+    return out
+
+
+def gen_laplace2d(nx, ny):
+    n, rp, ci, va = C.c_int(), _i(), _i(), _d()
+    _check(lib().mf_gen_laplace2d(nx, ny, C.byref(n), C.byref(rp), C.byref(ci), C.byref(va)))
+    return _take_csr(n.value, rp, ci, va)
+
+
+def gen_poisson3d(nx, ny, nz):
+    n, rp, ci, va = C.c_int(), _i(), _i(), _d()
+    _check(lib().mf_gen_poisson3d(nx, ny, nz, C.byref(n), C.byref(rp), C.byref(ci), C.byref(va)))
+    return _take_csr(n.value, rp, ci, va)
+
+
+def gen_convdiff3d(m, peclet=10.0, sigma=1.0, seed=0):
+    n, rp, ci, va = C.c_int(), _i(), _i(), _d()
+    _check(lib().mf_gen_convdiff3d(m, peclet, sigma, seed, C.byref(n), C.byref(rp), C.byref(ci), C.byref(va)))
+    return _take_csr(n.value, rp, ci, va)
+
+
+def gen_random_spd(n, zipf_a=2.0, lmin=2, lmax=2000, seed=0):
+    nnz, rp, ci, va = C.c_int(), _i(), _i(), _d()
+    _check(lib().mf_gen_random_spd(n, zipf_a, lmin, lmax, seed, C.byref(nnz), C.byref(rp), C.byref(ci),
+                                   C.byref(va)))
+    return _take_csr(n, rp, ci, va)
+
+
+def gen_stencil_lognormal(dim, n_target, sigma=1.5, seed=0):
+    n, rp, ci, va = C.c_int(), _i(), _i(), _d()
+    _check(lib().mf_gen_stencil_lognormal(dim, n_target, sigma, seed, C.byref(n), C.byref(rp), C.byref(ci),
+                                          C.byref(va)))
+    return _take_csr(n.value, rp, ci, va)
+
+
+def gen_uniform(seed, count, lo, hi):
+    out = np.zeros(count, np.float64)
+    _check(lib().mf_gen_uniform(seed, count, lo, hi, _p(out, C.c_double)))
+    return out
+
+
+def gen_normal(seed, count):
+    out = np.zeros(count, np.float64)
+    _check(lib().mf_gen_normal(seed, count, _p(out, C.c_double)))
+    return out
+
+
+def mmio(path):
+    L = lib()
+    m, n, nnz, sym = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    rc = L.mmio_info(C.byref(m), C.byref(n), C.byref(nnz), C.byref(sym), path.encode())
+    if rc:
+        return rc, None
+    rp = np.zeros(m.value + 1, np.int32)
+    ci = np.zeros(max(nnz.value, 1), np.int32)
+    va = np.zeros(max(nnz.value, 1), np.float64)
+    rc = L.mmio_data(_p(rp, C.c_int), _p(ci, C.c_int), _p(va, C.c_double), path.encode())
+    return rc, dict(m=m.value, n=n.value, nnz=nnz.value, sym=sym.value, rowptr=rp, col=ci[: nnz.value],
+                    val=va[: nnz.value])
+
+
+def compact_rows(ref_struct):
+    c = Compact()
+    _check(lib().mf_compact_rows(C.byref(ref_struct), C.byref(c)))
+    cnt = c.cnt_non_new
+    out = dict(cnt_non_new=cnt,
+               nonzero_row_new=_copy(c.nonzero_row_new, ref_struct.tilenum + 1, np.int32),
+               blockrowid_new=_copy(c.blockrowid_new, cnt + 1, np.uint8),
+               blockcsr_ptr_new=_copy(c.blockcsr_ptr_new, cnt + 1, np.uint8),
+               block_signal=_copy(c.block_signal, ref_struct.tilem + 1, np.int32),
+               tile_rowidx=_copy(c.tile_rowidx, ref_struct.tilenum, np.int32))
+    for q in (c.nonzero_row_new, c.blockrowid_new, c.blockcsr_ptr_new, c.block_signal, c.tile_rowidx):
+        lib().mf_free(C.cast(q, C.c_void_p))
+    return out
+
+
+def partition(ref_struct, nnzR, block_nnz, sm_num, mode):
+    p = Partition()
+    _check(lib().mf_partition_build(C.byref(ref_struct), nnzR, block_nnz, sm_num, mode, C.byref(p)))
+    n1 = ref_struct.tilenum + 1
+    out = dict(index=p.index, each_block_nnz=p.each_block_nnz, block_per_warp=p.block_per_warp,
+               vector_each_warp_32=p.vector_each_warp_32, vector_total_32=p.vector_total_32,
+               balance_tile_ptr=_copy(p.balance_tile_ptr, p.index + 1, np.int32),
+               index_each_block=_copy(p.index_each_block, n1, np.int32),
+               row_each_block=_copy(p.row_each_block, n1, np.int32),
+               non_each_block_offset=_copy(p.non_each_block_offset, n1, np.int32),
+               balance_tile_ptr_shared_end=_copy(p.balance_tile_ptr_shared_end, p.index + 1, np.int32))
+    for q in (p.balance_tile_ptr, p.index_each_block, p.row_each_block, p.non_each_block_offset,
+              p.balance_tile_ptr_shared_end):
+        lib().mf_free(C.cast(q, C.c_void_p))
+    return out
+
+
+def encode(cols, offset=0):
+    cols = np.ascontiguousarray(cols, np.uint8)
+    res = np.zeros(len(cols) // 2 + 1 + offset, np.uint8)
+    lib().encode(_p(cols, C.c_ubyte), _p(res, C.c_ubyte), len(cols), offset)
+    return res[: offset + (len(cols) + 1) // 2]
+
+
+def decode(packed, n):
+    packed = np.ascontiguousarray(packed, np.uint8)
+    res = np.zeros(n, np.uint8)
+    lib().decode(_p(packed, C.c_ubyte), _p(res, C.c_ubyte), n)
+    return res

@@ -1,0 +1,736 @@
+// Host side of libmfamd.so: Matrix Market input, the reference schedule's host
+// preprocessing (for the drop-in CSV rows), nibble codec and scans, CSV
+// writers and the seeded generators of the benchmark matrices.
+#include <algorithm>
+#include <cctype>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include <omp.h>
+#include <sys/stat.h>
+
+#include "mf_internal.h"
+
+namespace mf {
+
+static thread_local std::string g_err;
+
+int fail(int status, const std::string &msg)
+{
+    g_err = msg;
+    return status;
+}
+void clear_error() { g_err.clear(); }
+
+double now_ms()
+{
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+// ---------------------------------------------------------------- Matrix Market
+// Semantics of PC/mmio_highlevel.h:17-593 with the banner rules of
+// PC/mmio.h:398-590: the data type picks how an entry line is read (real /
+// integer -> value, pattern -> 1.0, complex -> real part), indices become
+// 0-based, and symmetric or hermitian files are expanded when the CSR is built.
+// The entry lines are parsed in parallel chunks split at line boundaries; the
+// order of entries is the file order.
+static bool read_file(const char *path, std::vector<char> &buf)
+{
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return false;
+    std::fseek(f, 0, SEEK_END);
+    long sz = std::ftell(f);
+    std::fseek(f, 0, SEEK_SET);
+    buf.resize((size_t)std::max(sz, 0L) + 1);
+    size_t got = sz > 0 ? std::fread(buf.data(), 1, (size_t)sz, f) : 0;
+    buf[got] = '\0';
+    buf.resize(got + 1);
+    std::fclose(f);
+    return true;
+}
+
+static const char *next_line(const char *p, const char *end)
+{
+    while (p < end && *p != '\n') ++p;
+    return p < end ? p + 1 : end;
+}
+
+int mm_parse(const char *path, MMData *out)
+{
+    *out = MMData();
+    std::vector<char> buf;
+    if (!read_file(path, buf)) return fail(MF_E_IO, std::string("cannot open ") + path);
+    const char *p = buf.data(), *end = buf.data() + buf.size() - 1;
+    // banner: 5 tokens, "%%MatrixMarket matrix coordinate|array <type> <storage>"
+    std::string line(p, next_line(p, end) - p);
+    char t[5][64] = {{0}};
+    if (std::sscanf(line.c_str(), "%63s %63s %63s %63s %63s", t[0], t[1], t[2], t[3], t[4]) != 5)
+        return fail(MF_E_BANNER, "Could not process Matrix Market banner");
+    for (int k = 1; k < 5; k++)
+        for (char *c = t[k]; *c; ++c) *c = (char)std::tolower((unsigned char)*c);
+    if (std::strncmp(t[0], "%%MatrixMarket", 14) || std::strcmp(t[1], "matrix") ||
+        (std::strcmp(t[2], "coordinate") && std::strcmp(t[2], "array")))
+        return fail(MF_E_BANNER, "Could not process Matrix Market banner");
+    int kind;  // 0 real, 1 complex, 2 pattern, 3 integer
+    if (!std::strcmp(t[3], "real")) kind = 0;
+    else if (!std::strcmp(t[3], "complex")) kind = 1;
+    else if (!std::strcmp(t[3], "pattern")) kind = 2;
+    else if (!std::strcmp(t[3], "integer")) kind = 3;
+    else return fail(MF_E_BANNER, "unsupported Matrix Market data type");
+    bool sym = !std::strcmp(t[4], "symmetric") || !std::strcmp(t[4], "hermitian");
+    if (!sym && std::strcmp(t[4], "general") && std::strcmp(t[4], "skew-symmetric"))
+        return fail(MF_E_BANNER, "unsupported Matrix Market storage scheme");
+    p = next_line(p, end);
+    while (p < end && *p == '%') p = next_line(p, end);
+    int m = 0, n = 0, nz = 0;
+    {
+        char *q;
+        long a = std::strtol(p, &q, 10), b = std::strtol(q, &q, 10), c = std::strtol(q, &q, 10);
+        if (q == p) return fail(MF_E_SIZE, "bad size line");
+        m = (int)a; n = (int)b; nz = (int)c;
+        p = next_line(q, end);
+    }
+    if (m < 0 || n < 0 || nz < 0) return fail(MF_E_SIZE, "negative size");
+    // split the entry region into chunks at line starts
+    int nth = std::max(1, std::min(omp_get_max_threads(), 64));
+    std::vector<const char *> cut(nth + 1);
+    cut[0] = p;
+    cut[nth] = end;
+    for (int k = 1; k < nth; k++) {
+        const char *q = p + (end - p) * (long long)k / nth;
+        if (q < cut[k - 1]) q = cut[k - 1];
+        if (q > p && q < end && q[-1] != '\n') q = next_line(q, end);
+        cut[k] = q;
+    }
+    // count data lines per chunk (non-empty lines)
+    std::vector<long long> cnt(nth + 1, 0);
+#pragma omp parallel for num_threads(nth) schedule(static, 1)
+    for (int k = 0; k < nth; k++) {
+        long long c = 0;
+        for (const char *q = cut[k]; q < cut[k + 1];) {
+            const char *s = q;
+            while (s < cut[k + 1] && (*s == ' ' || *s == '\t' || *s == '\r')) ++s;
+            if (s < cut[k + 1] && *s != '\n' && *s != '%') c++;
+            q = next_line(q, cut[k + 1]);
+        }
+        cnt[k + 1] = c;
+    }
+    for (int k = 0; k < nth; k++) cnt[k + 1] += cnt[k];
+    if (cnt[nth] < nz) return fail(MF_E_SIZE, "fewer entries than the size line reports");
+    out->m = m; out->n = n; out->nz_report = nz; out->is_sym = sym;
+    out->ri = (int *)std::malloc(sizeof(int) * (size_t)std::max(nz, 1));
+    out->ci = (int *)std::malloc(sizeof(int) * (size_t)std::max(nz, 1));
+    out->vv = (double *)std::malloc(sizeof(double) * (size_t)std::max(nz, 1));
+    if (!out->ri || !out->ci || !out->vv) { mm_free(out); return fail(MF_E_NOMEM, "mm_parse"); }
+    int bad = 0;
+#pragma omp parallel for num_threads(nth) schedule(static, 1) reduction(| : bad)
+    for (int k = 0; k < nth; k++) {
+        long long e = cnt[k];
+        for (const char *q = cut[k]; q < cut[k + 1] && e < nz;) {
+            const char *s = q;
+            while (s < cut[k + 1] && (*s == ' ' || *s == '\t' || *s == '\r')) ++s;
+            if (s < cut[k + 1] && *s != '\n' && *s != '%') {
+                char *r;
+                long a = std::strtol(s, &r, 10);
+                long b = std::strtol(r, &r, 10);
+                double v = 1.0;
+                if (kind == 0 || kind == 1) v = std::strtod(r, &r);  // %lg (complex: real part)
+                else if (kind == 3) v = (double)(int)std::strtol(r, &r, 10);
+                if (a < 1 || a > m || b < 1 || b > n) bad = 1;
+                out->ri[e] = (int)a - 1;
+                out->ci[e] = (int)b - 1;
+                out->vv[e] = v;
+                e++;
+            }
+            q = next_line(q, cut[k + 1]);
+        }
+    }
+    if (bad) { mm_free(out); return fail(MF_E_SIZE, "entry index outside the matrix"); }
+    return MF_OK;
+}
+
+void mm_free(MMData *d)
+{
+    std::free(d->ri); std::free(d->ci); std::free(d->vv);
+    d->ri = d->ci = nullptr;
+    d->vv = nullptr;
+}
+
+long long mm_nnz_expanded(const MMData &d)
+{
+    long long nnz = d.nz_report;
+    if (d.is_sym)
+        for (int i = 0; i < d.nz_report; i++) nnz += d.ri[i] != d.ci[i];
+    return nnz;
+}
+
+int mm_to_csr(const MMData &d, int *rowptr, int *colidx, double *val)
+{
+    std::vector<int> fill((size_t)d.m + 1, 0);
+    for (int i = 0; i <= d.m; i++) rowptr[i] = 0;
+    for (int i = 0; i < d.nz_report; i++) {
+        rowptr[d.ri[i] + 1]++;
+        if (d.is_sym && d.ri[i] != d.ci[i]) rowptr[d.ci[i] + 1]++;
+    }
+    for (int i = 0; i < d.m; i++) rowptr[i + 1] += rowptr[i];
+    // file order; a mirrored entry lands in row i first, then in row j
+    for (int i = 0; i < d.nz_report; i++) {
+        int r = d.ri[i], c = d.ci[i];
+        int o = rowptr[r] + fill[r]++;
+        colidx[o] = c;
+        val[o] = d.vv[i];
+        if (d.is_sym && r != c) {
+            o = rowptr[c] + fill[c]++;
+            colidx[o] = r;
+            val[o] = d.vv[i];
+        }
+    }
+    return MF_OK;
+}
+
+// ---------------------------------------------------------------------- CSV
+static int csv_write(const char *path, const char *matrix, const char *row)
+{
+    FILE *f = std::fopen(path, "a");
+    if (!f) return fail(MF_E_IO, "open error!");
+    std::fputs(matrix, f);
+    std::fputs(",", f);
+    std::fputs(row, f);
+    std::fclose(f);
+    return MF_OK;
+}
+
+// Row format of PC/Mille-feuille_CG_AMD.cu:2021.
+int csv_append_cg(const char *path, const char *matrix, int iterations, double time_spmv,
+                  int nnzR, double l2_norm, double time_format, int index, int each_block_nnz,
+                  double norm)
+{
+    char s[256];
+    std::snprintf(s, sizeof s, "%d,%.3f,%.3f,%d,%e,%lf,%d,%d,%e\n", iterations, time_spmv,
+                  time_spmv, nnzR, l2_norm, time_format, index, each_block_nnz, norm);
+    return csv_write(path, matrix, s);
+}
+
+// Row format of PC/Mille-feuille_BiCGSTAB_AMD.cu:1606.
+int csv_append_bicg(const char *path, const char *matrix, int itr, double time_spmv, int nnzR,
+                    double l2_norm, double resid_ratio, int each_block_nnz)
+{
+    char s[256];
+    std::snprintf(s, sizeof s, "%d,%.3f,%.3f,%d,%e,%e,%d,%d\n", itr, time_spmv, time_spmv, nnzR,
+                  l2_norm, resid_ratio, each_block_nnz, 1);
+    return csv_write(path, matrix, s);
+}
+
+}  // namespace mf
+
+using namespace mf;
+
+// =================================================================== C-ABI
+extern "C" {
+
+const char *mf_last_error(void) { return g_err.c_str(); }
+
+const char *mf_status_str(int s)
+{
+    switch (s) {
+    case MF_OK: return "ok";
+    case MF_E_IO: return "i/o error";
+    case MF_E_BANNER: return "bad Matrix Market banner";
+    case MF_E_SIZE: return "bad Matrix Market size line or entries";
+    case MF_E_ARG: return "invalid argument";
+    case MF_E_HIP: return "HIP runtime error";
+    case MF_E_NOMEM: return "out of memory";
+    case MF_E_FORMAT: return "matrix not representable in the device layout";
+    case MF_E_NODEV: return "no usable gfx950 device";
+    case MF_E_UNSUPPORTED: return "unsupported configuration";
+    case MF_E_BREAKDOWN: return "Krylov breakdown";
+    case MF_E_MAXIT: return "iteration limit reached";
+    case MF_E_TIMEOUT: return "grid barrier timeout";
+    case MF_E_NONFINITE: return "non-finite reduction";
+    default: return "unknown status";
+    }
+}
+
+void mf_free(void *p) { std::free(p); }
+
+// PC/mmio_highlevel.h:17
+int mmio_info(int *m, int *n, int *nnz, int *isSymmetric, char *filename)
+{
+    MMData d;
+    int rc = mm_parse(filename, &d);
+    if (rc) return rc;
+    *m = d.m;
+    *n = d.n;
+    *nnz = (int)mm_nnz_expanded(d);
+    *isSymmetric = d.is_sym ? 1 : 0;
+    mm_free(&d);
+    return 0;
+}
+
+// PC/mmio_highlevel.h:275
+int mmio_data(int *csrRowPtr, int *csrColIdx, double *csrVal, char *filename)
+{
+    MMData d;
+    int rc = mm_parse(filename, &d);
+    if (rc) return rc;
+    mm_to_csr(d, csrRowPtr, csrColIdx, csrVal);
+    mm_free(&d);
+    return 0;
+}
+
+// PC/encode.h:29 -- pairs (c[2k], c[2k+1]) into one byte, high nibble first.
+void encode(unsigned char *str, unsigned char *res, int str_len, int offset)
+{
+    int k = 0;
+    for (; k + 1 < str_len; k += 2) res[offset + k / 2] = (unsigned char)((str[k] << 4) + str[k + 1]);
+    if (str_len & 1) res[offset + str_len / 2] = (unsigned char)(str[str_len - 1] << 4);
+}
+
+// PC/encode.h:52
+void decode(unsigned char *str, unsigned char *res, int res_len)
+{
+    for (int k = 0; k + 1 < res_len; k += 2) {
+        res[k] = (unsigned char)(str[k / 2] >> 4);
+        res[k + 1] = (unsigned char)(str[k / 2] & 15);
+    }
+    if (res_len & 1) res[res_len - 1] = (unsigned char)(str[res_len / 2] >> 4);
+}
+
+// PC/utils.h:34
+void exclusive_scan(int *input, int length)
+{
+    if (length < 2) return;
+    int acc = 0;
+    for (int i = 0; i < length; i++) {
+        int v = input[i];
+        input[i] = acc;
+        acc += v;
+    }
+}
+
+// PC/utils.h:50 (mod 256)
+void exclusive_scan_char(unsigned char *input, int length)
+{
+    if (length < 2) return;
+    unsigned acc = 0;
+    for (int i = 0; i < length; i++) {
+        unsigned v = input[i];
+        input[i] = (unsigned char)acc;
+        acc = (acc + v) & 255u;
+    }
+}
+
+// PC/Mille-feuille_CG_AMD.cu:1421-1480: per tile one sentinel (row 0, stop 0)
+// and one (row, stop) pair per non-empty row; stops are kept as uint8.
+int mf_compact_rows(const mf_ref_tiles *t, mf_compact *c)
+{
+    if (!t || !c) return fail(MF_E_ARG, "mf_compact_rows: null");
+    std::memset(c, 0, sizeof(*c));
+    const int B = t->B, tilem = t->tilem, tilenum = t->tilenum;
+    c->nonzero_row_new = (int *)std::calloc((size_t)tilenum + 1, sizeof(int));
+    c->tile_rowidx = (int *)std::calloc((size_t)tilenum + 1, sizeof(int));
+    c->block_signal = (int *)std::calloc((size_t)tilem + 1, sizeof(int));
+    auto stop_of = [&](int tid, int ri, int rowlen) {
+        int base = t->csrptr_offset[tid];
+        return ri == rowlen - 1 ? t->blknnz[tid + 1] - t->blknnz[tid] : (int)t->Blockcsr_Ptr[base + ri + 1];
+    };
+    for (int bi = 0; bi < tilem; bi++) {
+        int rowlen = bi == tilem - 1 ? t->rowA - (tilem - 1) * B : B;
+        c->block_signal[bi] = t->tile_ptr[bi + 1] - t->tile_ptr[bi];
+        for (int tid = t->tile_ptr[bi]; tid < t->tile_ptr[bi + 1]; tid++) {
+            c->tile_rowidx[tid] = bi;
+            int nz = 1;
+            for (int ri = 0; ri < rowlen; ri++)
+                nz += stop_of(tid, ri, rowlen) != t->Blockcsr_Ptr[t->csrptr_offset[tid] + ri];
+            c->nonzero_row_new[tid] = nz;
+        }
+    }
+    exclusive_scan(c->nonzero_row_new, tilenum + 1);
+    int cnt = c->cnt_non_new = c->nonzero_row_new[tilenum];
+    c->blockrowid_new = (unsigned char *)std::calloc((size_t)cnt + 1, 1);
+    c->blockcsr_ptr_new = (unsigned char *)std::calloc((size_t)cnt + 1, 1);
+    for (int bi = 0; bi < tilem; bi++) {
+        int rowlen = bi == tilem - 1 ? t->rowA - (tilem - 1) * B : B;
+        for (int tid = t->tile_ptr[bi]; tid < t->tile_ptr[bi + 1]; tid++) {
+            int o = c->nonzero_row_new[tid];
+            c->blockrowid_new[o] = 0;
+            c->blockcsr_ptr_new[o] = 0;
+            ++o;
+            for (int ri = 0; ri < rowlen; ri++) {
+                int stop = stop_of(tid, ri, rowlen);
+                if (stop != t->Blockcsr_Ptr[t->csrptr_offset[tid] + ri]) {
+                    c->blockrowid_new[o] = (unsigned char)ri;
+                    c->blockcsr_ptr_new[o] = (unsigned char)stop;
+                    ++o;
+                }
+            }
+        }
+    }
+    return MF_OK;
+}
+
+// Two-pointer greedy over the tiles in natural order
+// (PC/Mille-feuille_CG_AMD.cu:1556-1590): each half-step takes the next tile
+// from the front (i) or the back (j); a group closes when the tile would bring
+// it to each_block_nnz nonzeros or to block_per_warp tiles.
+static int groups_of(const std::vector<int> &non, int ebn, int bpw, std::vector<int> *order,
+                     std::vector<int> *bounds)
+{
+    int n = (int)non.size();
+    int i = 0, j = n - 1, cnt = 0, groups = 1, taken = 0, step = 0;
+    auto take = [&](int q, bool close) {
+        if (order) (*order)[step] = q;
+        step++;
+        if (close) {
+            if (bounds) bounds->push_back(step);
+            groups++;
+            cnt = 0;
+            taken = 0;
+        } else {
+            cnt += non[q];
+            taken++;
+        }
+    };
+    while (i < j) {
+        bool fits = non[i] + cnt < ebn && taken + 1 < bpw;
+        take(i++, !fits);
+        fits = non[j] + cnt < ebn && taken + 1 < bpw;
+        take(j--, !fits);
+        if (i == j) {
+            if (order) (*order)[step] = j;
+            step++;
+            if (bounds) bounds->push_back(step);
+        } else if (i > j) {
+            if (order) (*order)[step] = j;
+            if (bounds) bounds->push_back(step);
+        }
+    }
+    return groups;
+}
+
+int mf_partition_build(const mf_ref_tiles *t, int nnzR, int block_nnz, int sm_num, int mode,
+                       mf_partition *p)
+{
+    if (!t || !p) return fail(MF_E_ARG, "mf_partition_build: null");
+    std::memset(p, 0, sizeof(*p));
+    const int tilenum = t->tilenum, tilem = t->tilem;
+    std::vector<int> non(tilenum);
+    for (int k = 0; k < tilenum; k++) non[k] = t->blknnz[k + 1] - t->blknnz[k];
+    int bpw = 180, ebn = block_nnz, index = 1;
+    if (mode == 1) {
+        // AMD autotune list (PC/Mille-feuille_CG_AMD.cu:1548); its 12th entry is
+        // read past the 11 initialisers there and is taken as 0 here.
+        const int list[13] = {16, 32, 64, 96, 128, 256, 512, 1024, 2048, 4096,
+                              sm_num ? nnzR / sm_num : 0, 0, 0};
+        bool done = false;
+        while (!done && bpw <= (1 << 28)) {
+            for (int k = 1; k <= 12 && !done; k++) {
+                ebn = list[k];
+                index = groups_of(non, ebn, bpw, nullptr, nullptr);
+                done = index < sm_num && index != tilem - 1 && index != tilem && index != tilem + 1;
+            }
+            if (!done) bpw *= 2;
+        }
+    } else {
+        index = groups_of(non, ebn, bpw, nullptr, nullptr);
+    }
+    p->index = index;
+    p->each_block_nnz = ebn;
+    p->block_per_warp = bpw;
+    if (index > 0 && index < tilem) {
+        int vew16 = (int)std::ceil((double)tilem / index);
+        p->vector_each_warp_32 = 2 * vew16;
+        int vt = ((tilem + 1) / 2) / p->vector_each_warp_32;
+        p->vector_total_32 = (vt / 8 + 1) * 8;
+    }
+    std::vector<int> order((size_t)tilenum + 1, -1), bounds;
+    bounds.push_back(0);
+    groups_of(non, ebn, bpw, &order, &bounds);
+    p->balance_tile_ptr = (int *)std::calloc((size_t)index + 1, sizeof(int));
+    for (int g = 0; g <= index && g < (int)bounds.size(); g++) p->balance_tile_ptr[g] = bounds[g];
+    p->index_each_block = (int *)std::calloc((size_t)tilenum + 1, sizeof(int));
+    p->row_each_block = (int *)std::calloc((size_t)tilenum + 1, sizeof(int));
+    std::vector<int> tile_row((size_t)tilenum + 1, 0), non_new((size_t)tilenum + 1, 0);
+    for (int bi = 0; bi < tilem; bi++)
+        for (int k = t->tile_ptr[bi]; k < t->tile_ptr[bi + 1]; k++) tile_row[k] = bi;
+    // slots the greedy never wrote stay 0, as in the reference's memset arrays
+    for (int s = 0; s <= tilenum; s++) {
+        int q = order[s];
+        if (q < 0) continue;
+        p->index_each_block[s] = q;
+        p->row_each_block[s] = tile_row[q];
+        non_new[s] = non[q];
+    }
+    // prefix of each group's first 256 nonzeros (PC/Mille-feuille_CG_AMD.cu:1701-1735)
+    p->non_each_block_offset = (int *)std::calloc((size_t)tilenum + 1, sizeof(int));
+    p->balance_tile_ptr_shared_end = (int *)std::calloc((size_t)index + 1, sizeof(int));
+    int *bt = p->balance_tile_ptr, *off = p->non_each_block_offset, *se = p->balance_tile_ptr_shared_end;
+    for (int g = 0; g <= index; g++) se[g] = bt[g];
+    for (int g = 0; g < index; g++) {
+        int acc = 0;
+        for (int q = bt[g]; q < bt[g + 1]; q++) {
+            acc += non_new[q];
+            if (q == bt[g]) off[q] = 0;
+            else if (acc <= 256) off[q] = off[q - 1] + non_new[q - 1];
+            if (acc > 256) { se[g + 1] = q; break; }
+        }
+    }
+    return MF_OK;
+}
+
+// ---------------------------------------------------------------- generators
+// Portable stream: mt19937_64, u = (next >> 11) * 2^-53, normal by Box-Muller.
+struct Rng {
+    std::mt19937_64 g;
+    explicit Rng(unsigned long long s) : g(s) {}
+    double uniform() { return (double)(g() >> 11) * (1.0 / 9007199254740992.0); }
+    double normal()
+    {
+        double u1 = 1.0 - uniform(), u2 = uniform();
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    }
+};
+
+int mf_gen_uniform(unsigned long long seed, int count, double lo, double hi, double *out)
+{
+    Rng r(seed);
+    for (int i = 0; i < count; i++) out[i] = lo + (hi - lo) * r.uniform();
+    return MF_OK;
+}
+
+int mf_gen_normal(unsigned long long seed, int count, double *out)
+{
+    Rng r(seed);
+    for (int i = 0; i < count; i++) out[i] = r.normal();
+    return MF_OK;
+}
+
+struct CsrBuild {
+    std::vector<int> rp{0}, ci;
+    std::vector<double> v;
+    void add(int c, double x) { ci.push_back(c); v.push_back(x); }
+    void end_row() { rp.push_back((int)ci.size()); }
+    int emit(int **rowptr, int **colidx, double **val)
+    {
+        *rowptr = (int *)std::malloc(sizeof(int) * rp.size());
+        *colidx = (int *)std::malloc(sizeof(int) * std::max<size_t>(ci.size(), 1));
+        *val = (double *)std::malloc(sizeof(double) * std::max<size_t>(v.size(), 1));
+        if (!*rowptr || !*colidx || !*val) return fail(MF_E_NOMEM, "generator");
+        std::memcpy(*rowptr, rp.data(), sizeof(int) * rp.size());
+        std::memcpy(*colidx, ci.data(), sizeof(int) * ci.size());
+        std::memcpy(*val, v.data(), sizeof(double) * v.size());
+        return MF_OK;
+    }
+};
+
+// 2D 5-point Laplacian, Dirichlet: diag 4, neighbours -1 (ascending columns).
+int mf_gen_laplace2d(int nx, int ny, int *n, int **rowptr, int **colidx, double **val)
+{
+    if (nx <= 0 || ny <= 0) return fail(MF_E_ARG, "laplace2d dims");
+    CsrBuild b;
+    b.ci.reserve((size_t)nx * ny * 5);
+    b.v.reserve((size_t)nx * ny * 5);
+    for (int y = 0; y < ny; y++)
+        for (int x = 0; x < nx; x++) {
+            int i = x + nx * y;
+            if (y > 0) b.add(i - nx, -1.0);
+            if (x > 0) b.add(i - 1, -1.0);
+            b.add(i, 4.0);
+            if (x < nx - 1) b.add(i + 1, -1.0);
+            if (y < ny - 1) b.add(i + nx, -1.0);
+            b.end_row();
+        }
+    *n = nx * ny;
+    return b.emit(rowptr, colidx, val);
+}
+
+// 3D 7-point Poisson, Dirichlet: diag 6, neighbours -1.
+int mf_gen_poisson3d(int nx, int ny, int nz, int *n, int **rowptr, int **colidx, double **val)
+{
+    if (nx <= 0 || ny <= 0 || nz <= 0) return fail(MF_E_ARG, "poisson3d dims");
+    size_t N = (size_t)nx * ny * nz;
+    if (N * 7 > 2000000000ull) return fail(MF_E_ARG, "poisson3d too large");
+    CsrBuild b;
+    b.ci.reserve(N * 7);
+    b.v.reserve(N * 7);
+    b.rp.reserve(N + 1);
+    const int sx = 1, sy = nx, sz = nx * ny;
+    for (int z = 0; z < nz; z++)
+        for (int y = 0; y < ny; y++)
+            for (int x = 0; x < nx; x++) {
+                int i = x + nx * (y + ny * z);
+                if (z > 0) b.add(i - sz, -1.0);
+                if (y > 0) b.add(i - sy, -1.0);
+                if (x > 0) b.add(i - sx, -1.0);
+                b.add(i, 6.0);
+                if (x < nx - 1) b.add(i + sx, -1.0);
+                if (y < ny - 1) b.add(i + sy, -1.0);
+                if (z < nz - 1) b.add(i + sz, -1.0);
+                b.end_row();
+            }
+    *n = (int)N;
+    return b.emit(rowptr, colidx, val);
+}
+
+// 27-point convection-diffusion on an m^3 grid (h = 1), nonsymmetric M-matrix.
+// Cell diffusion kappa = exp(sigma * N(0,1)) from the seeded stream (cell order
+// x fastest).  The coupling to each of the 26 neighbours d is
+// -w(d) * kappa_face with kappa_face the harmonic mean of the two cells and
+// w = 1, 1/2, 1/4 for face, edge and corner neighbours; Dirichlet neighbours
+// outside the grid add to the diagonal only.  First-order upwind convection
+// with velocity (v, v, v), v = peclet / sqrt(3) (unit reference diffusion):
+// for each axis the face neighbour at -e_a gets -v and the diagonal +v.
+int mf_gen_convdiff3d(int m, double peclet, double sigma, unsigned long long seed, int *n,
+                      int **rowptr, int **colidx, double **val)
+{
+    if (m <= 0) return fail(MF_E_ARG, "convdiff3d dims");
+    size_t N = (size_t)m * m * m;
+    if (N * 27 > 2000000000ull) return fail(MF_E_ARG, "convdiff3d too large");
+    std::vector<double> kap(N);
+    Rng rng(seed);
+    for (size_t i = 0; i < N; i++) kap[i] = std::exp(sigma * rng.normal());
+    const double vel = peclet / std::sqrt(3.0);
+    CsrBuild b;
+    b.ci.reserve(N * 27);
+    b.v.reserve(N * 27);
+    b.rp.reserve(N + 1);
+    for (int z = 0; z < m; z++)
+        for (int y = 0; y < m; y++)
+            for (int x = 0; x < m; x++) {
+                long long i = x + (long long)m * (y + (long long)m * z);
+                double diag = 0.0;
+                size_t diag_pos = 0;
+                for (int dz = -1; dz <= 1; dz++)
+                    for (int dy = -1; dy <= 1; dy++)
+                        for (int dx = -1; dx <= 1; dx++) {
+                            int nn = (dx != 0) + (dy != 0) + (dz != 0);
+                            if (nn == 0) { diag_pos = b.ci.size(); b.add((int)i, 0.0); continue; }
+                            double w = nn == 1 ? 1.0 : (nn == 2 ? 0.5 : 0.25);
+                            int X = x + dx, Y = y + dy, Z = z + dz;
+                            bool inside = X >= 0 && X < m && Y >= 0 && Y < m && Z >= 0 && Z < m;
+                            double kj = inside ? kap[X + (size_t)m * (Y + (size_t)m * Z)] : kap[i];
+                            double kf = 2.0 * kap[i] * kj / (kap[i] + kj);
+                            double a = w * kf;
+                            // upwind: only the -e_a face neighbours get the convective term
+                            double c = (nn == 1 && (dx < 0 || dy < 0 || dz < 0)) ? vel : 0.0;
+                            diag += a + c;
+                            if (inside) b.add(X + m * (Y + m * Z), -(a + c));
+                        }
+                b.v[diag_pos] = diag;
+                b.end_row();
+            }
+    *n = (int)N;
+    return b.emit(rowptr, colidx, val);
+}
+
+// Random SPD: off-diagonal counts L_i = clip(floor(1/u), lmin, lmax) (a
+// discrete Pareto with P(k) ~ k^-zipf_a for zipf_a = 2; for other a the
+// exponent is 1/(a-1)), columns uniform over the other rows, values N(0,1);
+// A = B + B^T with duplicates summed, then diag = 1.05 * off-diagonal |row| sum.
+int mf_gen_random_spd(int n, double zipf_a, int lmin, int lmax, unsigned long long seed, int *nnz,
+                      int **rowptr, int **colidx, double **val)
+{
+    if (n <= 1 || lmin < 0 || lmax < lmin) return fail(MF_E_ARG, "random_spd args");
+    Rng rng(seed);
+    std::vector<int> ri, cj;
+    std::vector<double> vv;
+    for (int i = 0; i < n; i++) {
+        double u = rng.uniform();
+        double k = std::floor(std::pow(1.0 - u, -1.0 / (zipf_a - 1.0)));
+        long long L = (long long)std::min<double>(std::max<double>(k, lmin), lmax);
+        for (long long q = 0; q < L; q++) {
+            int j = (int)(rng.uniform() * (n - 1));
+            if (j >= i) j++;
+            double v = rng.normal();
+            ri.push_back(i); cj.push_back(j); vv.push_back(v);
+            ri.push_back(j); cj.push_back(i); vv.push_back(v);
+        }
+    }
+    // stable sort by (row, col): mirrored duplicates are summed in the same
+    // order, so A stays exactly symmetric
+    std::vector<size_t> ord(ri.size());
+    for (size_t k = 0; k < ord.size(); k++) ord[k] = k;
+    std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) {
+        return ri[a] != ri[b] ? ri[a] < ri[b] : cj[a] < cj[b];
+    });
+    CsrBuild b;
+    b.ci.reserve(ord.size() + n);
+    b.v.reserve(ord.size() + n);
+    size_t k = 0;
+    for (int i = 0; i < n; i++) {
+        double abssum = 0.0;
+        size_t row_start = b.ci.size();
+        bool diag_done = false;
+        while (k < ord.size() && ri[ord[k]] == i) {
+            int c = cj[ord[k]];
+            double s = 0.0;
+            while (k < ord.size() && ri[ord[k]] == i && cj[ord[k]] == c) s += vv[ord[k++]];
+            if (!diag_done && c > i) { b.add(i, 0.0); diag_done = true; }
+            b.add(c, s);
+            abssum += std::fabs(s);
+        }
+        if (!diag_done) b.add(i, 0.0);
+        for (size_t q = row_start; q < b.ci.size(); q++)
+            if (b.ci[q] == i) b.v[q] = 1.05 * abssum;
+        b.end_row();
+    }
+    *nnz = (int)b.ci.size();
+    return b.emit(rowptr, colidx, val);
+}
+
+// Lognormal-coefficient SPD stencil for the batched systems: dim 2 (5-point) or
+// 3 (7-point) on near-square / near-cube dims whose product is >= n_target
+// (padded, not truncated); cell coefficients exp(sigma N(0,1)); face coupling
+// -harmonic mean; Dirichlet faces add the cell's own coefficient to the diagonal.
+int mf_gen_stencil_lognormal(int dim, int n_target, double sigma, unsigned long long seed, int *n,
+                             int **rowptr, int **colidx, double **val)
+{
+    if ((dim != 2 && dim != 3) || n_target <= 0) return fail(MF_E_ARG, "stencil args");
+    int nx, ny, nz;
+    if (dim == 2) {
+        nx = (int)std::ceil(std::sqrt((double)n_target));
+        ny = (n_target + nx - 1) / nx;
+        nz = 1;
+    } else {
+        nx = (int)std::ceil(std::cbrt((double)n_target));
+        ny = nx;
+        nz = (n_target + nx * ny - 1) / (nx * ny);
+    }
+    size_t N = (size_t)nx * ny * nz;
+    std::vector<double> kap(N);
+    Rng rng(seed);
+    for (size_t i = 0; i < N; i++) kap[i] = std::exp(sigma * rng.normal());
+    auto hm = [](double a, double c) { return 2.0 * a * c / (a + c); };
+    CsrBuild b;
+    for (int z = 0; z < nz; z++)
+        for (int y = 0; y < ny; y++)
+            for (int x = 0; x < nx; x++) {
+                size_t i = x + (size_t)nx * (y + (size_t)ny * z);
+                const int off[6][3] = {{0, 0, -1}, {0, -1, 0}, {-1, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+                double diag = 0.0;
+                size_t dpos = 0;
+                for (int q = 0; q < 6; q++) {
+                    if (q == 3) { dpos = b.ci.size(); b.add((int)i, 0.0); }
+                    if (dim == 2 && off[q][2] != 0) continue;
+                    int X = x + off[q][0], Y = y + off[q][1], Z = z + off[q][2];
+                    bool inside = X >= 0 && X < nx && Y >= 0 && Y < ny && Z >= 0 && Z < nz;
+                    size_t j = inside ? X + (size_t)nx * (Y + (size_t)ny * Z) : i;
+                    double a = hm(kap[i], kap[j]);
+                    diag += a;
+                    if (inside) b.add((int)j, -a);
+                }
+                b.v[dpos] = diag;
+                b.end_row();
+            }
+    *n = (int)N;
+    return b.emit(rowptr, colidx, val);
+}
+
+}  // extern "C"

@@ -1,0 +1,742 @@
+// Tiled SpMV and the persistent, single-launch CG / BiCGSTAB kernels.
+//
+// Persistent kernels: one workgroup per CU (cooperative launch, so every
+// workgroup is resident).  Workgroup g owns a contiguous chunk of tile rows
+// (XCD-aware chunk order); thread (group, lane) owns row lane of tile rows
+// chunk_start + i * groups + group for i < RPT, and keeps every Krylov vector
+// of those rows in registers for the whole solve.  Only the vector that the
+// SpMV gathers is published to global memory each iteration.  Reductions are
+// deterministic: per-workgroup partials are written to a slot and every
+// workgroup sums all partials in the same fixed order after the barrier, so
+// all workgroups take identical control decisions.
+//
+// CG uses the Chronopoulos-Gear form of Hestenes-Stiefel CG (one SpMV and one
+// fused reduction of (r,r) and (Ar,r) per iteration: two grid barriers).  The
+// reference's recurrences (PC/Mille-feuille_CG_AMD.cu:607-912: q = A d,
+// alpha = snew/(d.q), x += alpha d, r -= alpha q, beta = snew/sold,
+// d = r + beta d) are the same method in exact arithmetic.
+// BiCGSTAB follows PC/Mille-feuille_BiCGSTAB_AMD.cu:61-425 (van der Vorst),
+// with the omega reduction of :735 computed as intended.
+//
+// Both kernels stop on the true residual: when the updated residual reaches
+// the tolerance (or has dropped by rr_factor since the last replacement) they
+// publish x, compute b - A x with FP64 accumulation over the stored values,
+// and either finish or replace r by it (reliable update) and continue.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "mf_matrix.h"
+
+namespace mf {
+
+// ------------------------------------------------------------ SpMV kernel
+// 256 threads = 256/B tile rows per block, lane = row inside the tile row.
+template <int B, bool IDX8>
+__global__ __launch_bounds__(256) void k_spmv(DevL L, int tilem, const double *__restrict__ g64,
+                                              const float *__restrict__ g32, double *__restrict__ y)
+{
+    const int lb = xcd_chunk(blockIdx.x, gridDim.x);
+    const int tr = lb * (256 / B) + threadIdx.x / B, r = threadIdx.x % B;
+    const int lanebase = (threadIdx.x & 63) & ~(B - 1);
+    if (tr >= tilem) return;
+    y[(size_t)tr * B + r] = tile_row_spmv<B, IDX8>(L, tr, r, lanebase, g64, g32);
+}
+
+__global__ void k_to_f32(const double *__restrict__ x, float *__restrict__ y, long long n)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = (float)x[i];
+}
+
+// r = b - A x (FP64 accumulation over stored values); per-block partial ||r||^2
+template <int B, bool IDX8>
+__global__ __launch_bounds__(256) void k_residual(DevL L, int tilem, const double *__restrict__ b,
+                                                  const double *__restrict__ x, double *__restrict__ r,
+                                                  double *__restrict__ part)
+{
+    __shared__ double s_w[4];
+    const int tr = blockIdx.x * (256 / B) + threadIdx.x / B, lane = threadIdx.x % B;
+    const int lanebase = (threadIdx.x & 63) & ~(B - 1);
+    double v = 0.0;
+    if (tr < tilem) {
+        const double *g[1] = {x};
+        double acc[1] = {0.0};
+        tile_row_spmv_f64<B, IDX8, 1>(L, tr, lane, lanebase, g, acc);
+        const size_t row = (size_t)tr * B + lane;
+        const double rv = b[row] - acc[0];
+        r[row] = rv;
+        v = rv * rv;
+    }
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = ((s_w[0] + s_w[1]) + s_w[2]) + s_w[3];
+}
+
+__global__ void k_axpy(double *__restrict__ x, const double *__restrict__ d, long long n)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] += d[i];
+}
+
+// --------------------------------------------------------- solver kernels
+struct SolveArgs {
+    DevL L;
+    int tilem, G, trw;
+    const double *b;
+    double *x;
+    double *g64;       // published Krylov vector (FP64)
+    float *g32;        // its FP32 copy for low-precision tiles
+    double *h64, *q64; // publishing slots of the true-residual pass
+    double *partials;  // [4][G][8]
+    unsigned *bar;
+    int *ctl;          // 0 abort, 1 status, 2 iterations, 3 replacements
+    double *out;       // 0 ||b||, 1 final ||r_true||^2, 2 final updated ||r||^2
+    double tol, rr_factor;
+    int maxit, has64, haslow;
+    long long spin_limit;
+};
+
+template <int TPB, int K>
+struct Reducer {
+    double *s_red;  // [TPB/64][K]
+    double *s_bc;   // [K]
+    int *s_flag;
+    unsigned ep = 0;
+    int slot = 0;
+
+    __device__ bool barrier(const SolveArgs &A)
+    {
+        ++ep;
+        return grid_barrier(A.bar, A.ctl, ep * (unsigned)A.G, A.spin_limit, s_flag);
+    }
+
+    // Sums v over the grid; every workgroup obtains the same bits.
+    template <int N>
+    __device__ bool allreduce(const SolveArgs &A, double (&v)[N])
+    {
+        static_assert(N <= K, "reducer width");
+        const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+        for (int k = 0; k < N; k++) {
+            double s = wave_sum(v[k]);
+            if (lane == 0) s_red[wv * K + k] = s;
+        }
+        __syncthreads();
+        const int sl = slot;
+        slot = (slot + 1) & 3;
+        if (tid == 0) {
+#pragma unroll
+            for (int k = 0; k < N; k++) {
+                double s = 0.0;
+                for (int w = 0; w < TPB / 64; w++) s += s_red[w * K + k];
+                A.partials[((size_t)sl * A.G + blockIdx.x) * 8 + k] = s;
+            }
+        }
+        if (!barrier(A)) return false;
+        if (tid < 64) {
+#pragma unroll
+            for (int k = 0; k < N; k++) {
+                double s = 0.0;
+                for (int g = tid; g < A.G; g += 64) s += A.partials[((size_t)sl * A.G + g) * 8 + k];
+                s = wave_sum(s);
+                if (tid == 0) s_bc[k] = s;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < N; k++) v[k] = s_bc[k];
+        __syncthreads();
+        return true;
+    }
+};
+
+// slices per load batch inside the persistent kernels: 8 keeps enough loads
+// in flight per wave; 16 rows per thread leave registers for 4 only
+__host__ __device__ constexpr int spmv_u(int rpt) { return rpt >= 16 ? 4 : 8; }
+
+__device__ __forceinline__ bool finite_all(double a, double b) { return isfinite(a) && isfinite(b); }
+
+// Hides the value of v from the optimiser.  Applied to the thread's row base
+// at the top of every solver iteration so that the 64-bit addresses of the
+// RPT rows are recomputed where they are used instead of being hoisted out of
+// the loop (which would keep RPT x arrays addresses live and spill).
+__device__ __forceinline__ int opaque(int v)
+{
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
+// CG (Chronopoulos-Gear): registers hold x and p of the thread's rows, LDS
+// holds r and s = A p.  Per iteration: [r published, (r,r) reduced] ->
+// convergence test -> w = A r with s = w + beta s and (w, r) in the same
+// pass -> reduce (w, r) -> p, x, r updates -> publish r, reduce (r, r).
+template <int B, bool IDX8, int TPB, int RPT>
+__global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
+{
+    __shared__ double s_r[RPT * TPB];
+    __shared__ double s_s[RPT * TPB];
+    __shared__ double s_red[(TPB / 64) * 4];
+    __shared__ double s_bc[4];
+    __shared__ int s_flag;
+    Reducer<TPB, 4> R{s_red, s_bc, &s_flag};
+    constexpr int GPB = TPB / B;
+    const int tid = threadIdx.x;
+    const int chunk = xcd_chunk(blockIdx.x, gridDim.x);
+    const int grp = tid / B, lane = tid % B, lanebase = (tid & 63) & ~(B - 1);
+    const int tr0 = chunk * A.trw, tr1 = min(A.tilem, tr0 + A.trw);
+    double xr[RPT], pr[RPT];
+    int rbase = tr0 * B + tid;  // row of step i is rbase + i * TPB
+    auto tile_row = [&](int i) { return tr0 + i * GPB + grp; };
+    auto row_of = [&](int i) { return (size_t)(rbase + i * TPB); };
+    auto publish = [&](size_t row, double v) {
+        if (A.has64) A.g64[row] = v;
+        if (A.haslow) A.g32[row] = (float)v;
+    };
+    auto finish = [&](int status, int iter, int repl, double rt2, double g2) {
+#pragma clang loop unroll(full)
+        for (int i = 0; i < RPT; i++)
+            if (tile_row(i) < tr1) A.x[row_of(i)] = xr[i];
+        if (blockIdx.x == 0 && tid == 0) {
+            A.ctl[1] = status;
+            A.ctl[2] = iter;
+            A.ctl[3] = repl;
+            A.out[1] = rt2;
+            A.out[2] = g2;
+        }
+    };
+
+    double v1[1] = {0.0};
+#pragma clang loop unroll(full)
+    for (int i = 0; i < RPT; i++) {
+        xr[i] = pr[i] = 0.0;
+        s_r[i * TPB + tid] = 0.0;
+        s_s[i * TPB + tid] = 0.0;
+        if (tile_row(i) < tr1) {
+            const size_t row = row_of(i);
+            const double bv = A.b[row];
+            s_r[i * TPB + tid] = bv;
+            publish(row, bv);
+            v1[0] += bv * bv;
+        }
+    }
+    if (!R.allreduce(A, v1)) return;
+    const double bnorm2 = v1[0];
+    if (blockIdx.x == 0 && tid == 0) A.out[0] = sqrt(bnorm2);
+    const double tol2 = A.tol * A.tol * bnorm2;
+    if (bnorm2 == 0.0) { finish(MF_OK, 0, 0, 0.0, 0.0); return; }
+    double gamma = bnorm2, ref2 = bnorm2, gprev = 1.0, aprev = 1.0;
+    int iter = 0, repl = 0;
+    for (;;) {
+        rbase = opaque(rbase);
+        if (!isfinite(gamma)) { finish(MF_E_NONFINITE, iter, repl, -1.0, gamma); return; }
+        const bool conv = gamma <= tol2;
+        const bool due = A.rr_factor > 0.0 && gamma <= A.rr_factor * A.rr_factor * ref2;
+        if (conv || due) {
+            // true residual b - A x and s = A p in one FP64-accumulated pass
+#pragma clang loop unroll(full)
+            for (int i = 0; i < RPT; i++)
+                if (tile_row(i) < tr1) {
+                    A.h64[row_of(i)] = xr[i];
+                    A.q64[row_of(i)] = pr[i];
+                }
+            if (!R.barrier(A)) return;
+            double t1[1] = {0.0};
+#pragma unroll 1
+            for (int i = 0; i < RPT; i++) {
+                const int tr = tile_row(i);
+                if (tr < tr1) {
+                    const double *g[2] = {A.h64, A.q64};
+                    double acc[2] = {0.0, 0.0};
+                    tile_row_spmv_f64<B, IDX8, 2>(A.L, tr, lane, lanebase, g, acc);
+                    const size_t row = (size_t)tr * B + lane;
+                    const double rv = A.b[row] - acc[0];
+                    s_r[i * TPB + tid] = rv;
+                    s_s[i * TPB + tid] = acc[1];
+                    publish(row, rv);
+                    t1[0] += rv * rv;
+                }
+            }
+            if (!R.allreduce(A, t1)) return;
+            ++repl;
+            if (!isfinite(t1[0])) { finish(MF_E_NONFINITE, iter, repl, t1[0], gamma); return; }
+            if (t1[0] <= tol2) { finish(MF_OK, iter, repl, t1[0], gamma); return; }
+            gamma = ref2 = t1[0];
+            continue;
+        }
+        if (iter >= A.maxit) { finish(MF_E_MAXIT, iter, repl, -1.0, gamma); return; }
+        const double beta = iter ? gamma / gprev : 0.0;
+        // w = A r; s = w + beta s; (w, r)
+        double red[1] = {0.0};
+#pragma unroll 1
+        for (int i = 0; i < RPT; i++) {
+            const int tr = tile_row(i);
+            if (tr < tr1) {
+                const double w = tile_row_spmv<B, IDX8, spmv_u(RPT)>(A.L, tr, lane, lanebase, A.g64, A.g32);
+                red[0] += w * s_r[i * TPB + tid];
+                s_s[i * TPB + tid] = w + beta * s_s[i * TPB + tid];
+            }
+        }
+        if (!R.allreduce(A, red)) return;
+        const double delta = red[0];
+        const double denom = iter ? delta - beta * gamma / aprev : delta;
+        if (!isfinite(denom)) { finish(MF_E_NONFINITE, iter, repl, -1.0, gamma); return; }
+        if (!(denom > 0.0)) { finish(MF_E_BREAKDOWN, iter, repl, -1.0, gamma); return; }
+        const double alpha = gamma / denom;
+#pragma clang loop unroll(full)
+        for (int i = 0; i < RPT; i++)
+            if (tile_row(i) < tr1) {
+                pr[i] = s_r[i * TPB + tid] + beta * pr[i];
+                xr[i] = xr[i] + alpha * pr[i];
+            }
+        double g1[1] = {0.0};
+#pragma unroll 1
+        for (int i = 0; i < RPT; i++)
+            if (tile_row(i) < tr1) {
+                const double rv = s_r[i * TPB + tid] - alpha * s_s[i * TPB + tid];
+                s_r[i * TPB + tid] = rv;
+                publish(row_of(i), rv);
+                g1[0] += rv * rv;
+            }
+        gprev = gamma;
+        aprev = alpha;
+        ++iter;
+        if (!R.allreduce(A, g1)) return;
+        gamma = g1[0];
+    }
+}
+
+// BiCGSTAB: registers hold x, r, r^, p, s; LDS holds v = A p and t = A s.
+template <int B, bool IDX8, int TPB, int RPT>
+__global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
+{
+    __shared__ double s_v[RPT * TPB];
+    __shared__ double s_t[RPT * TPB];
+    __shared__ double s_red[(TPB / 64) * 4];
+    __shared__ double s_bc[4];
+    __shared__ int s_flag;
+    Reducer<TPB, 4> R{s_red, s_bc, &s_flag};
+    constexpr int GPB = TPB / B;
+    const int tid = threadIdx.x;
+    const int chunk = xcd_chunk(blockIdx.x, gridDim.x);
+    const int grp = tid / B, lane = tid % B, lanebase = (tid & 63) & ~(B - 1);
+    const int tr0 = chunk * A.trw, tr1 = min(A.tilem, tr0 + A.trw);
+    double xr[RPT], rr[RPT], rh[RPT], pr[RPT], sr[RPT];
+    int rbase = tr0 * B + tid;  // row of step i is rbase + i * TPB
+    auto tile_row = [&](int i) { return tr0 + i * GPB + grp; };
+    auto row_of = [&](int i) { return (size_t)(rbase + i * TPB); };
+    auto publish = [&](size_t row, double v) {
+        if (A.has64) A.g64[row] = v;
+        if (A.haslow) A.g32[row] = (float)v;
+    };
+    auto finish = [&](int status, int iter, int repl, double rt2, double g2) {
+#pragma clang loop unroll(full)
+        for (int i = 0; i < RPT; i++)
+            if (tile_row(i) < tr1) A.x[row_of(i)] = xr[i];
+        if (blockIdx.x == 0 && tid == 0) {
+            A.ctl[1] = status;
+            A.ctl[2] = iter;
+            A.ctl[3] = repl;
+            A.out[1] = rt2;
+            A.out[2] = g2;
+        }
+    };
+    auto spmv_into = [&](double *dst) {
+#pragma unroll 1
+        for (int i = 0; i < RPT; i++) {
+            const int tr = tile_row(i);
+            if (tr < tr1) dst[i * TPB + tid] = tile_row_spmv<B, IDX8, spmv_u(RPT)>(A.L, tr, lane, lanebase, A.g64, A.g32);
+        }
+    };
+
+    double v1[1] = {0.0};
+#pragma clang loop unroll(full)
+    for (int i = 0; i < RPT; i++) {
+        xr[i] = rr[i] = rh[i] = pr[i] = sr[i] = 0.0;
+        s_v[i * TPB + tid] = 0.0;
+        if (tile_row(i) < tr1) {
+            const size_t row = row_of(i);
+            rr[i] = rh[i] = pr[i] = A.b[row];
+            publish(row, pr[i]);
+            v1[0] += rr[i] * rr[i];
+        }
+    }
+    if (!R.allreduce(A, v1)) return;
+    const double bnorm2 = v1[0];
+    if (blockIdx.x == 0 && tid == 0) A.out[0] = sqrt(bnorm2);
+    const double tol2 = A.tol * A.tol * bnorm2;
+    if (bnorm2 == 0.0) { finish(MF_OK, 0, 0, 0.0, 0.0); return; }
+    double rho = bnorm2, ref2 = bnorm2;
+    int iter = 0, repl = 0;
+    for (;;) {
+        rbase = opaque(rbase);
+        spmv_into(s_v);  // v = A p
+        double a1[1] = {0.0};
+#pragma clang loop unroll(full)
+        for (int i = 0; i < RPT; i++)
+            if (tile_row(i) < tr1) a1[0] += rh[i] * s_v[i * TPB + tid];
+        if (!R.allreduce(A, a1)) return;
+        if (!(a1[0] != 0.0) || !isfinite(a1[0])) { finish(MF_E_BREAKDOWN, iter, repl, -1.0, -1.0); return; }
+        const double alpha = rho / a1[0];
+#pragma clang loop unroll(full)
+        for (int i = 0; i < RPT; i++)
+            if (tile_row(i) < tr1) {
+                sr[i] = rr[i] - alpha * s_v[i * TPB + tid];
+                publish(row_of(i), sr[i]);
+            }
+        if (!R.barrier(A)) return;
+        spmv_into(s_t);  // t = A s
+        double a2[2] = {0.0, 0.0};
+#pragma clang loop unroll(full)
+        for (int i = 0; i < RPT; i++)
+            if (tile_row(i) < tr1) {
+                const double t = s_t[i * TPB + tid];
+                a2[0] += t * sr[i];
+                a2[1] += t * t;
+            }
+        if (!R.allreduce(A, a2)) return;
+        const double omega = a2[1] > 0.0 ? a2[0] / a2[1] : 0.0;
+        double a3[2] = {0.0, 0.0};
+#pragma clang loop unroll(full)
+        for (int i = 0; i < RPT; i++)
+            if (tile_row(i) < tr1) {
+                xr[i] = xr[i] + alpha * pr[i] + omega * sr[i];
+                rr[i] = sr[i] - omega * s_t[i * TPB + tid];
+                a3[0] += rr[i] * rh[i];
+                a3[1] += rr[i] * rr[i];
+            }
+        if (!R.allreduce(A, a3)) return;
+        double rho_new = a3[0], g2 = a3[1];
+        ++iter;
+        if (!finite_all(rho_new, g2)) { finish(MF_E_NONFINITE, iter, repl, -1.0, g2); return; }
+        const bool conv = g2 <= tol2;
+        const bool due = A.rr_factor > 0.0 && g2 <= A.rr_factor * A.rr_factor * ref2;
+        if (conv || due) {
+#pragma clang loop unroll(full)
+            for (int i = 0; i < RPT; i++)
+                if (tile_row(i) < tr1) A.h64[row_of(i)] = xr[i];
+            if (!R.barrier(A)) return;
+#pragma unroll 1
+            for (int i = 0; i < RPT; i++) {
+                const int tr = tile_row(i);
+                if (tr < tr1) {
+                    const double *g[1] = {A.h64};
+                    double acc[1] = {0.0};
+                    tile_row_spmv_f64<B, IDX8, 1>(A.L, tr, lane, lanebase, g, acc);
+                    s_t[i * TPB + tid] = A.b[(size_t)tr * B + lane] - acc[0];
+                }
+            }
+            double t2[2] = {0.0, 0.0};
+#pragma clang loop unroll(full)
+            for (int i = 0; i < RPT; i++)
+                if (tile_row(i) < tr1) {
+                    rr[i] = s_t[i * TPB + tid];
+                    t2[0] += rr[i] * rr[i];
+                    t2[1] += rr[i] * rh[i];
+                }
+            if (!R.allreduce(A, t2)) return;
+            ++repl;
+            if (!isfinite(t2[0])) { finish(MF_E_NONFINITE, iter, repl, t2[0], g2); return; }
+            if (t2[0] <= tol2) { finish(MF_OK, iter, repl, t2[0], g2); return; }
+            ref2 = t2[0];
+            rho_new = t2[1];
+        }
+        if (iter >= A.maxit) { finish(MF_E_MAXIT, iter, repl, -1.0, g2); return; }
+        if (omega == 0.0 || rho == 0.0) { finish(MF_E_BREAKDOWN, iter, repl, -1.0, g2); return; }
+        const double beta = (rho_new / rho) * (alpha / omega);
+#pragma clang loop unroll(full)
+        for (int i = 0; i < RPT; i++)
+            if (tile_row(i) < tr1) {
+                pr[i] = rr[i] + beta * (pr[i] - omega * s_v[i * TPB + tid]);
+                publish(row_of(i), pr[i]);
+            }
+        rho = rho_new;
+        if (!R.barrier(A)) return;
+    }
+}
+
+// ------------------------------------------------------------ dispatch
+using KernelFn = void (*)(SolveArgs);
+
+struct KernelChoice {
+    KernelFn fn = nullptr;
+    int tpb = 0, rpt = 0;
+};
+
+// 512 threads (8 waves per CU, 256 VGPRs per wave) hold the register-resident
+// vectors of up to 16 rows per thread without spilling.
+template <int B, bool IDX8>
+static KernelChoice pick_kernel(int method, long long rows_per_wg)
+{
+    KernelChoice k;
+    const int q = (int)((rows_per_wg + 511) / 512);
+    if (method == MF_CG) {
+        if (q <= 1) k = {k_cg<B, IDX8, 512, 1>, 512, 1};
+        else if (q <= 2) k = {k_cg<B, IDX8, 512, 2>, 512, 2};
+        else if (q <= 4) k = {k_cg<B, IDX8, 512, 4>, 512, 4};
+        else if (q <= 8) k = {k_cg<B, IDX8, 512, 8>, 512, 8};
+        else if (q <= 16) k = {k_cg<B, IDX8, 512, 16>, 512, 16};
+    } else {
+        if (q <= 1) k = {k_bicgstab<B, IDX8, 512, 1>, 512, 1};
+        else if (q <= 2) k = {k_bicgstab<B, IDX8, 512, 2>, 512, 2};
+        else if (q <= 4) k = {k_bicgstab<B, IDX8, 512, 4>, 512, 4};
+        else if (q <= 8) k = {k_bicgstab<B, IDX8, 512, 8>, 512, 8};
+    }
+    return k;
+}
+
+static KernelChoice pick(int B, bool idx8, int method, long long rows_per_wg)
+{
+    if (B == 16) return idx8 ? pick_kernel<16, true>(method, rows_per_wg) : pick_kernel<16, false>(method, rows_per_wg);
+    return KernelChoice{};
+}
+
+void Workspace::release()
+{
+    void *ps[] = {g64, h64, q64, g32, partials, bar, ctl, out, r_outer, d, tmp, red};
+    for (void *p : ps)
+        if (p) (void)hipFree(p);
+    *this = Workspace();
+}
+
+static int ensure_ws(mf_matrix m, int G)
+{
+    Workspace &w = m->ws;
+    const size_t ncol = (size_t)std::max(m->tilem, m->tilen) * m->B;
+    if (w.ncol >= ncol && w.G >= G && w.g64) return MF_OK;
+    w.release();
+    w.ncol = ncol;
+    w.G = std::max(G, 1);
+    MF_HIP(hipMalloc((void **)&w.g64, sizeof(double) * ncol));
+    MF_HIP(hipMalloc((void **)&w.h64, sizeof(double) * ncol));
+    MF_HIP(hipMalloc((void **)&w.q64, sizeof(double) * ncol));
+    MF_HIP(hipMalloc((void **)&w.g32, sizeof(float) * ncol));
+    MF_HIP(hipMalloc((void **)&w.partials, sizeof(double) * 4 * 8 * w.G));
+    MF_HIP(hipMalloc((void **)&w.bar, sizeof(unsigned) * 4));
+    MF_HIP(hipMalloc((void **)&w.ctl, sizeof(int) * 8));
+    MF_HIP(hipMalloc((void **)&w.out, sizeof(double) * 4));
+    MF_HIP(hipMalloc((void **)&w.r_outer, sizeof(double) * ncol));
+    MF_HIP(hipMalloc((void **)&w.d, sizeof(double) * ncol));
+    MF_HIP(hipMalloc((void **)&w.tmp, sizeof(double) * ncol));
+    MF_HIP(hipMalloc((void **)&w.red, sizeof(double) * (ncol / 16 + 1024)));
+    // gather vectors are read up to tilen*B; entries beyond the rows stay zero
+    MF_HIP(hipMemset(w.g64, 0, sizeof(double) * ncol));
+    MF_HIP(hipMemset(w.g32, 0, sizeof(float) * ncol));
+    MF_HIP(hipMemset(w.h64, 0, sizeof(double) * ncol));
+    MF_HIP(hipMemset(w.q64, 0, sizeof(double) * ncol));
+    return MF_OK;
+}
+
+struct InnerResult {
+    int status = MF_OK, iterations = 0, replacements = 0;
+    double bnorm = 0, rt2 = 0, g2 = 0;
+};
+
+// Zeroes the barrier and control words with a kernel (no memset nodes).
+__global__ void k_reset_ctl(unsigned *bar, int *ctl)
+{
+    if (threadIdx.x < 4) bar[threadIdx.x] = 0u;
+    if (threadIdx.x < 8) ctl[threadIdx.x] = 0;
+}
+
+// One persistent launch: solves L x = b (x overwritten) on the stream.
+static int launch_inner(mf_matrix m, const Layout &L, int method, const double *d_b, double *d_x,
+                        double tol, const mf_opts &o, bool sync, InnerResult *res, int *grid_out,
+                        int *block_out, int *rpt_out)
+{
+    hipStream_t st = stream();
+    int G = o.grid > 0 ? o.grid : device_cu_count();
+    G = std::max(1, std::min(G, m->tilem));
+    const int trw = (m->tilem + G - 1) / G;
+    G = (m->tilem + trw - 1) / trw;  // no empty workgroups
+    KernelChoice kc = pick(m->B, L.idx8, method, (long long)trw * m->B);
+    if (!kc.fn)
+        return fail(MF_E_UNSUPPORTED,
+                    "system too large for register-resident vectors at this grid size");
+    int rc = ensure_ws(m, G);
+    if (rc) return rc;
+    Workspace &w = m->ws;
+    SolveArgs A;
+    A.L = L.view(m->d_tile_ptr);
+    A.tilem = m->tilem;
+    A.G = G;
+    A.trw = trw;
+    A.b = d_b;
+    A.x = d_x;
+    A.g64 = w.g64;
+    A.g32 = w.g32;
+    A.h64 = w.h64;
+    A.q64 = w.q64;
+    A.partials = w.partials;
+    A.bar = w.bar;
+    A.ctl = w.ctl;
+    A.out = w.out;
+    A.tol = tol;
+    A.rr_factor = o.rr_factor;
+    A.maxit = o.maxit;
+    A.has64 = L.has64 ? 1 : 0;
+    A.haslow = L.haslow ? 1 : 0;
+    A.spin_limit = o.spin_limit;
+    k_reset_ctl<<<1, 64, 0, st>>>(w.bar, w.ctl);
+    MF_HIP(hipGetLastError());
+    void *args[] = {&A};
+    MF_HIP(hipLaunchCooperativeKernel((const void *)kc.fn, dim3(G), dim3(kc.tpb), args, 0, st));
+    if (grid_out) *grid_out = G;
+    if (block_out) *block_out = kc.tpb;
+    if (rpt_out) *rpt_out = kc.rpt;
+    if (sync && res) {
+        int ctl[8];
+        double out[4];
+        MF_HIP(hipMemcpyAsync(ctl, w.ctl, sizeof(ctl), hipMemcpyDeviceToHost, st));
+        MF_HIP(hipMemcpyAsync(out, w.out, sizeof(out), hipMemcpyDeviceToHost, st));
+        MF_HIP(hipStreamSynchronize(st));
+        res->status = ctl[0] ? MF_E_TIMEOUT : ctl[1];
+        res->iterations = ctl[2];
+        res->replacements = ctl[3];
+        res->bnorm = out[0];
+        res->rt2 = out[1];
+        res->g2 = out[2];
+    }
+    return MF_OK;
+}
+
+// r = b - A x over layout L; returns ||r|| (FP64) via host sum of partials.
+static int residual(mf_matrix m, const Layout &L, const double *d_b, const double *d_x, double *d_r,
+                    double *norm)
+{
+    hipStream_t st = stream();
+    const int B = m->B;
+    const int nb = (m->tilem * B + 255) / 256;
+    DevL V = L.view(m->d_tile_ptr);
+#define MF_RES(BB, I8) k_residual<BB, I8><<<nb, 256, 0, st>>>(V, m->tilem, d_b, d_x, d_r, m->ws.red)
+    if (B == 16) { if (L.idx8) MF_RES(16, true); else MF_RES(16, false); }
+    else if (B == 8) { if (L.idx8) MF_RES(8, true); else MF_RES(8, false); }
+    else { if (L.idx8) MF_RES(32, true); else MF_RES(32, false); }
+#undef MF_RES
+    MF_HIP(hipGetLastError());
+    std::vector<double> part(nb);
+    MF_HIP(hipMemcpyAsync(part.data(), m->ws.red, sizeof(double) * nb, hipMemcpyDeviceToHost, st));
+    MF_HIP(hipStreamSynchronize(st));
+    double s = 0.0;
+    for (double v : part) s += v;
+    *norm = std::sqrt(s);
+    return MF_OK;
+}
+
+int solve_device(mf_matrix m, const double *d_b, double *d_x, const mf_opts &o, mf_report *rep)
+{
+    hipStream_t st = stream();
+    std::memset(rep, 0, sizeof(*rep));
+    if (m->rowA != m->colA) return fail(MF_E_ARG, "solver needs a square matrix");
+    double t0 = now_ms();
+    int rc;
+    if (!m->L64.built) {
+        rc = build_layout(m, m->L64, 0.0, 0);
+        if (rc) return rc;
+    }
+    const bool mixed = o.precision == MF_MIXED;
+    if (mixed && (!m->Lmix.built || m->Lmix.tau != o.tau || m->Lmix.allow_mask != o.allow_mask)) {
+        rc = build_layout(m, m->Lmix, o.tau, o.allow_mask);
+        if (rc) return rc;
+    }
+    rep->ms_setup = now_ms() - t0;
+    const Layout &L = mixed ? m->Lmix : m->L64;
+    // lossless layout: the stored operator is A itself, so the kernel's own
+    // true-residual test is final; otherwise refine in FP64 around it
+    const bool exact = !mixed || o.tau == 0.0;
+    rc = ensure_ws(m, std::max(1, o.grid > 0 ? o.grid : device_cu_count()));
+    if (rc) return rc;
+    Workspace &w = m->ws;
+    const size_t n = (size_t)m->tilem * m->B;
+    hipEvent_t e0, e1;
+    MF_HIP(hipEventCreate(&e0));
+    MF_HIP(hipEventCreate(&e1));
+    MF_HIP(hipEventRecord(e0, st));
+    InnerResult ir;
+    int G = 0, TPB = 0, RPT = 0;
+    int status = MF_OK, iters = 0, repl = 0, outer = 0;
+    double relres = -1.0;
+    if (exact) {
+        rc = launch_inner(m, L, o.method, d_b, d_x, o.tol, o, true, &ir, &G, &TPB, &RPT);
+        if (rc) return rc;
+        status = ir.status;
+        iters = ir.iterations;
+        repl = ir.replacements;
+        outer = 1;
+        if (ir.bnorm > 0 && ir.rt2 >= 0) relres = std::sqrt(ir.rt2) / ir.bnorm;
+    } else {
+        // x = 0; r = b; repeat: solve L d = r to a relative tolerance, x += d,
+        // r = b - A x in FP64 (FP64 layout)
+        MF_HIP(hipMemsetAsync(d_x, 0, sizeof(double) * n, st));
+        MF_HIP(hipMemcpyAsync(w.r_outer, d_b, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+        double bnorm = 0.0, rnorm = 0.0;
+        rc = residual(m, m->L64, d_b, d_x, w.r_outer, &bnorm);
+        if (rc) return rc;
+        rnorm = bnorm;
+        status = MF_E_MAXIT;
+        for (outer = 0; outer < o.ir_max_outer && rnorm > o.tol * bnorm; ++outer) {
+            const double inner_tol = std::max(o.tol * bnorm / rnorm, o.ir_inner_tol);
+            rc = launch_inner(m, L, o.method, w.r_outer, w.d, inner_tol, o, true, &ir, &G, &TPB, &RPT);
+            if (rc) return rc;
+            iters += ir.iterations;
+            repl += ir.replacements;
+            if (ir.status == MF_E_TIMEOUT || ir.status == MF_E_NONFINITE) { status = ir.status; break; }
+            k_axpy<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(d_x, w.d, (long long)n);
+            rc = residual(m, m->L64, d_b, d_x, w.r_outer, &rnorm);
+            if (rc) return rc;
+        }
+        if (rnorm <= o.tol * bnorm) status = MF_OK;
+        relres = bnorm > 0 ? rnorm / bnorm : 0.0;
+    }
+    MF_HIP(hipEventRecord(e1, st));
+    MF_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    MF_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    // independent FP64 check of the returned x (outside the timed region)
+    double bn = 0.0, rn = 0.0;
+    MF_HIP(hipMemsetAsync(w.tmp, 0, sizeof(double) * n, st));
+    rc = residual(m, m->L64, d_b, w.tmp, w.d, &bn);
+    if (rc) return rc;
+    rc = residual(m, m->L64, d_b, d_x, w.d, &rn);
+    if (rc) return rc;
+    rep->relres_true = bn > 0 ? rn / bn : rn;
+    (void)relres;
+    rep->status = status;
+    rep->iterations = iters;
+    rep->outer = outer;
+    rep->replacements = repl;
+    rep->ms_solve = ms;
+    rep->grid = G;
+    rep->block = TPB;
+    rep->rows_per_thread = RPT;
+    return status == MF_E_TIMEOUT ? fail(MF_E_TIMEOUT, "grid barrier timeout") : MF_OK;
+}
+
+// SpMV y = A x (device pointers); x has tilen*B entries.
+int spmv_device(mf_matrix m, const Layout &L, const double *d_x, double *d_y, bool convert)
+{
+    hipStream_t st = stream();
+    const int B = m->B;
+    const long long ncol = (long long)m->tilen * B;
+    int rc = ensure_ws(m, 1);
+    if (rc) return rc;
+    if (L.haslow && convert)
+        k_to_f32<<<(unsigned)((ncol + 255) / 256), 256, 0, st>>>(d_x, m->ws.g32, ncol);
+    const int nb = (int)(((long long)m->tilem * B + 255) / 256);
+    DevL V = L.view(m->d_tile_ptr);
+#define MF_SPMV(BB, I8) k_spmv<BB, I8><<<nb, 256, 0, st>>>(V, m->tilem, d_x, m->ws.g32, d_y)
+    if (B == 16) { if (L.idx8) MF_SPMV(16, true); else MF_SPMV(16, false); }
+    else if (B == 8) { if (L.idx8) MF_SPMV(8, true); else MF_SPMV(8, false); }
+    else { if (L.idx8) MF_SPMV(32, true); else MF_SPMV(32, false); }
+#undef MF_SPMV
+    MF_HIP(hipGetLastError());
+    return MF_OK;
+}
+
+}  // namespace mf

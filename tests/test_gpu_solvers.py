@@ -1,0 +1,181 @@
+"""GPU SpMV and persistent-solver parity (needs an MI355X).
+
+SpMV: bit-exact against the oracle's tile SpMV (same accumulation order, fma).
+Solvers: true relative residual <= 1e-10 computed on the GPU and re-checked on
+the CPU, iteration counts within 5% (+2) of the oracle's FP64 CG / BiCGSTAB,
+and the solution within the conditioning-scaled tolerance of the oracle's."""
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from fixtures import GOLDEN, load_golden
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def _true_relres(rp, ci, va, x, b):
+    A = sp.csr_matrix((va, ci, rp), shape=(len(rp) - 1, len(x)))
+    return np.linalg.norm(b - A @ x) / np.linalg.norm(b)
+
+
+def _iters_close(a, b):
+    return abs(a - b) <= 0.05 * b + 2
+
+
+@pytest.mark.parametrize("name", ["1138_bus", "nontrivial_mc64_example", "wide", "ragged", "dup_wrap"])
+def test_gpu_spmv_fp64_bit_exact(gpu, orc, name):
+    g = load_golden(name)
+    m, n = int(g["m"]), int(g["n"])
+    A = gpu.Matrix(g["rowptr"], g["col"], g["val"], rowA=m, colA=n)
+    t = orc.tiles(16, m, n, g["rowptr"], g["col"], g["val"])
+    x = np.random.default_rng(0).standard_normal(t.d["tilen"] * 16)
+    x[n:] = 0.0
+    print(f"case {name}", flush=True)
+    assert np.array_equal(A.spmv(x), t.spmv(x))
+
+
+@pytest.mark.parametrize("tau", [0.0, 2.0 ** -11, 2.0 ** -4])
+def test_gpu_spmv_mixed_bit_exact(gpu, orc, tau):
+    for rp, ci, va in (gpu.gen_convdiff3d(20, 10.0, 1.0, 0), gpu.gen_poisson3d(24, 24, 24),
+                       gpu.gen_random_spd(20000, seed=1)):
+        m = len(rp) - 1
+        A = gpu.Matrix(rp, ci, va)
+        A.layout(tau)
+        t = orc.tiles(16, m, m, rp, ci, va)
+        codes = t.classes(tau)
+        x = np.random.default_rng(1).standard_normal(m)
+        print(f"case n={m} tau={tau} codes={np.bincount(codes, minlength=5).tolist()}", flush=True)
+        assert np.array_equal(A.spmv(x, mixed=True), t.spmv_mixed(codes, x))
+        assert np.array_equal(A.spmv(x, mixed=False), t.spmv(x))
+
+
+def _cg_case(gpu, orc, rp, ci, va, b, **opts):
+    A = gpu.Matrix(rp, ci, va)
+    x, rep = A.solve(b, method=gpu.MF_CG, tol=TOL, **opts)
+    xo, orep = orc.cg(rp, ci, va, b, TOL, 20000)
+    print(f"cg n={len(b)} gpu it={rep['iterations']} repl={rep['replacements']} oracle it={orep.iterations} "
+          f"relres={rep['relres_true']:.2e} ms={rep['ms_solve']:.2f}", flush=True)
+    assert rep["status"] == 0
+    assert rep["relres_true"] <= TOL and _true_relres(rp, ci, va, x, b) <= 1.5 * TOL
+    assert _iters_close(rep["iterations"], orep.iterations)
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-6
+    return rep
+
+
+def test_gpu_cg_fp64_small_configs(gpu, orc):
+    rp, ci, va = gpu.gen_laplace2d(256, 256)
+    _cg_case(gpu, orc, rp, ci, va, orc.csr_spmv(rp, ci, va, np.ones(len(rp) - 1)))
+    rp, ci, va = gpu.gen_poisson3d(32, 32, 32)
+    xs = gpu.gen_uniform(0, len(rp) - 1, -1.0, 1.0)
+    _cg_case(gpu, orc, rp, ci, va, orc.csr_spmv(rp, ci, va, xs))
+
+
+def test_gpu_cg_fp64_1138_bus_ragged_rows(gpu, orc):
+    # 1138 rows: the last tile row is partial and padded on the device
+    g = load_golden("1138_bus")
+    rp, ci, va = g["rowptr"], g["col"], g["val"]
+    b = orc.csr_spmv(rp, ci, va, np.ones(1138))
+    _cg_case(gpu, orc, rp, ci, va, b)
+
+
+def test_gpu_cg_mixed_lossless(gpu, orc):
+    rp, ci, va = gpu.gen_poisson3d(40, 40, 40)
+    xs = gpu.gen_uniform(0, len(rp) - 1, -1.0, 1.0)
+    b = orc.csr_spmv(rp, ci, va, xs)
+    rep = _cg_case(gpu, orc, rp, ci, va, b, precision=gpu.MF_MIXED, tau=0.0, rr_factor=1e-2)
+    assert rep["outer"] == 1
+
+
+def test_gpu_cg_grid_sizes_agree(gpu, orc):
+    # rows per thread 1, 2, 4, 8, 16 through the grid size; same answer
+    rp, ci, va = gpu.gen_poisson3d(32, 32, 64)
+    b = orc.csr_spmv(rp, ci, va, gpu.gen_uniform(2, len(rp) - 1, -1, 1))
+    A = gpu.Matrix(rp, ci, va)
+    seen = set()
+    for grid in (256, 64, 32, 16, 8):
+        x, rep = A.solve(b, method=gpu.MF_CG, tol=TOL, grid=grid)
+        print(f"grid={grid} rpt={rep['rows_per_thread']} it={rep['iterations']} relres={rep['relres_true']:.2e}",
+              flush=True)
+        assert rep["status"] == 0 and rep["relres_true"] <= TOL
+        seen.add(rep["rows_per_thread"])
+    assert len(seen) >= 4
+
+
+def test_gpu_cg_deterministic(gpu, orc):
+    rp, ci, va = gpu.gen_poisson3d(24, 24, 24)
+    b = orc.csr_spmv(rp, ci, va, gpu.gen_uniform(3, len(rp) - 1, -1, 1))
+    A = gpu.Matrix(rp, ci, va)
+    x1, r1 = A.solve(b, method=gpu.MF_CG, tol=TOL)
+    x2, r2 = A.solve(b, method=gpu.MF_CG, tol=TOL)
+    assert r1["iterations"] == r2["iterations"] and np.array_equal(x1, x2)
+
+
+def _bicg_case(gpu, orc, rp, ci, va, b, check_iters=True, **opts):
+    A = gpu.Matrix(rp, ci, va)
+    x, rep = A.solve(b, method=gpu.MF_BICGSTAB, tol=TOL, **opts)
+    xo, orep = orc.bicgstab(rp, ci, va, b, TOL, 20000)
+    print(f"bicgstab n={len(b)} gpu it={rep['iterations']} outer={rep['outer']} repl={rep['replacements']} "
+          f"oracle it={orep.iterations} relres={rep['relres_true']:.2e} ms={rep['ms_solve']:.2f}", flush=True)
+    assert rep["status"] == 0
+    assert rep["relres_true"] <= TOL and _true_relres(rp, ci, va, x, b) <= 1.5 * TOL
+    if check_iters:
+        assert _iters_close(rep["iterations"], orep.iterations)
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-6
+    return rep
+
+
+def test_gpu_bicgstab_fp64(gpu, orc):
+    rp, ci, va = gpu.gen_convdiff3d(24, 10.0, 1.0, 0)
+    b = orc.csr_spmv(rp, ci, va, gpu.gen_uniform(0, len(rp) - 1, -1, 1))
+    _bicg_case(gpu, orc, rp, ci, va, b)
+    g = load_golden("nontrivial_mc64_example")
+    A = sp.csr_matrix((g["val"], g["col"], g["rowptr"]), shape=(int(g["m"]), int(g["n"])))
+    b = A @ np.ones(A.shape[0])
+    xo, orep = orc.bicgstab(g["rowptr"], g["col"], g["val"], b, TOL, 20000)
+    if orep.converged:
+        _bicg_case(gpu, orc, g["rowptr"], g["col"], g["val"], b)
+
+
+@pytest.mark.parametrize("tau", [2.0 ** -24, 2.0 ** -11])
+def test_gpu_bicgstab_mixed_refinement(gpu, orc, tau):
+    rp, ci, va = gpu.gen_convdiff3d(24, 10.0, 1.0, 0)
+    b = orc.csr_spmv(rp, ci, va, gpu.gen_uniform(0, len(rp) - 1, -1, 1))
+    rep = _bicg_case(gpu, orc, rp, ci, va, b, check_iters=False, precision=gpu.MF_MIXED, tau=tau,
+                     rr_factor=1e-2)
+    assert rep["outer"] >= 1
+
+
+def test_gpu_solver_edge_cases(gpu, orc):
+    rp, ci, va = gpu.gen_poisson3d(16, 16, 16)
+    n = len(rp) - 1
+    A = gpu.Matrix(rp, ci, va)
+    x, rep = A.solve(np.zeros(n), method=gpu.MF_CG)
+    assert rep["status"] == 0 and np.all(x == 0)
+    x, rep = A.solve(np.ones(n), method=gpu.MF_CG, maxit=3)
+    assert rep["status"] == -21 and rep["iterations"] == 3  # MF_E_MAXIT
+    x, rep = A.solve(np.ones(n), method=gpu.MF_BICGSTAB, maxit=2)
+    assert rep["status"] == -21
+    # a tiny spin limit forces the bounded barrier wait to give up: every
+    # workgroup leaves through the abort word and the call reports a timeout
+    with pytest.raises(gpu.MFError) as e:
+        A.solve(np.ones(n), method=gpu.MF_CG, spin_limit=1)
+    assert e.value.status == -22
+    x, rep = A.solve(np.ones(n), method=gpu.MF_CG)  # the next launch starts clean
+    assert rep["status"] == 0 and rep["relres_true"] <= TOL
+
+
+def test_gpu_dropin_cg_solve_inc_csv(gpu, tmp_path):
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "mille-feillue_amd", "mf_cg")
+    mtx = os.path.join(GOLDEN, "1138_bus.mtx")
+    r = subprocess.run([exe, mtx, "16"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    row = (tmp_path / "cg_syncfree_amd_mi200.csv").read_text().strip().split(",")
+    # path,iterations,time,time,nnzR,l2_norm,time_format,index,each_block_nnz,norm
+    assert len(row) == 10 and row[0] == mtx and int(row[1]) > 0 and int(row[4]) == 4054
+    assert float(row[5]) <= TOL

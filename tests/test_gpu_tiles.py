@@ -1,0 +1,92 @@
+"""GPU tile builder and compute-layout parity (needs an MI355X).
+
+The reference tile layout built on the GPU must equal the reference's own
+Tile_create output (golden fixtures) and the oracle bit-for-bit; the nibble
+codec, the per-tile precision codes and the Memory-Cost classifier must match
+the oracle exactly."""
+import numpy as np
+import pytest
+
+from fixtures import golden_cases, golden_tiles, golden_variants, load_golden
+from orc import REF_ARRAYS
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("name", golden_cases())
+def test_gpu_tile_build_matches_reference(gpu, name):
+    g = load_golden(name)
+    for key, B, rowA in golden_variants(g):
+        print(f"case {name} {key}", flush=True)
+        A = gpu.Matrix(g["rowptr"], g["col"], g["val"], rowA=rowA, colA=int(g["n"]), B=B)
+        d = A.export()
+        ref = golden_tiles(g, key)
+        for k in REF_ARRAYS:
+            assert np.array_equal(d[k], ref[k]), (key, k)
+        assert np.array_equal(A.encode_cols(), g[f"{key}_encode"]), key
+        A.close()
+
+
+def _configs(mf):
+    out = {}
+    out["laplace2d_256"] = mf.gen_laplace2d(256, 256)
+    out["poisson3d_32"] = mf.gen_poisson3d(32, 32, 32)
+    out["convdiff27_20"] = mf.gen_convdiff3d(20, 10.0, 1.0, 0)
+    out["random_spd_30k"] = mf.gen_random_spd(30000, seed=0)
+    out["stencil_ln_3d"] = mf.gen_stencil_lognormal(3, 5000, 1.5, 1)
+    # one tile row with 16 x 400 entries: more than the LDS sort holds
+    rng = np.random.default_rng(9)
+    rows = [np.sort(rng.choice(20000, 400, replace=False)) if i < 16 else np.array([i]) for i in range(64)]
+    rp = np.zeros(65, np.int32)
+    rp[1:] = np.cumsum([len(r) for r in rows])
+    ci = np.concatenate(rows).astype(np.int32)
+    out["big_tile_row"] = (rp, ci, rng.standard_normal(len(ci)), 20000)
+    return out
+
+
+@pytest.mark.parametrize("B", [16, 8, 32])
+def test_gpu_tile_build_matches_oracle_on_configs(gpu, orc, B):
+    for name, c in _configs(gpu).items():
+        rp, ci, va = c[:3]
+        ncol = c[3] if len(c) > 3 else len(rp) - 1
+        m = len(rp) - 1
+        print(f"case {name} B={B} n={m} nnz={len(ci)}", flush=True)
+        A = gpu.Matrix(rp, ci, va, rowA=m, colA=ncol, B=B)
+        d = A.export()
+        t = orc.tiles(B, m, ncol, rp, ci, va)
+        for k in REF_ARRAYS:
+            assert np.array_equal(d[k], t.d[k]), (name, B, k)
+        A.close()
+
+
+@pytest.mark.parametrize("tau,mask", [(0.0, 28), (2.0 ** -11, 28), (2.0 ** -4, 28), (2.0 ** -24, 16), (0.5, 4)])
+def test_gpu_classes_match_oracle(gpu, orc, tau, mask):
+    for name in ("convdiff27_20", "poisson3d_32", "random_spd_30k"):
+        rp, ci, va = _configs(gpu)[name][:3]
+        m = len(rp) - 1
+        A = gpu.Matrix(rp, ci, va)
+        t = orc.tiles(16, m, m, rp, ci, va)
+        assert np.array_equal(A.classes(tau, mask), t.classes(tau, mask)), (name, tau, mask)
+        A.close()
+
+
+def test_gpu_mc_classifier_matches_oracle(gpu, orc):
+    rp, ci, va = gpu.gen_convdiff3d(16, 10.0, 1.0, 0)
+    va = va * 37.0  # spread the values over all four bins
+    m = len(rp) - 1
+    A = gpu.Matrix(rp, ci, va)
+    mb, counts = A.mc_classify()
+    c = orc.tiles(16, m, m, rp, ci, va).compact()
+    assert counts == c["mc_counts"] and mb == c["mc_memory_MB"]
+    assert all(x > 0 for x in counts)
+
+
+def test_gpu_layout_info(gpu):
+    rp, ci, va = gpu.gen_poisson3d(32, 32, 32)
+    A = gpu.Matrix(rp, ci, va)
+    A.layout(0.0)
+    info = A.info()
+    # 6 and -1 are exact in E4M3: the lossless mixed layout is all FP8
+    assert info["tiles_per_code"][2] == info["tilenum"]
+    assert info["nnz_per_code"][2] == len(ci)
+    assert info["bytes_alg_mixed"] < info["bytes_alg_fp64"]

@@ -1,0 +1,154 @@
+"""CPU tests of the product's host side: the C-ABI library loads and exports
+every declared symbol, and the host functions (Matrix Market reader, nibble
+codec, scans, reference-schedule preprocessing, generators) agree with the
+oracle and the golden fixtures.  No GPU call is made here."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from fixtures import GOLDEN, golden_cases, golden_tiles, golden_variants, load_golden, to_ref_struct
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    txt = open(os.path.join(ROOT, "include", "mf.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    names = re.findall(r"^[A-Za-z_][\w \*]*?\b([A-Za-z_]\w*)\s*\(", txt, flags=re.M)
+    return sorted(set(n for n in names if n not in ("if", "for", "while")))
+
+
+def test_header_declarations_are_exported(mf):
+    L = mf.lib()
+    decl = declared_functions()
+    assert len(decl) >= 35
+    missing = [n for n in decl if not hasattr(L, n)]
+    assert not missing, missing
+    assert sorted(mf.EXPORTS) == sorted(decl)
+
+
+@pytest.mark.parametrize("name", ["1138_bus", "ani1_nonsymm", "LFAT5"])
+def test_mmio_matches_reference_reader(mf, name):
+    rc, d = mf.mmio(os.path.join(GOLDEN, f"{name}.mtx"))
+    g = load_golden(name)
+    assert rc == 0 and d["nnz"] == int(g["nnz"]) and d["sym"] == int(g["sym"])
+    for k in ("rowptr", "col", "val"):
+        assert np.array_equal(d[k], g[k]), k
+
+
+def test_mmio_variants_and_errors(mf, orc, tmp_path):
+    p = tmp_path / "m.mtx"
+    # pattern, integer, complex (real part), comments, blank tail, hermitian
+    cases = [
+        "%%MatrixMarket matrix coordinate pattern general\n% c\n3 3 4\n1 1\n2 3\n3 1\n3 3\n",
+        "%%MatrixMarket matrix coordinate integer symmetric\n3 3 3\n1 1 7\n3 1 -2\n2 2 5\n\n",
+        "%%MatrixMarket matrix coordinate complex hermitian\n2 2 2\n1 1 1.5 0\n2 1 2.25e-1 -1\n",
+        "%%MatrixMarket MATRIX COORDINATE REAL GENERAL\n2 3 3\n1 3 1e300\n2 1 -0.1\n1 1 3.0000000000000004\n",
+    ]
+    for txt in cases:
+        p.write_text(txt)
+        rc1, a = mf.mmio(str(p))
+        rc2, b = orc.mmio(str(p))
+        assert rc1 == rc2 == 0
+        for k in ("rowptr", "col", "val"):
+            assert np.array_equal(a[k], b[k]), (txt, k)
+    # the banner token itself is compared case-sensitively (PC/mmio.h:449)
+    p.write_text("%%MATRIXMARKET matrix coordinate real general\n1 1 1\n1 1 1\n")
+    assert mf.mmio(str(p))[0] == orc.mmio(str(p))[0] == -2
+    assert mf.mmio(str(tmp_path / "missing.mtx"))[0] == -1
+    p.write_text("%%MatrixMarket matrix coordinate real weird\n1 1 1\n1 1 1\n")
+    assert mf.mmio(str(p))[0] == -2
+    p.write_text("%%MatrixMarket matrix coordinate real general\n")
+    assert mf.mmio(str(p))[0] == -4
+
+
+@pytest.mark.parametrize("name", golden_cases())
+def test_encode_decode_match_reference(mf, name):
+    g = load_golden(name)
+    for key, B, rowA in golden_variants(g):
+        cols = g[f"{key}_Tile_csr_Col"]
+        packed = mf.encode(cols)
+        assert np.array_equal(packed, g[f"{key}_encode"]), key
+        if B == 16:
+            assert np.array_equal(mf.decode(packed, len(cols)), cols)
+
+
+def test_scans_match_oracle(mf, orc):
+    import ctypes as C
+    rng = np.random.default_rng(5)
+    for n in (0, 1, 2, 17, 1000):
+        a = rng.integers(0, 300, n).astype(np.int32)
+        b, c = a.copy(), a.copy()
+        mf.lib().exclusive_scan(b.ctypes.data_as(C.POINTER(C.c_int)), n)
+        orc.L.orc_exclusive_scan(c.ctypes.data_as(C.POINTER(C.c_int)), n)
+        assert np.array_equal(b, c)
+        u = rng.integers(0, 256, n).astype(np.uint8)
+        v, w = u.copy(), u.copy()
+        mf.lib().exclusive_scan_char(v.ctypes.data_as(C.POINTER(C.c_ubyte)), n)
+        orc.L.orc_exclusive_scan_char(w.ctypes.data_as(C.POINTER(C.c_ubyte)), n)
+        assert np.array_equal(v, w)
+
+
+@pytest.mark.parametrize("name", golden_cases())
+def test_partition_and_compact_match_oracle(mf, orc, name):
+    g = load_golden(name)
+    for key, B, rowA in golden_variants(g):
+        if B != 16:
+            continue
+        d = golden_tiles(g, key)
+        ref = to_ref_struct(mf, B, rowA, int(g["n"]), d)
+        ot = orc.tiles(B, rowA, int(g["n"]), g["rowptr"], g["col"], g["val"])
+        oc = ot.compact()
+        pc = mf.compact_rows(ref)
+        for k in ("nonzero_row_new", "blockrowid_new", "blockcsr_ptr_new", "block_signal", "tile_rowidx"):
+            assert np.array_equal(pc[k], oc[k]), (key, k)
+        for mode, bnnz, sm in ((0, 16, 6912), (1, 16, 6912), (1, 16, 8192), (0, 64, 6912)):
+            pp = mf.partition(ref, len(g["col"]), bnnz, sm, mode)
+            op = ot.partition(len(g["col"]), bnnz, sm, mode)
+            for k in op:
+                assert np.array_equal(np.asarray(pp[k]), np.asarray(op[k])), (key, mode, k)
+
+
+def test_partition_known_answer_1138_bus(mf):
+    g = load_golden("1138_bus")
+    d = golden_tiles(g, "b16_trunc")
+    ref = to_ref_struct(mf, 16, int(g["b16_trunc_rowA"]), int(g["n"]), d)
+    assert mf.partition(ref, int(g["nnz"]), 16, 6912, 0)["index"] == 149
+
+
+def test_generators_shapes_and_properties(mf):
+    import scipy.sparse as sp
+    rp, ci, va = mf.gen_laplace2d(256, 256)
+    assert len(rp) - 1 == 65536 and rp[-1] == 326656  # 2D 5-point, 256^2
+    rp, ci, va = mf.gen_poisson3d(16, 16, 16)
+    assert rp[-1] == 16 ** 3 * 7 - 6 * 16 ** 2
+    A = sp.csr_matrix((va, ci, rp))
+    assert abs(A - A.T).max() == 0 and np.all(A.diagonal() == 6.0)
+    m = 12
+    rp, ci, va = mf.gen_convdiff3d(m, 10.0, 1.0, 0)
+    assert rp[-1] == (3 * m - 2) ** 3  # 27-point: nnz = (3m-2)^3
+    A = sp.csr_matrix((va, ci, rp))
+    off = abs(A).sum(axis=1).A1 - abs(A.diagonal())
+    assert np.all(A.diagonal() >= off * (1 - 1e-14))  # weakly diagonally dominant M-matrix
+    assert abs(A - A.T).max() > 0  # convection makes it nonsymmetric
+    rp, ci, va = mf.gen_random_spd(2000, seed=0)
+    A = sp.csr_matrix((va, ci, rp))
+    assert abs(A - A.T).max() == 0
+    off = abs(A).sum(axis=1).A1 - abs(A.diagonal())
+    assert np.allclose(A.diagonal(), 1.05 * off)
+    for dim in (2, 3):
+        rp, ci, va = mf.gen_stencil_lognormal(dim, 1000, 1.5, 3)
+        A = sp.csr_matrix((va, ci, rp))
+        assert A.shape[0] >= 1000 and abs(A - A.T).max() < 1e-12
+    # the portable stream is deterministic and uniform in [lo, hi)
+    u1, u2 = mf.gen_uniform(0, 1000, -1, 1), mf.gen_uniform(0, 1000, -1, 1)
+    assert np.array_equal(u1, u2) and u1.min() >= -1 and u1.max() < 1 and abs(u1.mean()) < 0.1
+    z = mf.gen_normal(0, 20000)
+    assert abs(z.mean()) < 0.03 and abs(z.std() - 1) < 0.03
+
+
+def test_cli_binaries_built():
+    for exe in ("mf_cg", "mf_bicgstab"):
+        assert os.access(os.path.join(ROOT, "mille-feillue_amd", exe), os.X_OK)

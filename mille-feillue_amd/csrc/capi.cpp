@@ -102,7 +102,7 @@ void mf_opts_default(mf_opts *o)
     o->ir_inner_tol = 1e-6;
     o->grid = 0;
     o->block = 0;
-    o->spin_limit = 1LL << 31;
+    o->spin_limit = 1LL << 26;  // a few seconds of polling
 }
 
 int mf_matrix_create(int rowA, int colA, int nnz, const int *rowptr, const int *colidx,

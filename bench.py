@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Benchmark of the headline workload on one MI355X.
+
+Workload: 3D 7-point Poisson on a 128^3 grid (n = 2,097,152, nnz = 14,581,760),
+b = A x*, x* ~ U(-1, 1) from the library's seeded stream (seed 0), x0 = 0, CG to
+true relative residual 1e-10.  One step = one full solve from x0 = 0 with b and
+x resident in HBM (device buffers owned by torch, passed to mf_solve_device).
+
+Reported: time to tolerance of the per-tile mixed-precision solve (the value),
+the same persistent kernel run FP64-only, their ratio, the standalone tiled
+SpMV kernel's bandwidth, the roofline of the dominant kernel (the persistent
+CG kernel), and the oracle's tile-SpMV CG timed on the host cores.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "time to rel. residual 1e-10 (ms), SpMV HBM GB/s; mixed vs FP64-only speedup"
+HBM_PEAK = 8.0e12
+TOL = 1e-10
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(rp, ci, va, b, gpu_iters):
+    """The oracle's tile-SpMV CG (FP64, OpenMP over the host cores) run to the
+    same tolerance on the same system."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from orc import Oracle
+
+    O = Oracle()
+    n = len(b)
+    cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16), 64)
+    T = O.tiles(16, n, n, rp, ci, va)
+    t0 = time.perf_counter()
+    x, rep = O.cg(rp, ci, va, b, TOL, 20000, tiles=T, nthreads=cores)
+    dt = (time.perf_counter() - t0) * 1e3
+    return {
+        "value": round(dt, 3), "unit": "ms", "cores": cores, "kind": "port",
+        "sample": f"full solve: oracle tile-SpMV CG (FP64, {cores} OpenMP threads) to true relres "
+                  f"{TOL:g} on the same 128^3 system, {rep.iterations} iterations "
+                  f"(GPU FP64: {gpu_iters}), relres {rep.relres_true:.2e}",
+        "iterations": rep.iterations,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--grid", type=int, default=128)
+    ap.add_argument("--rr", type=float, default=1e-2, help="residual replacement factor (mixed)")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU sharding is a design in DESIGN.md, not a runnable path", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from __graft_entry__ import load_package
+
+    mf = load_package()
+    if not torch.cuda.is_available():
+        raise RuntimeError("bench.py needs the GPU")
+    rc = mf.lib().mf_init(0)
+    if rc != 0:
+        raise RuntimeError(mf.lib().mf_last_error().decode())
+
+    g = args.grid
+    t0 = time.perf_counter()
+    rp, ci, va = mf.gen_poisson3d(g, g, g)
+    n = len(rp) - 1
+    xs = mf.gen_uniform(0, n, -1.0, 1.0)
+    import scipy.sparse as sp
+    b = sp.csr_matrix((va, ci, rp), shape=(n, n)) @ xs
+    log(f"generated n={n} nnz={len(ci)} in {time.perf_counter() - t0:.1f}s")
+    t0 = time.perf_counter()
+    A = mf.Matrix(rp, ci, va)
+    A.layout(0.0)  # lossless per-tile classes
+    info = A.info()
+    log(f"tiles built on the GPU in {time.perf_counter() - t0:.1f}s: tilenum={info['tilenum']} "
+        f"codes={info['tiles_per_code']}")
+
+    dev = torch.device("cuda:0")
+    npad = info["tilem"] * info["B"]
+    d_b = torch.zeros(npad, dtype=torch.float64, device=dev)
+    d_b[:n] = torch.from_numpy(b).to(dev)
+    d_x = torch.zeros(npad, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+
+    def run(precision, steps, warmup):
+        opts = dict(method=mf.MF_CG, tol=TOL, precision=precision)
+        if precision == mf.MF_MIXED:
+            opts.update(tau=0.0, rr_factor=args.rr)
+        for _ in range(warmup):
+            mf_rep = A.solve_device(d_b.data_ptr(), d_x.data_ptr(), **opts)
+        torch.cuda.synchronize()
+        reps = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            reps.append(A.solve_device(d_b.data_ptr(), d_x.data_ptr(), **opts))
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) * 1e3 / steps
+        for r in reps:
+            if r["status"] != 0 or r["relres_true"] > TOL:
+                raise RuntimeError(f"solve did not reach tolerance: {r}")
+        return reps, wall
+
+    reps64, wall64 = run(mf.MF_FP64, args.steps, args.warmup)
+    x64 = d_x[:n].cpu().numpy()
+    repsmx, wallmx = run(mf.MF_MIXED, args.steps, args.warmup)
+    xmx = d_x[:n].cpu().numpy()
+    ms64 = statistics.median(r["ms_solve"] for r in reps64)
+    msmx = statistics.median(r["ms_solve"] for r in repsmx)
+    it64, itmx = reps64[0]["iterations"], repsmx[0]["iterations"]
+    err64 = float(np.linalg.norm(x64 - xs) / np.linalg.norm(xs))
+    errmx = float(np.linalg.norm(xmx - xs) / np.linalg.norm(xs))
+    log(f"fp64: {ms64:.3f} ms, {it64} it; mixed: {msmx:.3f} ms, {itmx} it, "
+        f"{repsmx[0]['replacements']} true-residual passes")
+
+    # standalone SpMV kernel
+    spmv64 = A.spmv_bench(50, mixed=False)
+    spmvmx = A.spmv_bench(50, mixed=True)
+    b64, bmx = info["bytes_alg_fp64"], info["bytes_alg_mixed"]
+
+    # dominant kernel: the persistent CG launch; one SpMV's algorithmic bytes
+    # per Krylov iteration (true-residual passes not counted)
+    ach = itmx * bmx / (msmx * 1e-3)
+    roof = {"bound": "hbm", "achieved": round(ach / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK, 4), "traffic": None,
+            "kernel": "k_cg<16,true,512,16> (persistent mixed CG, one launch per solve)",
+            "bytes_per_unit": bmx, "units_per_launch": itmx, "launch_ms": round(msmx, 4)}
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(rp, ci, va, b, it64)
+        log(f"cpu: {cpu['value']} ms")
+
+    out = {
+        "metric": METRIC,
+        "value": round(msmx, 4),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wallmx, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 vectors/dots; e4m3 tiles, f32 accumulation",
+        "data": "synthetic",
+        "config": {
+            "workload": f"3D 7-point Poisson {g}^3, CG to true relres 1e-10, b = A x*, x* ~ U(-1,1) seed 0",
+            "n": n, "nnz": int(len(ci)), "tile": 16, "precision": "per-tile mixed (lossless: all tiles E4M3)",
+            "iterations_mixed": itmx, "iterations_fp64": it64,
+            "replacements_mixed": repsmx[0]["replacements"],
+            "grid": repsmx[0]["grid"], "block": repsmx[0]["block"],
+            "rows_per_thread": repsmx[0]["rows_per_thread"],
+        },
+        "fp64_ms": round(ms64, 4),
+        "fp64_ms_per_step_wall": round(wall64, 4),
+        "speedup_mixed_vs_fp64": round(ms64 / msmx, 3),
+        "relres_true": {"fp64": reps64[0]["relres_true"], "mixed": repsmx[0]["relres_true"]},
+        "err_vs_xstar": {"fp64": err64, "mixed": errmx},
+        "spmv": {
+            "fp64": {"ms": round(spmv64, 5), "GB/s": round(b64 / spmv64 / 1e6, 1),
+                     "frac": round(b64 / (spmv64 * 1e-3) / HBM_PEAK, 4), "bytes_alg": b64},
+            "mixed": {"ms": round(spmvmx, 5), "GB/s": round(bmx / spmvmx / 1e6, 1),
+                      "frac": round(bmx / (spmvmx * 1e-3) / HBM_PEAK, 4), "bytes_alg": bmx},
+        },
+        "solver_GBps": {"fp64": round(it64 * b64 / ms64 / 1e6, 1), "mixed": round(ach / 1e9, 1)},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -115,6 +115,7 @@ def lib():
     L.mf_spmv_bench.argtypes = [C.c_void_p, C.c_int, C.c_int, _d]
     L.mf_opts_default.argtypes = [C.POINTER(Opts)]
     L.mf_solve.argtypes = [C.c_void_p, _d, _d, C.POINTER(Opts), C.POINTER(Report)]
+    L.mf_solve_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(Opts), C.POINTER(Report)]
     L.mf_mc_classify.argtypes = [C.c_void_p, C.POINTER(C.c_longlong)]
     L.mf_mc_classify.restype = C.c_double
     L.mf_compact_rows.argtypes = [C.POINTER(RefTiles), C.POINTER(Compact)]
@@ -274,6 +275,13 @@ class Matrix:
         x = np.zeros(self.rowA, np.float64)
         _check(lib().mf_solve(self.h, _p(bb, C.c_double), _p(x, C.c_double), C.byref(o), C.byref(rep)))
         return x, rep.as_dict()
+
+    def solve_device(self, d_b, d_x, **opts):
+        """Solve with device pointers (tilem*B doubles each, b zero padded)."""
+        o = default_opts(**opts)
+        rep = Report()
+        _check(lib().mf_solve_device(self.h, C.c_void_p(d_b), C.c_void_p(d_x), C.byref(o), C.byref(rep)))
+        return rep.as_dict()
 
     def mc_classify(self):
         cnt = (C.c_longlong * 4)()

@@ -206,6 +206,31 @@ int mf_matrix_export(mf_matrix m, mf_ref_tiles *t)
     }
     if (m->csrptrlen)
         MF_HIP(hipMemcpy(t->Blockcsr_Ptr, m->d_Blockcsr_Ptr, m->csrptrlen, hipMemcpyDeviceToHost));
+    // The reference copies a tile row by row from the uint8 row starts
+    // (PC/csr2block.h:239-253): row ri covers [start[ri], next start or nnz).
+    // When the starts wrapped or were never scanned, some positions are not
+    // covered and keep the zero of the reference's memset.
+    for (int bi = 0; bi < m->tilem; bi++) {
+        const int rowlen = bi == m->tilem - 1 ? m->rowA - (m->tilem - 1) * m->B : m->B;
+        for (int tid = t->tile_ptr[bi]; tid < t->tile_ptr[bi + 1]; tid++) {
+            const int nz = t->tile_nnz[tid + 1] - t->tile_nnz[tid];
+            if (nz <= 255 && rowlen > 1) continue;
+            const int off = t->csr_offset[tid], po = t->csrptr_offset[tid];
+            for (int k = 0; k < nz; k++) {
+                bool cov = false;
+                for (int ri = 0; ri < rowlen && !cov; ri++) {
+                    const int a = t->Blockcsr_Ptr[po + ri];
+                    const int b = ri == rowlen - 1 ? nz : t->Blockcsr_Ptr[po + ri + 1];
+                    cov = a <= k && k < b;
+                }
+                if (!cov) {
+                    t->Blockcsr_Val[off + k] = 0.0;
+                    t->Blockcsr_Val_Low[off + k] = 0.0f;
+                    t->Tile_csr_Col[off + k] = 0;
+                }
+            }
+        }
+    }
     return MF_OK;
 }
 

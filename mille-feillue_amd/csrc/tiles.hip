@@ -8,9 +8,10 @@
 // segments up to SORT_CAP entries, in global scratch above that -- and every
 // Blockcsr array of the tile row is then written at rowptr[first row] + k.
 // Row starts are the uint8 exclusive scan of the per-row counts
-// (PC/csr2block.h:85, :237); where duplicates wrap them, the reference copy
-// loop (:239-253) leaves some positions unwritten and the builder zeroes the
-// same positions.
+// (PC/csr2block.h:85, :237).  Where those wrap (duplicate entries) or the
+// scan is a no-op (a one-row tile row), the reference copy loop (:239-253)
+// leaves positions unwritten; the device keeps every value (the compute
+// layout is built from them) and mf_matrix_export zeroes those positions.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -289,14 +290,16 @@ __global__ __launch_bounds__(256) void k_fill(const int *__restrict__ rowptr, co
     }
 }
 
-// One thread per tile: uint8 row starts, nnz mod 256, row-slot offsets, and
-// the reference copy-loop coverage for tiles whose starts wrapped.
+// One thread per tile: uint8 row starts, nnz mod 256 and row-slot offsets.
+// The row starts are the reference's in-place exclusive scan of uint8
+// per-row counts (PC/csr2block.h:85, :237), including its no-op for a
+// single-row tile row (PC/utils.h:50).  The values stay complete on the
+// device; the export applies the reference copy loop's coverage.
 template <int B>
 __global__ __launch_bounds__(256) void k_tiles(const int *__restrict__ tile_ptr, int tilem, int tilenum,
                                                int rowA, const int *__restrict__ tile_start,
                                                const uint8_t *__restrict__ rowtmp, uint8_t *bptr,
-                                               uint8_t *blknnznnz, int *csrptr_offset, double *bval,
-                                               float *bval_low, uint8_t *bcol)
+                                               uint8_t *blknnznnz, int *csrptr_offset)
 {
     const int t = blockIdx.x * 256 + threadIdx.x;
     const int last = tile_ptr[tilem - 1];
@@ -308,17 +311,16 @@ __global__ __launch_bounds__(256) void k_tiles(const int *__restrict__ tile_ptr,
     if (t >= tilenum) return;
     int lo = 0, hi = tilem;  // tile row: largest bi with tile_ptr[bi] <= t
     while (hi - lo > 1) {
-        int mid = (lo + hi) >> 1;
+        const int mid = (lo + hi) >> 1;
         if (tile_ptr[mid] <= t) lo = mid;
         else hi = mid;
     }
-    const int bi = lo;
-    const int rowlen = bi == tilem - 1 ? rowlen_last : B;
+    const int rowlen = lo == tilem - 1 ? rowlen_last : B;
     const int s = tile_start[t], e = tile_start[t + 1], nz = e - s;
     const int po = t <= last ? B * t : B * last + rowlen_last * (t - last);
     csrptr_offset[t] = po;
     blknnznnz[t] = (uint8_t)nz;
-    int cnt[B];
+    unsigned cnt[B];
 #pragma unroll
     for (int r = 0; r < B; r++) cnt[r] = 0;
     for (int k = s; k < e; k++) {
@@ -326,27 +328,15 @@ __global__ __launch_bounds__(256) void k_tiles(const int *__restrict__ tile_ptr,
 #pragma unroll
         for (int r = 0; r < B; r++) cnt[r] += rr == r;
     }
-    unsigned st8[B];
+    if (rowlen == 1) {
+        bptr[po] = (uint8_t)cnt[0];
+        return;
+    }
     unsigned run = 0;
 #pragma unroll
     for (int r = 0; r < B; r++) {
-        st8[r] = run & 255u;
-        run += (unsigned)(cnt[r] & 255);
-    }
-    for (int r = 0; r < rowlen; r++) bptr[po + r] = (uint8_t)st8[r];
-    if (nz > 255) {
-        for (int k = 0; k < nz; k++) {
-            bool cov = false;
-            for (int r = 0; r < rowlen && !cov; r++) {
-                const int a = (int)st8[r], b = r == rowlen - 1 ? nz : (int)st8[r + 1];
-                cov = a <= k && k < b;
-            }
-            if (!cov) {
-                bval[s + k] = 0.0;
-                bval_low[s + k] = 0.0f;
-                bcol[s + k] = 0;
-            }
-        }
+        if (r < rowlen) bptr[po + r] = (uint8_t)run;
+        run = (run + (cnt[r] & 255u)) & 255u;
     }
 }
 
@@ -629,8 +619,7 @@ static int build_reference_tiles_B(mf_matrix m)
     if (tilem > 0)
         k_tiles<B><<<(tilenum + 1 + 255) / 256, 256, 0, st>>>(
             m->d_tile_ptr, tilem, tilenum, m->rowA, m->d_tile_start, m->d_rowtmp, m->d_Blockcsr_Ptr,
-            m->d_blknnznnz, m->d_csrptr_offset, m->d_Blockcsr_Val, m->d_Blockcsr_Val_Low,
-            m->d_Tile_csr_Col);
+            m->d_blknnznnz, m->d_csrptr_offset);
     MF_HIP(hipGetLastError());
     MF_HIP(hipFreeAsync(tcount, st));
     MF_HIP(hipFreeAsync(perm, st));

@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -584,7 +585,23 @@ static int launch_inner(mf_matrix m, const Layout &L, int method, const double *
     k_reset_ctl<<<1, 64, 0, st>>>(w.bar, w.ctl);
     MF_HIP(hipGetLastError());
     void *args[] = {&A};
-    MF_HIP(hipLaunchCooperativeKernel((const void *)kc.fn, dim3(G), dim3(kc.tpb), args, 0, st));
+    // Cooperative launch guarantees that all G workgroups are resident.  Some
+    // runtimes refuse it (e.g. while another process holds the device); then
+    // the occupancy API must show G resident workgroups and a plain launch is
+    // used.  Either way the barrier waits are bounded and the abort word is
+    // sticky, so a workgroup that could not start in time finds the abort
+    // raised and every workgroup exits.
+    static const bool no_coop = std::getenv("MF_NO_COOP") != nullptr;
+    hipError_t le = no_coop ? hipErrorNotSupported
+                            : hipLaunchCooperativeKernel((const void *)kc.fn, dim3(G), dim3(kc.tpb), args, 0, st);
+    if (le != hipSuccess) {
+        (void)hipGetLastError();
+        int per_cu = 0;
+        MF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)kc.fn, kc.tpb, 0));
+        if (per_cu * device_cu_count() < G)
+            return fail(MF_E_UNSUPPORTED, "grid does not fit the device for a persistent launch");
+        MF_HIP(hipLaunchKernel((const void *)kc.fn, dim3(G), dim3(kc.tpb), args, 0, st));
+    }
     if (grid_out) *grid_out = G;
     if (block_out) *block_out = kc.tpb;
     if (rpt_out) *rpt_out = kc.rpt;

@@ -639,12 +639,50 @@ int build_reference_tiles(mf_matrix m)
     }
 }
 
+// The reference's column array: tile-local columns with the positions its
+// copy loop leaves unwritten set to 0 (tiles whose uint8 row starts wrapped,
+// or tiles of a one-row last tile row).  One thread per tile.
+template <int B>
+__global__ __launch_bounds__(256) void k_ref_cols(const int *__restrict__ tile_ptr, int tilem, int tilenum,
+                                                  int rowA, const int *__restrict__ tile_start,
+                                                  const int *__restrict__ csrptr_offset,
+                                                  const uint8_t *__restrict__ bptr,
+                                                  const uint8_t *__restrict__ cols, uint8_t *out)
+{
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= tilenum) return;
+    const int s = tile_start[t], nz = tile_start[t + 1] - s;
+    const int rowlen = t >= tile_ptr[tilem - 1] ? rowA - (tilem - 1) * B : B;
+    const bool holes = nz > 255 || rowlen == 1;
+    const int po = csrptr_offset[t];
+    for (int k = 0; k < nz; k++) {
+        bool cov = !holes;
+        for (int ri = 0; ri < rowlen && !cov; ri++) {
+            const int a = bptr[po + ri], b = ri == rowlen - 1 ? nz : bptr[po + ri + 1];
+            cov = a <= k && k < b;
+        }
+        out[s + k] = cov ? cols[s + k] : (uint8_t)0;
+    }
+}
+
 int encode_cols_device(mf_matrix m, uint8_t *d_out)
 {
     hipStream_t st = stream();
     long long n = m->csrsize, nb = (n + 1) / 2;
-    if (nb > 0) k_encode<<<(unsigned)((nb + 255) / 256), 256, 0, st>>>(m->d_Tile_csr_Col, d_out, n);
+    if (nb <= 0) return MF_OK;
+    uint8_t *cols = nullptr;
+    MF_HIP(hipMallocAsync((void **)&cols, (size_t)n, st));
+    const unsigned g = (unsigned)((m->tilenum + 255) / 256);
+#define MF_REFCOLS(BB)                                                                                 \
+    k_ref_cols<BB><<<g, 256, 0, st>>>(m->d_tile_ptr, m->tilem, m->tilenum, m->rowA, m->d_tile_start, \
+                                      m->d_csrptr_offset, m->d_Blockcsr_Ptr, m->d_Tile_csr_Col, cols)
+    if (m->B == 8) MF_REFCOLS(8);
+    else if (m->B == 16) MF_REFCOLS(16);
+    else MF_REFCOLS(32);
+#undef MF_REFCOLS
+    k_encode<<<(unsigned)((nb + 255) / 256), 256, 0, st>>>(cols, d_out, n);
     MF_HIP(hipGetLastError());
+    MF_HIP(hipFreeAsync(cols, st));
     MF_HIP(hipStreamSynchronize(st));
     return MF_OK;
 }

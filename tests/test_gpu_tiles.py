@@ -72,7 +72,7 @@ def test_gpu_classes_match_oracle(gpu, orc, tau, mask):
 
 def test_gpu_mc_classifier_matches_oracle(gpu, orc):
     rp, ci, va = gpu.gen_convdiff3d(16, 10.0, 1.0, 0)
-    va = va * 37.0  # spread the values over all four bins
+    va = va * np.exp(np.random.default_rng(4).uniform(0.0, 9.0, len(va)))  # spread over all four bins
     m = len(rp) - 1
     A = gpu.Matrix(rp, ci, va)
     mb, counts = A.mc_classify()

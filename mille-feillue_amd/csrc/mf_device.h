@@ -132,14 +132,26 @@ __device__ __forceinline__ void slab_spmv(const DevL &L, const RowCtx &c, int2 h
     }
 }
 
+// Context of tile row tr from headers already at hand (t0, nt, slab
+// headers); only the tile-column word of the lane's slot is loaded.
+template <bool IDX8>
+__device__ __forceinline__ RowCtx row_ctx_from(const DevL &L, int t0, int nt, int4 h01, int4 h23, int r)
+{
+    RowCtx c;
+    c.t0 = t0;
+    c.tc_reg = (IDX8 && r < nt) ? L.stc[t0 + r] : 0;
+    c.h01 = h01;
+    c.h23 = h23;
+    return c;
+}
+
 // y_r of tile row tr for lane r: FP64 tiles accumulate in FP64 over g64, then
 // FP32, FP16 and FP8 tiles (in that order) accumulate in FP32 over g32.
 template <int B, bool IDX8, int U = 8>
-__device__ __forceinline__ double tile_row_spmv(const DevL &L, int tr, int r, int lanebase,
-                                                const double *__restrict__ g64,
-                                                const float *__restrict__ g32)
+__device__ __forceinline__ double tile_row_spmv_c(const DevL &L, const RowCtx &c, int r, int lanebase,
+                                                  const double *__restrict__ g64,
+                                                  const float *__restrict__ g32)
 {
-    const RowCtx c = row_ctx<B, IDX8>(L, tr, r);
     double a64 = 0.0;
     float a32 = 0.0f;
     if (c.h01.y) slab_spmv<B, IDX8, 0, U>(L, c, make_int2(c.h01.x, c.h01.y), r, lanebase, g64, a64);
@@ -147,6 +159,14 @@ __device__ __forceinline__ double tile_row_spmv(const DevL &L, int tr, int r, in
     if (c.h23.y) slab_spmv<B, IDX8, 2, U>(L, c, make_int2(c.h23.x, c.h23.y), r, lanebase, g32, a32);
     if (c.h23.w) slab_spmv<B, IDX8, 3, U>(L, c, make_int2(c.h23.z, c.h23.w), r, lanebase, g32, a32);
     return a64 + (double)a32;
+}
+
+template <int B, bool IDX8, int U = 8>
+__device__ __forceinline__ double tile_row_spmv(const DevL &L, int tr, int r, int lanebase,
+                                                const double *__restrict__ g64,
+                                                const float *__restrict__ g32)
+{
+    return tile_row_spmv_c<B, IDX8, U>(L, row_ctx<B, IDX8>(L, tr, r), r, lanebase, g64, g32);
 }
 
 // FP64-accumulated product of NV gathered vectors at once (true residuals and
@@ -169,14 +189,20 @@ __device__ __forceinline__ void slab_spmv_f64(const DevL &L, const RowCtx &c, in
 }
 
 template <int B, bool IDX8, int NV>
-__device__ __forceinline__ void tile_row_spmv_f64(const DevL &L, int tr, int r, int lanebase,
-                                                  const double *const (&g)[NV], double (&acc)[NV])
+__device__ __forceinline__ void tile_row_spmv_f64_c(const DevL &L, const RowCtx &c, int r, int lanebase,
+                                                    const double *const (&g)[NV], double (&acc)[NV])
 {
-    const RowCtx c = row_ctx<B, IDX8>(L, tr, r);
     if (c.h01.y) slab_spmv_f64<B, IDX8, 0, NV>(L, c, make_int2(c.h01.x, c.h01.y), r, lanebase, g, acc);
     if (c.h01.w) slab_spmv_f64<B, IDX8, 1, NV>(L, c, make_int2(c.h01.z, c.h01.w), r, lanebase, g, acc);
     if (c.h23.y) slab_spmv_f64<B, IDX8, 2, NV>(L, c, make_int2(c.h23.x, c.h23.y), r, lanebase, g, acc);
     if (c.h23.w) slab_spmv_f64<B, IDX8, 3, NV>(L, c, make_int2(c.h23.z, c.h23.w), r, lanebase, g, acc);
+}
+
+template <int B, bool IDX8, int NV>
+__device__ __forceinline__ void tile_row_spmv_f64(const DevL &L, int tr, int r, int lanebase,
+                                                  const double *const (&g)[NV], double (&acc)[NV])
+{
+    tile_row_spmv_f64_c<B, IDX8, NV>(L, row_ctx<B, IDX8>(L, tr, r), r, lanebase, g, acc);
 }
 
 // ---------------------------------------------------------- wave64 helpers
@@ -215,22 +241,32 @@ __device__ __forceinline__ int ld_agent(const int *p)
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Grid barrier over a monotone arrival counter (never reset inside a launch,
-// so there is no reset race): every block adds 1 per barrier and waits for
-// target = barriers_passed * gridDim.  Writes before the barrier are released
-// at agent scope and acquired after it.  The wait is bounded: after
+// Grid barrier over monotone arrival counters (never reset inside a launch,
+// so there is no reset race).  Blocks are split into 8 groups by blockIdx % 8
+// (the XCD the dispatcher deals them to); each block adds 1 to its group's
+// counter, the group's last arriver adds 1 to the global counter, and every
+// block waits for the global counter to reach barriers_passed * groups.
+// Writes before the barrier are released at agent scope and acquired after
+// it.  Spreading the arrivals over 9 counters instead of one is what keeps
+// a barrier of 256 blocks at a few microseconds.  The wait is bounded: after
 // spin_limit polls the block raises the abort word, and every waiter leaves
 // when it sees it.  Returns false on abort (the caller exits the kernel).
-__device__ __forceinline__ bool grid_barrier(unsigned *bar, int *abort_word, unsigned target,
+// Counter layout: bar[0] global, bar[32 * (1 + x)] group x (separate lines).
+__device__ __forceinline__ bool grid_barrier(unsigned *bar, int *abort_word, unsigned ep,
                                              long long spin_limit, int *s_flag)
 {
     __syncthreads();
     if (threadIdx.x == 0) {
-        __threadfence();
-        __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned G = gridDim.x, x = blockIdx.x & 7u;
+        const unsigned groups = G < 8u ? G : 8u;
+        const unsigned nx = (G >> 3) + (x < (G & 7u) ? 1u : 0u);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const unsigned old =
+            __hip_atomic_fetch_add(bar + 32 * (1 + x), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (old + 1u == ep * nx) __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         int ab = 0;
         long long spins = 0;
-        while (ld_agent(bar) < target) {
+        while (ld_agent(bar) < ep * groups) {
             if (ld_agent(abort_word)) { ab = 1; break; }
             if (++spins > spin_limit) {
                 __hip_atomic_store(abort_word, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -239,7 +275,7 @@ __device__ __forceinline__ bool grid_barrier(unsigned *bar, int *abort_word, uns
             }
             __builtin_amdgcn_s_sleep(1);
         }
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         *s_flag = ab;
     }
     __syncthreads();

@@ -85,6 +85,8 @@ __global__ void k_axpy(double *__restrict__ x, const double *__restrict__ d, lon
 }
 
 // --------------------------------------------------------- solver kernels
+constexpr int BAR_WORDS = 32 * 9;  // barrier counters: global + 8 groups
+
 struct SolveArgs {
     DevL L;
     int tilem, G, trw;
@@ -113,7 +115,7 @@ struct Reducer {
     __device__ bool barrier(const SolveArgs &A)
     {
         ++ep;
-        return grid_barrier(A.bar, A.ctl, ep * (unsigned)A.G, A.spin_limit, s_flag);
+        return grid_barrier(A.bar, A.ctl, ep, A.spin_limit, s_flag);
     }
 
     // Sums v over the grid; every workgroup obtains the same bits.
@@ -157,8 +159,8 @@ struct Reducer {
 };
 
 // slices per load batch inside the persistent kernels: 8 keeps enough loads
-// in flight per wave; 16 rows per thread leave registers for 4 only
-__host__ __device__ constexpr int spmv_u(int rpt) { return rpt >= 16 ? 4 : 8; }
+// in flight per wave
+__host__ __device__ constexpr int spmv_u(int rpt) { return rpt >= 32 ? 4 : 8; }
 
 __device__ __forceinline__ bool finite_all(double a, double b) { return isfinite(a) && isfinite(b); }
 
@@ -191,6 +193,25 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
     const int grp = tid / B, lane = tid % B, lanebase = (tid & 63) & ~(B - 1);
     const int tr0 = chunk * A.trw, tr1 = min(A.tilem, tr0 + A.trw);
     double xr[RPT], pr[RPT];
+    // headers of the block's tile rows, loaded once: the per-iteration SpMV
+    // then issues the tile-column word and the slab loads together
+    constexpr int TRW = RPT * GPB;
+    __shared__ int4 s_h[2 * TRW];
+    __shared__ int s_t0[TRW + 1];
+    for (int q = tid; q <= TRW; q += TPB) {
+        const int tr = tr0 + q;
+        if (q < TRW && tr < tr1) {
+            const int4 *h = reinterpret_cast<const int4 *>(A.L.trh + (size_t)tr * 4);
+            s_h[2 * q] = h[0];
+            s_h[2 * q + 1] = h[1];
+        }
+        s_t0[q] = A.L.tile_ptr[min(tr, A.tilem)];
+    }
+    __syncthreads();
+    auto ctx = [&](int i) {
+        const int q = i * GPB + grp;
+        return row_ctx_from<IDX8>(A.L, s_t0[q], s_t0[q + 1] - s_t0[q], s_h[2 * q], s_h[2 * q + 1], lane);
+    };
     int rbase = tr0 * B + tid;  // row of step i is rbase + i * TPB
     auto tile_row = [&](int i) { return tr0 + i * GPB + grp; };
     auto row_of = [&](int i) { return (size_t)(rbase + i * TPB); };
@@ -253,7 +274,7 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
                 if (tr < tr1) {
                     const double *g[2] = {A.h64, A.q64};
                     double acc[2] = {0.0, 0.0};
-                    tile_row_spmv_f64<B, IDX8, 2>(A.L, tr, lane, lanebase, g, acc);
+                    tile_row_spmv_f64_c<B, IDX8, 2>(A.L, ctx(i), lane, lanebase, g, acc);
                     const size_t row = (size_t)tr * B + lane;
                     const double rv = A.b[row] - acc[0];
                     s_r[i * TPB + tid] = rv;
@@ -277,7 +298,7 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
         for (int i = 0; i < RPT; i++) {
             const int tr = tile_row(i);
             if (tr < tr1) {
-                const double w = tile_row_spmv<B, IDX8, spmv_u(RPT)>(A.L, tr, lane, lanebase, A.g64, A.g32);
+                const double w = tile_row_spmv_c<B, IDX8, spmv_u(RPT)>(A.L, ctx(i), lane, lanebase, A.g64, A.g32);
                 red[0] += w * s_r[i * TPB + tid];
                 s_s[i * TPB + tid] = w + beta * s_s[i * TPB + tid];
             }
@@ -327,6 +348,25 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
     const int grp = tid / B, lane = tid % B, lanebase = (tid & 63) & ~(B - 1);
     const int tr0 = chunk * A.trw, tr1 = min(A.tilem, tr0 + A.trw);
     double xr[RPT], rr[RPT], rh[RPT], pr[RPT], sr[RPT];
+    // headers of the block's tile rows, loaded once: the per-iteration SpMV
+    // then issues the tile-column word and the slab loads together
+    constexpr int TRW = RPT * GPB;
+    __shared__ int4 s_h[2 * TRW];
+    __shared__ int s_t0[TRW + 1];
+    for (int q = tid; q <= TRW; q += TPB) {
+        const int tr = tr0 + q;
+        if (q < TRW && tr < tr1) {
+            const int4 *h = reinterpret_cast<const int4 *>(A.L.trh + (size_t)tr * 4);
+            s_h[2 * q] = h[0];
+            s_h[2 * q + 1] = h[1];
+        }
+        s_t0[q] = A.L.tile_ptr[min(tr, A.tilem)];
+    }
+    __syncthreads();
+    auto ctx = [&](int i) {
+        const int q = i * GPB + grp;
+        return row_ctx_from<IDX8>(A.L, s_t0[q], s_t0[q + 1] - s_t0[q], s_h[2 * q], s_h[2 * q + 1], lane);
+    };
     int rbase = tr0 * B + tid;  // row of step i is rbase + i * TPB
     auto tile_row = [&](int i) { return tr0 + i * GPB + grp; };
     auto row_of = [&](int i) { return (size_t)(rbase + i * TPB); };
@@ -350,7 +390,7 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
 #pragma unroll 1
         for (int i = 0; i < RPT; i++) {
             const int tr = tile_row(i);
-            if (tr < tr1) dst[i * TPB + tid] = tile_row_spmv<B, IDX8, spmv_u(RPT)>(A.L, tr, lane, lanebase, A.g64, A.g32);
+            if (tr < tr1) dst[i * TPB + tid] = tile_row_spmv_c<B, IDX8, spmv_u(RPT)>(A.L, ctx(i), lane, lanebase, A.g64, A.g32);
         }
     };
 
@@ -427,7 +467,7 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
                 if (tr < tr1) {
                     const double *g[1] = {A.h64};
                     double acc[1] = {0.0};
-                    tile_row_spmv_f64<B, IDX8, 1>(A.L, tr, lane, lanebase, g, acc);
+                    tile_row_spmv_f64_c<B, IDX8, 1>(A.L, ctx(i), lane, lanebase, g, acc);
                     s_t[i * TPB + tid] = A.b[(size_t)tr * B + lane] - acc[0];
                 }
             }
@@ -517,7 +557,7 @@ static int ensure_ws(mf_matrix m, int G)
     MF_HIP(hipMalloc((void **)&w.q64, sizeof(double) * ncol));
     MF_HIP(hipMalloc((void **)&w.g32, sizeof(float) * ncol));
     MF_HIP(hipMalloc((void **)&w.partials, sizeof(double) * 4 * 8 * w.G));
-    MF_HIP(hipMalloc((void **)&w.bar, sizeof(unsigned) * 4));
+    MF_HIP(hipMalloc((void **)&w.bar, sizeof(unsigned) * BAR_WORDS));
     MF_HIP(hipMalloc((void **)&w.ctl, sizeof(int) * 8));
     MF_HIP(hipMalloc((void **)&w.out, sizeof(double) * 4));
     MF_HIP(hipMalloc((void **)&w.r_outer, sizeof(double) * ncol));
@@ -537,10 +577,11 @@ struct InnerResult {
     double bnorm = 0, rt2 = 0, g2 = 0;
 };
 
-// Zeroes the barrier and control words with a kernel (no memset nodes).
+// Zeroes the barrier counters (global + 8 groups, one 128-byte line apart)
+// and the control words with a kernel (no memset nodes).
 __global__ void k_reset_ctl(unsigned *bar, int *ctl)
 {
-    if (threadIdx.x < 4) bar[threadIdx.x] = 0u;
+    for (int i = threadIdx.x; i < BAR_WORDS; i += blockDim.x) bar[i] = 0u;
     if (threadIdx.x < 8) ctl[threadIdx.x] = 0;
 }
 
@@ -582,7 +623,7 @@ static int launch_inner(mf_matrix m, const Layout &L, int method, const double *
     A.has64 = L.has64 ? 1 : 0;
     A.haslow = L.haslow ? 1 : 0;
     A.spin_limit = o.spin_limit;
-    k_reset_ctl<<<1, 64, 0, st>>>(w.bar, w.ctl);
+    k_reset_ctl<<<1, 256, 0, st>>>(w.bar, w.ctl);
     MF_HIP(hipGetLastError());
     void *args[] = {&A};
     // Cooperative launch guarantees that all G workgroups are resident.  Some

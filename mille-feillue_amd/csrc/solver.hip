@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
@@ -102,6 +103,27 @@ struct SolveArgs {
     double tol, rr_factor;
     int maxit, has64, haslow;
     long long spin_limit;
+    unsigned long long *phase;  // optional: [0] spmv, [1] reduce after spmv,
+                                // [2] vector update, [3] reduce after update,
+                                // [4] true-residual passes (100 MHz ticks, block 0)
+};
+
+// Phase clock of block 0 (diagnostic; enabled when A.phase is set).
+struct PhaseClock {
+    unsigned long long *acc;
+    unsigned long long t;
+    __device__ PhaseClock(unsigned long long *p) : acc(blockIdx.x == 0 && threadIdx.x == 0 ? p : nullptr), t(0)
+    {
+        if (acc) t = wall_clock64();
+    }
+    __device__ void lap(int k)
+    {
+        if (acc) {
+            const unsigned long long now = wall_clock64();
+            acc[k] += now - t;
+            t = now;
+        }
+    }
 };
 
 template <int TPB, int K>
@@ -253,6 +275,7 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
     if (bnorm2 == 0.0) { finish(MF_OK, 0, 0, 0.0, 0.0); return; }
     double gamma = bnorm2, ref2 = bnorm2, gprev = 1.0, aprev = 1.0;
     int iter = 0, repl = 0;
+    PhaseClock clk(A.phase);
     for (;;) {
         rbase = opaque(rbase);
         if (!isfinite(gamma)) { finish(MF_E_NONFINITE, iter, repl, -1.0, gamma); return; }
@@ -284,6 +307,7 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
                 }
             }
             if (!R.allreduce(A, t1)) return;
+            clk.lap(4);
             ++repl;
             if (!isfinite(t1[0])) { finish(MF_E_NONFINITE, iter, repl, t1[0], gamma); return; }
             if (t1[0] <= tol2) { finish(MF_OK, iter, repl, t1[0], gamma); return; }
@@ -303,7 +327,9 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
                 s_s[i * TPB + tid] = w + beta * s_s[i * TPB + tid];
             }
         }
+        clk.lap(0);
         if (!R.allreduce(A, red)) return;
+        clk.lap(1);
         const double delta = red[0];
         const double denom = iter ? delta - beta * gamma / aprev : delta;
         if (!isfinite(denom)) { finish(MF_E_NONFINITE, iter, repl, -1.0, gamma); return; }
@@ -327,7 +353,9 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
         gprev = gamma;
         aprev = alpha;
         ++iter;
+        clk.lap(2);
         if (!R.allreduce(A, g1)) return;
+        clk.lap(3);
         gamma = g1[0];
     }
 }
@@ -623,6 +651,11 @@ static int launch_inner(mf_matrix m, const Layout &L, int method, const double *
     A.has64 = L.has64 ? 1 : 0;
     A.haslow = L.haslow ? 1 : 0;
     A.spin_limit = o.spin_limit;
+    static const bool phase_profile = std::getenv("MF_PHASE_PROFILE") != nullptr;
+    static unsigned long long *d_phase = nullptr;
+    if (phase_profile && !d_phase) MF_HIP(hipMalloc((void **)&d_phase, 8 * sizeof(unsigned long long)));
+    A.phase = phase_profile ? d_phase : nullptr;
+    if (A.phase) MF_HIP(hipMemsetAsync(d_phase, 0, 8 * sizeof(unsigned long long), st));
     k_reset_ctl<<<1, 256, 0, st>>>(w.bar, w.ctl);
     MF_HIP(hipGetLastError());
     void *args[] = {&A};
@@ -658,6 +691,16 @@ static int launch_inner(mf_matrix m, const Layout &L, int method, const double *
         res->bnorm = out[0];
         res->rt2 = out[1];
         res->g2 = out[2];
+        if (A.phase) {
+            unsigned long long ph[8];
+            MF_HIP(hipMemcpy(ph, A.phase, sizeof(ph), hipMemcpyDeviceToHost));
+            const double it = std::max(1, res->iterations);
+            std::fprintf(stderr,
+                         "[mf phase] G=%d TPB=%d RPT=%d it=%d  us/iter: spmv %.2f  reduce1 %.2f  update %.2f  "
+                         "reduce2 %.2f  | true-residual passes %.1f us total\n",
+                         G, kc.tpb, kc.rpt, res->iterations, ph[0] / it / 100.0, ph[1] / it / 100.0,
+                         ph[2] / it / 100.0, ph[3] / it / 100.0, ph[4] / 100.0);
+        }
     }
     return MF_OK;
 }

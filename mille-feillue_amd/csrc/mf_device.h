@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace mf {
 
 // ----------------------------------------------------------------- layout
@@ -12,12 +14,15 @@ namespace mf {
 // ordered by precision class (reference order inside a class); slot s is a
 // tile's position in that order and stc[tile_ptr[tr] + s] its tile column.
 // For each class c present in the tile row, row r's nonzeros in the class's
-// tiles (slot order, then element order) form a slab of L_c slices: slice k
-// holds the k-th nonzero of every row, B consecutive values followed (after
-// all L_c value slices) by B consecutive indices (slot << log2(B) | column in
-// tile; one byte when every tile row has at most B tiles, else two).  Short
-// rows are padded with value 0 / index 0.  trh[tr * 4 + c] = {slab offset / 8,
-// L_c}.  Class c: 0 FP64, 1 FP32, 2 FP16, 3 FP8 E4M3 (OCP).
+// tiles (slot order, then element order) are entries k = 0 .. L_c - 1 of a
+// slab of ceil(L_c / 4) quads: entry k of row r sits at element
+// ((k / 4) * B + r) * 4 + k % 4, so one lane reads four consecutive entries of
+// its row with one load and the B lanes of a tile row read one contiguous
+// block.  All value quads come first, then the index quads (slot << log2(B) |
+// column in tile; one byte when every tile row has at most B tiles, else
+// two).  Short rows are padded with value 0 / index 0.  Every slab size is a
+// multiple of 16 bytes.  trh[tr * 4 + c] = {slab offset / 8, L_c}.  Class c:
+// 0 FP64, 1 FP32, 2 FP16, 3 FP8 E4M3 (OCP).
 struct DevL {
     const int *tile_ptr;
     const int2 *trh;
@@ -43,24 +48,6 @@ __device__ __forceinline__ float f16_to_f32(uint16_t h)
     __builtin_memcpy(&v, &h, 2);
     return (float)v;
 }
-
-template <int CLS> struct ClsT;
-template <> struct ClsT<0> {
-    using T = double;
-    __device__ static double dec(double v) { return v; }
-};
-template <> struct ClsT<1> {
-    using T = float;
-    __device__ static float dec(float v) { return v; }
-};
-template <> struct ClsT<2> {
-    using T = uint16_t;
-    __device__ static float dec(uint16_t v) { return f16_to_f32(v); }
-};
-template <> struct ClsT<3> {
-    using T = uint8_t;
-    __device__ static float dec(uint8_t v) { return e4m3_to_f32(v); }
-};
 
 __device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
@@ -96,39 +83,92 @@ __device__ __forceinline__ int gather_index(uint32_t id, const RowCtx &c, const 
     return tc * B + col;
 }
 
-template <bool IDX8>
-__device__ __forceinline__ uint32_t load_index(const uint8_t *idx, size_t e)
+// One quad (four consecutive entries of one lane's row) of values and of
+// indices, loaded with one or two vector loads.
+template <int CLS> struct Quad;
+template <> struct Quad<0> {
+    double2 a, b;
+    __device__ void load(const uint8_t *p, size_t q)
+    {
+        const double2 *v = reinterpret_cast<const double2 *>(p) + 2 * q;
+        a = v[0];
+        b = v[1];
+    }
+    __device__ void zero() { a = b = make_double2(0.0, 0.0); }
+    template <int E> __device__ double get() const { return E == 0 ? a.x : E == 1 ? a.y : E == 2 ? b.x : b.y; }
+};
+template <> struct Quad<1> {
+    float4 a;
+    __device__ void load(const uint8_t *p, size_t q) { a = reinterpret_cast<const float4 *>(p)[q]; }
+    __device__ void zero() { a = make_float4(0.f, 0.f, 0.f, 0.f); }
+    template <int E> __device__ float get() const { return E == 0 ? a.x : E == 1 ? a.y : E == 2 ? a.z : a.w; }
+};
+template <> struct Quad<2> {
+    uint2 a;
+    __device__ void load(const uint8_t *p, size_t q) { a = reinterpret_cast<const uint2 *>(p)[q]; }
+    __device__ void zero() { a = make_uint2(0u, 0u); }
+    template <int E> __device__ float get() const
+    {
+        return f16_to_f32((uint16_t)(((E < 2 ? a.x : a.y) >> (16 * (E & 1))) & 0xffffu));
+    }
+};
+template <> struct Quad<3> {
+    uint32_t a;
+    __device__ void load(const uint8_t *p, size_t q) { a = reinterpret_cast<const uint32_t *>(p)[q]; }
+    __device__ void zero() { a = 0u; }
+    template <int E> __device__ float get() const { return e4m3_to_f32((a >> (8 * E)) & 0xffu); }
+};
+template <bool IDX8> struct IdxQuad;
+template <> struct IdxQuad<true> {
+    uint32_t a;
+    __device__ void load(const uint8_t *p, size_t q) { a = reinterpret_cast<const uint32_t *>(p)[q]; }
+    __device__ void zero() { a = 0u; }
+    template <int E> __device__ uint32_t get() const { return (a >> (8 * E)) & 0xffu; }
+};
+template <> struct IdxQuad<false> {
+    uint2 a;
+    __device__ void load(const uint8_t *p, size_t q) { a = reinterpret_cast<const uint2 *>(p)[q]; }
+    __device__ void zero() { a = make_uint2(0u, 0u); }
+    template <int E> __device__ uint32_t get() const { return ((E < 2 ? a.x : a.y) >> (16 * (E & 1))) & 0xffffu; }
+};
+
+template <int N, typename F> __device__ __forceinline__ void static_for(F &&f)
 {
-    return IDX8 ? (uint32_t)idx[e] : (uint32_t)reinterpret_cast<const uint16_t *>(idx)[e];
+    if constexpr (N > 0) {
+        static_for<N - 1>(f);
+        f(std::integral_constant<int, N - 1>{});
+    }
 }
 
-// One class slab, U slices per step: all value and index loads of the step
+// One class slab, UQ quads per step: all value and index loads of the step
 // are issued before the gathers, and all gathers before the FMAs.  AccT is
-// double for class 0 (gather g64) and float otherwise (gather g32).
-template <int B, bool IDX8, int CLS, int U, typename AccT, typename GT>
+// double for class 0 (gather g64) and float otherwise (gather g32).  Padding
+// entries (k >= L_c) are loaded but not accumulated.
+template <int B, bool IDX8, int CLS, int UQ, typename AccT, typename GT>
 __device__ __forceinline__ void slab_spmv(const DevL &L, const RowCtx &c, int2 h, int r, int lanebase,
                                           const GT *__restrict__ g, AccT &acc)
 {
-    using VT = typename ClsT<CLS>::T;
-    const int Lc = h.y;
-    const uint8_t *base = L.blob + (size_t)(unsigned)h.x * 8;
-    const VT *vals = reinterpret_cast<const VT *>(base);
-    const uint8_t *idx = base + (size_t)Lc * B * sizeof(VT);
-    for (int k0 = 0; k0 < Lc; k0 += U) {
-        VT v[U];
-        uint32_t id[U];
+    const int Lc = h.y, nq = (Lc + 3) >> 2;
+    const uint8_t *vals = L.blob + (size_t)(unsigned)h.x * 8;
+    const uint8_t *idx = vals + (size_t)nq * B * 4 * cls_width(CLS);
+    for (int j0 = 0; j0 < nq; j0 += UQ) {
+        Quad<CLS> v[UQ];
+        IdxQuad<IDX8> id[UQ];
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int k = min(k0 + u, Lc - 1);
-            v[u] = vals[(size_t)k * B + r];
-            id[u] = load_index<IDX8>(idx, (size_t)k * B + r);
+        for (int u = 0; u < UQ; u++) {
+            const size_t q = (size_t)min(j0 + u, nq - 1) * B + r;
+            v[u].load(vals, q);
+            id[u].load(idx, q);
         }
-        GT xg[U];
+        GT xg[UQ][4];
 #pragma unroll
-        for (int u = 0; u < U; u++) xg[u] = g[gather_index<B, IDX8>(id[u], c, L, lanebase)];
+        for (int u = 0; u < UQ; u++)
+            static_for<4>([&](auto e) { xg[u][e] = g[gather_index<B, IDX8>(id[u].template get<e>(), c, L, lanebase)]; });
 #pragma unroll
-        for (int u = 0; u < U; u++)
-            if (k0 + u < Lc) acc = fma_t((AccT)ClsT<CLS>::dec(v[u]), (AccT)xg[u], acc);
+        for (int u = 0; u < UQ; u++)
+            static_for<4>([&](auto e) {
+                if ((j0 + u) * 4 + e < Lc) acc = fma_t((AccT)v[u].template get<e>(), (AccT)xg[u][e], acc);
+            });
     }
 }
 
@@ -147,26 +187,146 @@ __device__ __forceinline__ RowCtx row_ctx_from(const DevL &L, int t0, int nt, in
 
 // y_r of tile row tr for lane r: FP64 tiles accumulate in FP64 over g64, then
 // FP32, FP16 and FP8 tiles (in that order) accumulate in FP32 over g32.
-template <int B, bool IDX8, int U = 8>
+template <int B, bool IDX8, int UQ = 2>
 __device__ __forceinline__ double tile_row_spmv_c(const DevL &L, const RowCtx &c, int r, int lanebase,
                                                   const double *__restrict__ g64,
                                                   const float *__restrict__ g32)
 {
     double a64 = 0.0;
     float a32 = 0.0f;
-    if (c.h01.y) slab_spmv<B, IDX8, 0, U>(L, c, make_int2(c.h01.x, c.h01.y), r, lanebase, g64, a64);
-    if (c.h01.w) slab_spmv<B, IDX8, 1, U>(L, c, make_int2(c.h01.z, c.h01.w), r, lanebase, g32, a32);
-    if (c.h23.y) slab_spmv<B, IDX8, 2, U>(L, c, make_int2(c.h23.x, c.h23.y), r, lanebase, g32, a32);
-    if (c.h23.w) slab_spmv<B, IDX8, 3, U>(L, c, make_int2(c.h23.z, c.h23.w), r, lanebase, g32, a32);
+    if (c.h01.y) slab_spmv<B, IDX8, 0, UQ>(L, c, make_int2(c.h01.x, c.h01.y), r, lanebase, g64, a64);
+    if (c.h01.w) slab_spmv<B, IDX8, 1, UQ>(L, c, make_int2(c.h01.z, c.h01.w), r, lanebase, g32, a32);
+    if (c.h23.y) slab_spmv<B, IDX8, 2, UQ>(L, c, make_int2(c.h23.x, c.h23.y), r, lanebase, g32, a32);
+    if (c.h23.w) slab_spmv<B, IDX8, 3, UQ>(L, c, make_int2(c.h23.z, c.h23.w), r, lanebase, g32, a32);
     return a64 + (double)a32;
 }
 
-template <int B, bool IDX8, int U = 8>
+template <int B, bool IDX8, int UQ = 2>
 __device__ __forceinline__ double tile_row_spmv(const DevL &L, int tr, int r, int lanebase,
                                                 const double *__restrict__ g64,
                                                 const float *__restrict__ g32)
 {
-    return tile_row_spmv_c<B, IDX8, U>(L, row_ctx<B, IDX8>(L, tr, r), r, lanebase, g64, g32);
+    return tile_row_spmv_c<B, IDX8, UQ>(L, row_ctx<B, IDX8>(L, tr, r), r, lanebase, g64, g32);
+}
+
+// Element idx of vector g with a 32-bit byte offset from the uniform base
+// (vectors are below 4 GB, which the host checks).
+template <typename T> __device__ __forceinline__ T ld_elem(const T *__restrict__ g, uint32_t idx)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(g) + (idx * (uint32_t)sizeof(T)));
+}
+
+// Tile column of slot of this lane's tile row: from the lane group's register
+// copy (one-byte index form) or from stc.
+template <int B, bool IDX8>
+__device__ __forceinline__ uint32_t gather_col(uint32_t id, int tcr, int t0, const DevL &L, int lanebase)
+{
+    const int slot = (int)(id >> log2_of<B>());
+    const int tc = IDX8 ? __shfl(tcr, lanebase + slot, 64) : L.stc[t0 + slot];
+    return (uint32_t)tc * B + (id & (B - 1));
+}
+
+// Tile-row headers staged in LDS: hd[2 q] = {off0, L0, off1, L1} and
+// hd[2 q + 1] = {off2, L2, off3, L3} of local tile row q; t0s[q] is its first
+// tile (t0s[q + 1] - t0s[q] tiles).
+template <int CLS> __device__ __forceinline__ int2 lds_header(const int4 *hd, int q)
+{
+    return reinterpret_cast<const int2 *>(hd + 2 * q)[CLS];
+}
+
+template <bool IDX8> __device__ __forceinline__ int load_tcr(const DevL &L, const int *t0s, int q, int r)
+{
+    if (!IDX8) return 0;
+    const int t0 = t0s[q], nt = t0s[q + 1] - t0;
+    return r < nt ? L.stc[t0 + r] : 0;
+}
+
+// One class slab of NR tile rows at once (local rows q0 + k * QS, this lane's
+// row in each): per step the quads of NB rows are loaded together, then every
+// gather is issued, then the FMAs run in each row's own entry order, so each
+// row's sum is bit-identical to slab_spmv.  FP64 values go 1 quad x NR/2 rows
+// per step (their quads are 4x wider), the other classes 2 quads x NR rows.
+template <int B, bool IDX8, int CLS, int NR, int QS, typename AccT, typename GT>
+__device__ __forceinline__ void slab_spmv_rows(const DevL &L, const int4 *hd, const int *t0s, int q0,
+                                               const int (&tcr)[NR], int r, int lanebase,
+                                               const GT *__restrict__ g, AccT (&acc)[NR])
+{
+    constexpr int UQ = CLS == 0 ? 1 : 2;
+    constexpr int NB = CLS == 0 && NR > 1 ? NR / 2 : NR;
+#pragma unroll
+    for (int kb = 0; kb < NR; kb += NB) {
+        int2 h[NB];
+        int nqmax = 0;
+#pragma unroll
+        for (int k = 0; k < NB; k++) {
+            h[k] = lds_header<CLS>(hd, q0 + (kb + k) * QS);
+            nqmax = max(nqmax, (h[k].y + 3) >> 2);
+        }
+        for (int j0 = 0; j0 < nqmax; j0 += UQ) {
+            Quad<CLS> v[NB][UQ];
+            IdxQuad<IDX8> id[NB][UQ];
+#pragma unroll
+            for (int k = 0; k < NB; k++) {
+                const int nq = (h[k].y + 3) >> 2;
+                const uint8_t *vals = L.blob + (size_t)(unsigned)h[k].x * 8;
+                const uint8_t *idx = vals + (size_t)nq * B * 4 * cls_width(CLS);
+#pragma unroll
+                for (int u = 0; u < UQ; u++) {
+                    if (j0 + u < nq) {
+                        v[k][u].load(vals, (size_t)(j0 + u) * B + r);
+                        id[k][u].load(idx, (size_t)(j0 + u) * B + r);
+                    } else {
+                        v[k][u].zero();
+                        id[k][u].zero();
+                    }
+                }
+            }
+            GT xg[NB][UQ][4];
+#pragma unroll
+            for (int k = 0; k < NB; k++) {
+                const int t0 = IDX8 ? 0 : t0s[q0 + (kb + k) * QS];
+#pragma unroll
+                for (int u = 0; u < UQ; u++)
+                    static_for<4>([&](auto e) {
+                        xg[k][u][e] =
+                            ld_elem(g, gather_col<B, IDX8>(id[k][u].template get<e>(), tcr[kb + k], t0, L, lanebase));
+                    });
+            }
+#pragma unroll
+            for (int k = 0; k < NB; k++)
+#pragma unroll
+                for (int u = 0; u < UQ; u++)
+                    static_for<4>([&](auto e) {
+                        if ((j0 + u) * 4 + e < h[k].y)
+                            acc[kb + k] = fma_t((AccT)v[k][u].template get<e>(), (AccT)xg[k][u][e], acc[kb + k]);
+                    });
+        }
+    }
+}
+
+// y of NR tile rows (local rows q0 + k * QS, headers in LDS), the same sums
+// as tile_row_spmv_c.
+template <int B, bool IDX8, int NR, int QS>
+__device__ __forceinline__ void tile_rows_spmv(const DevL &L, const int4 *hd, const int *t0s, int q0, int r,
+                                               int lanebase, const double *__restrict__ g64,
+                                               const float *__restrict__ g32, double (&y)[NR])
+{
+    int tcr[NR];
+#pragma unroll
+    for (int k = 0; k < NR; k++) tcr[k] = load_tcr<IDX8>(L, t0s, q0 + k * QS, r);
+    double a64[NR];
+    float a32[NR];
+#pragma unroll
+    for (int k = 0; k < NR; k++) {
+        a64[k] = 0.0;
+        a32[k] = 0.0f;
+    }
+    slab_spmv_rows<B, IDX8, 0, NR, QS>(L, hd, t0s, q0, tcr, r, lanebase, g64, a64);
+    slab_spmv_rows<B, IDX8, 1, NR, QS>(L, hd, t0s, q0, tcr, r, lanebase, g32, a32);
+    slab_spmv_rows<B, IDX8, 2, NR, QS>(L, hd, t0s, q0, tcr, r, lanebase, g32, a32);
+    slab_spmv_rows<B, IDX8, 3, NR, QS>(L, hd, t0s, q0, tcr, r, lanebase, g32, a32);
+#pragma unroll
+    for (int k = 0; k < NR; k++) y[k] = a64[k] + (double)a32[k];
 }
 
 // FP64-accumulated product of NV gathered vectors at once (true residuals and
@@ -175,16 +335,22 @@ template <int B, bool IDX8, int CLS, int NV>
 __device__ __forceinline__ void slab_spmv_f64(const DevL &L, const RowCtx &c, int2 h, int r, int lanebase,
                                               const double *const (&g)[NV], double (&acc)[NV])
 {
-    using VT = typename ClsT<CLS>::T;
-    const int Lc = h.y;
-    const uint8_t *base = L.blob + (size_t)(unsigned)h.x * 8;
-    const VT *vals = reinterpret_cast<const VT *>(base);
-    const uint8_t *idx = base + (size_t)Lc * B * sizeof(VT);
-    for (int k = 0; k < Lc; k++) {
-        const double v = (double)ClsT<CLS>::dec(vals[(size_t)k * B + r]);
-        const int gi = gather_index<B, IDX8>(load_index<IDX8>(idx, (size_t)k * B + r), c, L, lanebase);
+    const int Lc = h.y, nq = (Lc + 3) >> 2;
+    const uint8_t *vals = L.blob + (size_t)(unsigned)h.x * 8;
+    const uint8_t *idx = vals + (size_t)nq * B * 4 * cls_width(CLS);
+    for (int j = 0; j < nq; j++) {
+        Quad<CLS> v;
+        IdxQuad<IDX8> id;
+        v.load(vals, (size_t)j * B + r);
+        id.load(idx, (size_t)j * B + r);
+        static_for<4>([&](auto e) {
+            if (j * 4 + e < Lc) {
+                const double a = (double)v.template get<e>();
+                const int gi = gather_index<B, IDX8>(id.template get<e>(), c, L, lanebase);
 #pragma unroll
-        for (int q = 0; q < NV; q++) acc[q] = __builtin_fma(v, g[q][gi], acc[q]);
+                for (int q = 0; q < NV; q++) acc[q] = __builtin_fma(a, g[q][gi], acc[q]);
+            }
+        });
     }
 }
 

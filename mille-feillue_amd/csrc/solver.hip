@@ -36,16 +36,37 @@
 namespace mf {
 
 // ------------------------------------------------------------ SpMV kernel
-// 256 threads = 256/B tile rows per block, lane = row inside the tile row.
+// 256 threads = 256/B lane groups, lane = row inside the tile row.  A block
+// covers SPMV_NR * 256/B consecutive tile rows; lane group q handles tile
+// rows base + k * 256/B + q (k < SPMV_NR) together.
+constexpr int SPMV_NR = 4;
 template <int B, bool IDX8>
 __global__ __launch_bounds__(256) void k_spmv(DevL L, int tilem, const double *__restrict__ g64,
                                               const float *__restrict__ g32, double *__restrict__ y)
 {
-    const int lb = xcd_chunk(blockIdx.x, gridDim.x);
-    const int tr = lb * (256 / B) + threadIdx.x / B, r = threadIdx.x % B;
+    constexpr int GPB = 256 / B, TRB = GPB * SPMV_NR;
+    __shared__ int4 s_h[2 * TRB];
+    __shared__ int s_t0[TRB + 1];
+    const int tb = xcd_chunk(blockIdx.x, gridDim.x) * TRB;
+    for (int q = threadIdx.x; q <= TRB; q += 256) {
+        const int tr = tb + q;
+        if (q < TRB) {
+            const int4 *h = reinterpret_cast<const int4 *>(L.trh + (size_t)min(tr, tilem) * 4);
+            s_h[2 * q] = tr < tilem ? h[0] : make_int4(0, 0, 0, 0);
+            s_h[2 * q + 1] = tr < tilem ? h[1] : make_int4(0, 0, 0, 0);
+        }
+        s_t0[q] = L.tile_ptr[min(tr, tilem)];
+    }
+    __syncthreads();
+    const int grp = threadIdx.x / B, r = threadIdx.x % B;
     const int lanebase = (threadIdx.x & 63) & ~(B - 1);
-    if (tr >= tilem) return;
-    y[(size_t)tr * B + r] = tile_row_spmv<B, IDX8>(L, tr, r, lanebase, g64, g32);
+    double v[SPMV_NR];
+    tile_rows_spmv<B, IDX8, SPMV_NR, GPB>(L, s_h, s_t0, grp, r, lanebase, g64, g32, v);
+#pragma unroll
+    for (int k = 0; k < SPMV_NR; k++) {
+        const int tr = tb + k * GPB + grp;
+        if (tr < tilem) y[(size_t)tr * B + r] = v[k];
+    }
 }
 
 __global__ void k_to_f32(const double *__restrict__ x, float *__restrict__ y, long long n)
@@ -126,6 +147,8 @@ struct PhaseClock {
     }
 };
 
+constexpr unsigned PHASE_EP0 = 201;
+
 template <int TPB, int K>
 struct Reducer {
     double *s_red;  // [TPB/64][K]
@@ -137,7 +160,14 @@ struct Reducer {
     __device__ bool barrier(const SolveArgs &A)
     {
         ++ep;
-        return grid_barrier(A.bar, A.ctl, ep, A.spin_limit, s_flag);
+        // diagnostic stamps of barriers PHASE_EP0 .. PHASE_EP0 + 3: arrival and
+        // exit time of every workgroup
+        const bool stamp = A.phase && threadIdx.x == 0 && ep - PHASE_EP0 < 4u;
+        unsigned long long *st = stamp ? A.phase + 16 + (size_t)(ep - PHASE_EP0) * 2 * A.G + 2 * blockIdx.x : nullptr;
+        if (stamp) st[0] = wall_clock64();
+        const bool ok = grid_barrier(A.bar, A.ctl, ep, A.spin_limit, s_flag);
+        if (stamp) st[1] = wall_clock64();
+        return ok;
     }
 
     // Sums v over the grid; every workgroup obtains the same bits.
@@ -180,9 +210,8 @@ struct Reducer {
     }
 };
 
-// slices per load batch inside the persistent kernels: 8 keeps enough loads
-// in flight per wave
-__host__ __device__ constexpr int spmv_u(int rpt) { return rpt >= 32 ? 4 : 8; }
+// tile rows whose loads are interleaved inside the persistent kernels' SpMV
+__host__ __device__ constexpr int spmv_rows(int rpt) { return rpt < 4 ? rpt : 4; }
 
 __device__ __forceinline__ bool finite_all(double a, double b) { return isfinite(a) && isfinite(b); }
 
@@ -209,7 +238,7 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
     __shared__ double s_bc[4];
     __shared__ int s_flag;
     Reducer<TPB, 4> R{s_red, s_bc, &s_flag};
-    constexpr int GPB = TPB / B;
+    constexpr int GPB = TPB / B, NR = spmv_rows(RPT);
     const int tid = threadIdx.x;
     const int chunk = xcd_chunk(blockIdx.x, gridDim.x);
     const int grp = tid / B, lane = tid % B, lanebase = (tid & 63) & ~(B - 1);
@@ -226,6 +255,8 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
             const int4 *h = reinterpret_cast<const int4 *>(A.L.trh + (size_t)tr * 4);
             s_h[2 * q] = h[0];
             s_h[2 * q + 1] = h[1];
+        } else if (q < TRW) {  // beyond the chunk: no slabs
+            s_h[2 * q] = s_h[2 * q + 1] = make_int4(0, 0, 0, 0);
         }
         s_t0[q] = A.L.tile_ptr[min(tr, A.tilem)];
     }
@@ -319,12 +350,16 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
         // w = A r; s = w + beta s; (w, r)
         double red[1] = {0.0};
 #pragma unroll 1
-        for (int i = 0; i < RPT; i++) {
-            const int tr = tile_row(i);
-            if (tr < tr1) {
-                const double w = tile_row_spmv_c<B, IDX8, spmv_u(RPT)>(A.L, ctx(i), lane, lanebase, A.g64, A.g32);
-                red[0] += w * s_r[i * TPB + tid];
-                s_s[i * TPB + tid] = w + beta * s_s[i * TPB + tid];
+        for (int i0 = 0; i0 < RPT; i0 += NR) {
+            double w[NR];
+            tile_rows_spmv<B, IDX8, NR, GPB>(A.L, s_h, s_t0, i0 * GPB + grp, lane, lanebase, A.g64, A.g32, w);
+#pragma unroll
+            for (int k = 0; k < NR; k++) {
+                const int i = i0 + k;
+                if (tile_row(i) < tr1) {
+                    red[0] += w[k] * s_r[i * TPB + tid];
+                    s_s[i * TPB + tid] = w[k] + beta * s_s[i * TPB + tid];
+                }
             }
         }
         clk.lap(0);
@@ -370,7 +405,7 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
     __shared__ double s_bc[4];
     __shared__ int s_flag;
     Reducer<TPB, 4> R{s_red, s_bc, &s_flag};
-    constexpr int GPB = TPB / B;
+    constexpr int GPB = TPB / B, NR = spmv_rows(RPT);
     const int tid = threadIdx.x;
     const int chunk = xcd_chunk(blockIdx.x, gridDim.x);
     const int grp = tid / B, lane = tid % B, lanebase = (tid & 63) & ~(B - 1);
@@ -387,6 +422,8 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
             const int4 *h = reinterpret_cast<const int4 *>(A.L.trh + (size_t)tr * 4);
             s_h[2 * q] = h[0];
             s_h[2 * q + 1] = h[1];
+        } else if (q < TRW) {  // beyond the chunk: no slabs
+            s_h[2 * q] = s_h[2 * q + 1] = make_int4(0, 0, 0, 0);
         }
         s_t0[q] = A.L.tile_ptr[min(tr, A.tilem)];
     }
@@ -416,9 +453,12 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
     };
     auto spmv_into = [&](double *dst) {
 #pragma unroll 1
-        for (int i = 0; i < RPT; i++) {
-            const int tr = tile_row(i);
-            if (tr < tr1) dst[i * TPB + tid] = tile_row_spmv_c<B, IDX8, spmv_u(RPT)>(A.L, ctx(i), lane, lanebase, A.g64, A.g32);
+        for (int i0 = 0; i0 < RPT; i0 += NR) {
+            double w[NR];
+            tile_rows_spmv<B, IDX8, NR, GPB>(A.L, s_h, s_t0, i0 * GPB + grp, lane, lanebase, A.g64, A.g32, w);
+#pragma unroll
+            for (int k = 0; k < NR; k++)
+                if (tile_row(i0 + k) < tr1) dst[(i0 + k) * TPB + tid] = w[k];
         }
     };
 
@@ -653,9 +693,10 @@ static int launch_inner(mf_matrix m, const Layout &L, int method, const double *
     A.spin_limit = o.spin_limit;
     static const bool phase_profile = std::getenv("MF_PHASE_PROFILE") != nullptr;
     static unsigned long long *d_phase = nullptr;
-    if (phase_profile && !d_phase) MF_HIP(hipMalloc((void **)&d_phase, 8 * sizeof(unsigned long long)));
+    const size_t phase_words = 16 + 8 * 1024;
+    if (phase_profile && !d_phase) MF_HIP(hipMalloc((void **)&d_phase, phase_words * sizeof(unsigned long long)));
     A.phase = phase_profile ? d_phase : nullptr;
-    if (A.phase) MF_HIP(hipMemsetAsync(d_phase, 0, 8 * sizeof(unsigned long long), st));
+    if (A.phase) MF_HIP(hipMemsetAsync(d_phase, 0, phase_words * sizeof(unsigned long long), st));
     k_reset_ctl<<<1, 256, 0, st>>>(w.bar, w.ctl);
     MF_HIP(hipGetLastError());
     void *args[] = {&A};
@@ -700,6 +741,22 @@ static int launch_inner(mf_matrix m, const Layout &L, int method, const double *
                          "reduce2 %.2f  | true-residual passes %.1f us total\n",
                          G, kc.tpb, kc.rpt, res->iterations, ph[0] / it / 100.0, ph[1] / it / 100.0,
                          ph[2] / it / 100.0, ph[3] / it / 100.0, ph[4] / 100.0);
+            std::vector<unsigned long long> sv((size_t)8 * G);
+            MF_HIP(hipMemcpy(sv.data(), A.phase + 16, sv.size() * 8, hipMemcpyDeviceToHost));
+            for (int e = 0; e < 4; e++) {
+                const unsigned long long *p = sv.data() + (size_t)e * 2 * G;
+                if (!p[0]) continue;
+                unsigned long long a0 = ~0ull, a1 = 0, x0 = ~0ull, x1 = 0;
+                for (int g = 0; g < G; g++) {
+                    a0 = std::min(a0, p[2 * g]);
+                    a1 = std::max(a1, p[2 * g]);
+                    x0 = std::min(x0, p[2 * g + 1]);
+                    x1 = std::max(x1, p[2 * g + 1]);
+                }
+                std::fprintf(stderr, "[mf phase] barrier %u: arrival spread %.2f us, last arrival -> first exit %.2f us, "
+                             "exit spread %.2f us\n", PHASE_EP0 + e, (a1 - a0) / 100.0, ((double)x0 - (double)a1) / 100.0,
+                             (x1 - x0) / 100.0);
+            }
         }
     }
     return MF_OK;
@@ -829,7 +886,7 @@ int spmv_device(mf_matrix m, const Layout &L, const double *d_x, double *d_y, bo
     if (rc) return rc;
     if (L.haslow && convert)
         k_to_f32<<<(unsigned)((ncol + 255) / 256), 256, 0, st>>>(d_x, m->ws.g32, ncol);
-    const int nb = (int)(((long long)m->tilem * B + 255) / 256);
+    const int nb = (int)(((long long)m->tilem * B + 256 * SPMV_NR - 1) / (256 * SPMV_NR));
     DevL V = L.view(m->d_tile_ptr);
 #define MF_SPMV(BB, I8) k_spmv<BB, I8><<<nb, 256, 0, st>>>(V, m->tilem, d_x, m->ws.g32, d_y)
     if (B == 16) { if (L.idx8) MF_SPMV(16, true); else MF_SPMV(16, false); }

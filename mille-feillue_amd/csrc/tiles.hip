@@ -477,7 +477,7 @@ __global__ __launch_bounds__(256) void k_slab_sizes(const int *__restrict__ tile
     for (int c = 0; c < 4; c++) {
         int Lc = 0;
         for (int r = 0; r < B; r++) Lc = max(Lc, len[c][r]);
-        u += ((unsigned long long)Lc * B * (cls_width(c) + iw) + 7) / 8;
+        u += (unsigned long long)((Lc + 3) >> 2) * B * 4 * (cls_width(c) + iw) / 8;
     }
     units[tr] = u;
 }
@@ -511,15 +511,15 @@ __global__ __launch_bounds__(256) void k_slab_pack(const int *__restrict__ tile_
         if (o > 0xFFFFFFFFull) { atomicOr(err, 2); return; }
         trh[(size_t)tr * 4 + c] = make_int2((int)(unsigned)o, Lc);
         if (!Lc) continue;
-        const int w = cls_width(c);
+        const int w = cls_width(c), nq = (Lc + 3) >> 2;
         uint8_t *vals = blob + o * 8;
-        uint8_t *idx = vals + (size_t)Lc * B * w;
+        uint8_t *idx = vals + (size_t)nq * B * 4 * w;
         for (int t = t0; t < t1; t++) {
             if (code_to_cls(codes ? codes[t] : 0) != c) continue;
             stc[t0 + slot] = tile_columnidx[t];
             for (int k = tile_start[t]; k < tile_start[t + 1]; k++) {
-                const int r = rowtmp[k];
-                const size_t e = (size_t)(len[c][r]++) * B + r;
+                const int r = rowtmp[k], q = len[c][r]++;
+                const size_t e = ((size_t)(q >> 2) * B + r) * 4 + (q & 3);
                 const double v = bval[k];
                 switch (c) {
                 case 0: reinterpret_cast<double *>(vals)[e] = v; break;
@@ -533,7 +533,7 @@ __global__ __launch_bounds__(256) void k_slab_pack(const int *__restrict__ tile_
             }
             slot++;
         }
-        o += ((unsigned long long)Lc * B * (w + iw) + 7) / 8;
+        o += (unsigned long long)nq * B * 4 * (w + iw) / 8;
     }
 }
 

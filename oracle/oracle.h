@@ -22,9 +22,17 @@
  *                         (MP/data/cg_syncfree_reduce_2757.csv); invariants.
  *   orc_mc_classify       closed form on hand-built values at the thresholds.
  *   orc_e4m3/f16          brute force over all codes (nearest, ties-to-even).
+ *   orc_tile_classes      hand-built values whose exact formats are known.
+ *   orc_tile_spmv         scipy CSR product.
+ *   orc_tile_spmv_mixed   all-FP64 codes reduce to orc_tile_spmv; lossless
+ *                         low codes agree with scipy's product to FP32
+ *                         accumulation error.
  *   orc_cg / orc_bicgstab scipy.sparse.linalg.spsolve solution; textbook
  *                         properties (CG converges in <= n steps on tiny SPD,
  *                         exact after k steps with k distinct eigenvalues).
+ *   orc_petsc_cg/_bcgs    parity unpinned against PETSc itself (PETSc cannot
+ *                         be built here); only checked to agree with orc_cg
+ *                         within one iteration and to converge.
  */
 #ifndef MF_ORACLE_H
 #define MF_ORACLE_H

@@ -201,6 +201,29 @@ def test_mixed_spmv_with_fp64_codes_is_tile_spmv(orc):
     assert np.allclose(y, A @ x[: int(g["n"])], rtol=1e-13, atol=1e-12)
 
 
+def test_mixed_spmv_low_codes_against_scipy(orc):
+    """Lossless low-precision tiles: the mixed product differs from the exact
+    one only by FP32 rounding of x and FP32 accumulation, so it must agree
+    with scipy's FP64 product to ~1e-6 relative -- a dropped term, a wrong
+    sign or a wrong column would be off by O(1)."""
+    rng = np.random.default_rng(5)
+    n = 200
+    A = sp.random(n, n, density=0.05, random_state=6, format="csr")
+    A.data = rng.choice([-2.0, -1.0, -0.5, 0.25, 1.0, 3.0, 6.0], A.nnz)  # exact in E4M3
+    A = (A + sp.diags(np.full(n, 8.0))).tocsr()
+    A.data[: n // 3] = 1000.0 + np.arange(n // 3)  # forces FP16/FP32 in some tiles
+    t = orc.tiles(16, n, n, A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data)
+    codes = t.classes(0.0)
+    assert set(codes.tolist()) >= {2, 3}
+    x = rng.standard_normal(t.d["tilen"] * 16)
+    x[n:] = 0.0
+    y = t.spmv_mixed(codes, x)[:n]
+    ref = A @ x[:n]
+    scale = np.abs(A) @ np.abs(x[:n])
+    assert np.all(np.abs(y - ref) <= 1e-6 * scale + 1e-300)
+    assert not np.array_equal(y, ref)  # the FP32 path really ran
+
+
 def test_cg_oracle_against_direct_solve(orc):
     g = load_golden("isai_spd")
     A = sp.csr_matrix((g["val"], g["col"], g["rowptr"]), shape=(int(g["m"]), int(g["n"])))

@@ -102,7 +102,7 @@ void mf_opts_default(mf_opts *o)
     o->ir_inner_tol = 1e-6;
     o->grid = 0;
     o->block = 0;
-    o->spin_limit = 1LL << 26;  // a few seconds of polling
+    o->spin_limit = 1LL << 22;  // a few seconds of polling
 }
 
 int mf_matrix_create(int rowA, int colA, int nnz, const int *rowptr, const int *colidx,
@@ -127,6 +127,9 @@ int mf_matrix_create(int rowA, int colA, int nnz, const int *rowptr, const int *
     for (long long k = 0; k < nz; k++)
         if (colidx[k] < 0 || colidx[k] >= colA) bad |= 1;
     if (bad) return fail(MF_E_ARG, "column index outside [0, colA)");
+    // the kernels address vectors with 32-bit byte offsets
+    const long long npad = ((long long)std::max(rowA, colA) + B - 1) / B * B;
+    if (npad * 8 >= (1ll << 32)) return fail(MF_E_UNSUPPORTED, "vectors of 4 GB or more");
     int rc = ensure_device();
     if (rc) return rc;
     mf_matrix m = new mf_matrix_s();

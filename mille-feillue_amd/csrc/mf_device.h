@@ -116,7 +116,9 @@ template <> struct Quad<3> {
     uint32_t a;
     __device__ void load(const uint8_t *p, size_t q) { a = reinterpret_cast<const uint32_t *>(p)[q]; }
     __device__ void zero() { a = 0u; }
-    template <int E> __device__ float get() const { return e4m3_to_f32((a >> (8 * E)) & 0xffu); }
+    // gfx950's FP8 conversion reads OCP E4M3; it equals e4m3_to_f32 on every
+    // non-NaN code (the layout never stores NaN)
+    template <int E> __device__ float get() const { return __builtin_amdgcn_cvt_f32_fp8((int)a, E); }
 };
 template <bool IDX8> struct IdxQuad;
 template <> struct IdxQuad<true> {
@@ -140,38 +142,6 @@ template <int N, typename F> __device__ __forceinline__ void static_for(F &&f)
     }
 }
 
-// One class slab, UQ quads per step: all value and index loads of the step
-// are issued before the gathers, and all gathers before the FMAs.  AccT is
-// double for class 0 (gather g64) and float otherwise (gather g32).  Padding
-// entries (k >= L_c) are loaded but not accumulated.
-template <int B, bool IDX8, int CLS, int UQ, typename AccT, typename GT>
-__device__ __forceinline__ void slab_spmv(const DevL &L, const RowCtx &c, int2 h, int r, int lanebase,
-                                          const GT *__restrict__ g, AccT &acc)
-{
-    const int Lc = h.y, nq = (Lc + 3) >> 2;
-    const uint8_t *vals = L.blob + (size_t)(unsigned)h.x * 8;
-    const uint8_t *idx = vals + (size_t)nq * B * 4 * cls_width(CLS);
-    for (int j0 = 0; j0 < nq; j0 += UQ) {
-        Quad<CLS> v[UQ];
-        IdxQuad<IDX8> id[UQ];
-#pragma unroll
-        for (int u = 0; u < UQ; u++) {
-            const size_t q = (size_t)min(j0 + u, nq - 1) * B + r;
-            v[u].load(vals, q);
-            id[u].load(idx, q);
-        }
-        GT xg[UQ][4];
-#pragma unroll
-        for (int u = 0; u < UQ; u++)
-            static_for<4>([&](auto e) { xg[u][e] = g[gather_index<B, IDX8>(id[u].template get<e>(), c, L, lanebase)]; });
-#pragma unroll
-        for (int u = 0; u < UQ; u++)
-            static_for<4>([&](auto e) {
-                if ((j0 + u) * 4 + e < Lc) acc = fma_t((AccT)v[u].template get<e>(), (AccT)xg[u][e], acc);
-            });
-    }
-}
-
 // Context of tile row tr from headers already at hand (t0, nt, slab
 // headers); only the tile-column word of the lane's slot is loaded.
 template <bool IDX8>
@@ -185,30 +155,6 @@ __device__ __forceinline__ RowCtx row_ctx_from(const DevL &L, int t0, int nt, in
     return c;
 }
 
-// y_r of tile row tr for lane r: FP64 tiles accumulate in FP64 over g64, then
-// FP32, FP16 and FP8 tiles (in that order) accumulate in FP32 over g32.
-template <int B, bool IDX8, int UQ = 2>
-__device__ __forceinline__ double tile_row_spmv_c(const DevL &L, const RowCtx &c, int r, int lanebase,
-                                                  const double *__restrict__ g64,
-                                                  const float *__restrict__ g32)
-{
-    double a64 = 0.0;
-    float a32 = 0.0f;
-    if (c.h01.y) slab_spmv<B, IDX8, 0, UQ>(L, c, make_int2(c.h01.x, c.h01.y), r, lanebase, g64, a64);
-    if (c.h01.w) slab_spmv<B, IDX8, 1, UQ>(L, c, make_int2(c.h01.z, c.h01.w), r, lanebase, g32, a32);
-    if (c.h23.y) slab_spmv<B, IDX8, 2, UQ>(L, c, make_int2(c.h23.x, c.h23.y), r, lanebase, g32, a32);
-    if (c.h23.w) slab_spmv<B, IDX8, 3, UQ>(L, c, make_int2(c.h23.z, c.h23.w), r, lanebase, g32, a32);
-    return a64 + (double)a32;
-}
-
-template <int B, bool IDX8, int UQ = 2>
-__device__ __forceinline__ double tile_row_spmv(const DevL &L, int tr, int r, int lanebase,
-                                                const double *__restrict__ g64,
-                                                const float *__restrict__ g32)
-{
-    return tile_row_spmv_c<B, IDX8, UQ>(L, row_ctx<B, IDX8>(L, tr, r), r, lanebase, g64, g32);
-}
-
 // Element idx of vector g with a 32-bit byte offset from the uniform base
 // (vectors are below 4 GB, which the host checks).
 template <typename T> __device__ __forceinline__ T ld_elem(const T *__restrict__ g, uint32_t idx)
@@ -216,14 +162,17 @@ template <typename T> __device__ __forceinline__ T ld_elem(const T *__restrict__
     return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(g) + (idx * (uint32_t)sizeof(T)));
 }
 
-// Tile column of slot of this lane's tile row: from the lane group's register
-// copy (one-byte index form) or from stc.
+// Element of the gathered vector for one slab index (slot << log2 B | col):
+// tcb is the lane's copy of tile column * B of slot (lane - lanebase) (one-byte
+// index form), fetched from the slot's lane with ds_bpermute; in the two-byte
+// form the tile column comes from stc.
 template <int B, bool IDX8>
-__device__ __forceinline__ uint32_t gather_col(uint32_t id, int tcr, int t0, const DevL &L, int lanebase)
+__device__ __forceinline__ uint32_t gather_col(uint32_t id, int tcb, int t0, const DevL &L, int lanebase)
 {
-    const int slot = (int)(id >> log2_of<B>());
-    const int tc = IDX8 ? __shfl(tcr, lanebase + slot, 64) : L.stc[t0 + slot];
-    return (uint32_t)tc * B + (id & (B - 1));
+    const uint32_t slot = id >> log2_of<B>(), col = id & (B - 1);
+    const int base = IDX8 ? __builtin_amdgcn_ds_bpermute((int)((lanebase + slot) << 2), tcb)
+                          : L.stc[t0 + (int)slot] * B;
+    return (uint32_t)base + col;
 }
 
 // Tile-row headers staged in LDS: hd[2 q] = {off0, L0, off1, L1} and
@@ -234,11 +183,12 @@ template <int CLS> __device__ __forceinline__ int2 lds_header(const int4 *hd, in
     return reinterpret_cast<const int2 *>(hd + 2 * q)[CLS];
 }
 
-template <bool IDX8> __device__ __forceinline__ int load_tcr(const DevL &L, const int *t0s, int q, int r)
+// tile column * B of slot r of local tile row q (one-byte index form)
+template <int B, bool IDX8> __device__ __forceinline__ int load_tcr(const DevL &L, const int *t0s, int q, int r)
 {
     if (!IDX8) return 0;
     const int t0 = t0s[q], nt = t0s[q + 1] - t0;
-    return r < nt ? L.stc[t0 + r] : 0;
+    return r < nt ? L.stc[t0 + r] * B : 0;
 }
 
 // One class slab of NR tile rows at once (local rows q0 + k * QS, this lane's
@@ -292,13 +242,14 @@ __device__ __forceinline__ void slab_spmv_rows(const DevL &L, const int4 *hd, co
                             ld_elem(g, gather_col<B, IDX8>(id[k][u].template get<e>(), tcr[kb + k], t0, L, lanebase));
                     });
             }
+            // padding entries (value 0) add +0 or -0 to a sum that starts at +0
+            // and is never -0, so they leave it unchanged
 #pragma unroll
             for (int k = 0; k < NB; k++)
 #pragma unroll
                 for (int u = 0; u < UQ; u++)
                     static_for<4>([&](auto e) {
-                        if ((j0 + u) * 4 + e < h[k].y)
-                            acc[kb + k] = fma_t((AccT)v[k][u].template get<e>(), (AccT)xg[k][u][e], acc[kb + k]);
+                        acc[kb + k] = fma_t((AccT)v[k][u].template get<e>(), (AccT)xg[k][u][e], acc[kb + k]);
                     });
         }
     }
@@ -313,7 +264,7 @@ __device__ __forceinline__ void tile_rows_spmv(const DevL &L, const int4 *hd, co
 {
     int tcr[NR];
 #pragma unroll
-    for (int k = 0; k < NR; k++) tcr[k] = load_tcr<IDX8>(L, t0s, q0 + k * QS, r);
+    for (int k = 0; k < NR; k++) tcr[k] = load_tcr<B, IDX8>(L, t0s, q0 + k * QS, r);
     double a64[NR];
     float a32[NR];
 #pragma unroll
@@ -405,47 +356,6 @@ __device__ __forceinline__ unsigned ld_agent(const unsigned *p)
 __device__ __forceinline__ int ld_agent(const int *p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Grid barrier over monotone arrival counters (never reset inside a launch,
-// so there is no reset race).  Blocks are split into 8 groups by blockIdx % 8
-// (the XCD the dispatcher deals them to); each block adds 1 to its group's
-// counter, the group's last arriver adds 1 to the global counter, and every
-// block waits for the global counter to reach barriers_passed * groups.
-// Writes before the barrier are released at agent scope and acquired after
-// it.  Spreading the arrivals over 9 counters instead of one is what keeps
-// a barrier of 256 blocks at a few microseconds.  The wait is bounded: after
-// spin_limit polls the block raises the abort word, and every waiter leaves
-// when it sees it.  Returns false on abort (the caller exits the kernel).
-// Counter layout: bar[0] global, bar[32 * (1 + x)] group x (separate lines).
-__device__ __forceinline__ bool grid_barrier(unsigned *bar, int *abort_word, unsigned ep,
-                                             long long spin_limit, int *s_flag)
-{
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned G = gridDim.x, x = blockIdx.x & 7u;
-        const unsigned groups = G < 8u ? G : 8u;
-        const unsigned nx = (G >> 3) + (x < (G & 7u) ? 1u : 0u);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        const unsigned old =
-            __hip_atomic_fetch_add(bar + 32 * (1 + x), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (old + 1u == ep * nx) __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        int ab = 0;
-        long long spins = 0;
-        while (ld_agent(bar) < ep * groups) {
-            if (ld_agent(abort_word)) { ab = 1; break; }
-            if (++spins > spin_limit) {
-                __hip_atomic_store(abort_word, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                ab = 1;
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        *s_flag = ab;
-    }
-    __syncthreads();
-    return *s_flag == 0;
 }
 
 }  // namespace mf

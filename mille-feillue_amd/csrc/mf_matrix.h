@@ -44,8 +44,7 @@ struct Workspace {
     int G = 0;
     double *g64 = nullptr, *h64 = nullptr, *q64 = nullptr;
     float *g32 = nullptr;
-    double *partials = nullptr;
-    unsigned *bar = nullptr;
+    unsigned long long *slots = nullptr;
     int *ctl = nullptr;
     double *out = nullptr;
     double *r_outer = nullptr, *d = nullptr, *tmp = nullptr;

@@ -4,14 +4,14 @@
 // workgroup is resident).  Workgroup g owns a contiguous chunk of tile rows
 // (XCD-aware chunk order); thread (group, lane) owns row lane of tile rows
 // chunk_start + i * groups + group for i < RPT, and keeps every Krylov vector
-// of those rows in registers for the whole solve.  Only the vector that the
-// SpMV gathers is published to global memory each iteration.  Reductions are
-// deterministic: per-workgroup partials are written to a slot and every
-// workgroup sums all partials in the same fixed order after the barrier, so
-// all workgroups take identical control decisions.
+// of those rows in registers or LDS for the whole solve.  Only the vector that
+// the SpMV gathers is published to global memory each iteration.  Reductions
+// are deterministic: every workgroup sums all workgroups' partials in the same
+// fixed order (Reducer), so all take identical control decisions.
 //
-// CG uses the Chronopoulos-Gear form of Hestenes-Stiefel CG (one SpMV and one
-// fused reduction of (r,r) and (Ar,r) per iteration: two grid barriers).  The
+// CG uses the Chronopoulos-Gear form of Hestenes-Stiefel CG (one SpMV per
+// iteration; (Ar, r) is reduced after it and (r, r) together with the
+// publication of r: two grid syncs per iteration).  The
 // reference's recurrences (PC/Mille-feuille_CG_AMD.cu:607-912: q = A d,
 // alpha = snew/(d.q), x += alpha d, r -= alpha q, beta = snew/sold,
 // d = r + beta d) are the same method in exact arithmetic.
@@ -107,7 +107,7 @@ __global__ void k_axpy(double *__restrict__ x, const double *__restrict__ d, lon
 }
 
 // --------------------------------------------------------- solver kernels
-constexpr int BAR_WORDS = 32 * 9;  // barrier counters: global + 8 groups
+constexpr int SYNC_K = 4;  // values per grid allreduce, at most
 
 struct SolveArgs {
     DevL L;
@@ -117,8 +117,7 @@ struct SolveArgs {
     double *g64;       // published Krylov vector (FP64)
     float *g32;        // its FP32 copy for low-precision tiles
     double *h64, *q64; // publishing slots of the true-residual pass
-    double *partials;  // [4][G][8]
-    unsigned *bar;
+    unsigned long long *slots;  // sync words [2][G][SYNC_K][2]
     int *ctl;          // 0 abort, 1 status, 2 iterations, 3 replacements
     double *out;       // 0 ||b||, 1 final ||r_true||^2, 2 final updated ||r||^2
     double tol, rr_factor;
@@ -149,64 +148,125 @@ struct PhaseClock {
 
 constexpr unsigned PHASE_EP0 = 201;
 
+// Stores of vector entries that other workgroups read after the next grid
+// sync: agent-scope stores, complete (visible to every XCD) once the storing
+// wave's memory counter has drained.
+__device__ __forceinline__ void st_agent(double *p, double v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(float *p, float v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Grid synchronisation and deterministic allreduce without read-modify-write
+// atomics or cache write-back.  Every workgroup writes each FP64 partial as
+// two 64-bit words, each holding the sync epoch in its high half and one half
+// of the partial in its low half; a word is stored and loaded whole, so a
+// word showing the current epoch holds the current half whatever order the
+// words arrive in.  Wave 0 of every workgroup polls all G x N x 2 words until
+// every epoch matches, then sums the partials in workgroup order (lane-strided
+// sums, then a fixed butterfly), so all workgroups obtain identical bits and
+// take identical branches.  Two word buffers alternate by epoch parity: a
+// workgroup writes buffer p at epoch e only after passing epoch e - 1, which
+// every workgroup reaches only after it has read buffer p at epoch e - 2.
+// publish: vector entries written before the sync (with st_agent) are read
+// by other workgroups after it -- each wave drains its stores before the
+// workgroup barrier that precedes the epoch words, and the readers invalidate
+// their caches (agent-scope acquire) after the poll.  The wait is bounded:
+// after spin_limit polls the workgroup raises the abort word and every
+// workgroup leaves (returns false).
 template <int TPB, int K>
 struct Reducer {
+    static_assert(K <= SYNC_K, "reducer width");
     double *s_red;  // [TPB/64][K]
     double *s_bc;   // [K]
     int *s_flag;
     unsigned ep = 0;
-    int slot = 0;
 
-    __device__ bool barrier(const SolveArgs &A)
-    {
-        ++ep;
-        // diagnostic stamps of barriers PHASE_EP0 .. PHASE_EP0 + 3: arrival and
-        // exit time of every workgroup
-        const bool stamp = A.phase && threadIdx.x == 0 && ep - PHASE_EP0 < 4u;
-        unsigned long long *st = stamp ? A.phase + 16 + (size_t)(ep - PHASE_EP0) * 2 * A.G + 2 * blockIdx.x : nullptr;
-        if (stamp) st[0] = wall_clock64();
-        const bool ok = grid_barrier(A.bar, A.ctl, ep, A.spin_limit, s_flag);
-        if (stamp) st[1] = wall_clock64();
-        return ok;
-    }
-
-    // Sums v over the grid; every workgroup obtains the same bits.
     template <int N>
-    __device__ bool allreduce(const SolveArgs &A, double (&v)[N])
+    __device__ bool allreduce(const SolveArgs &A, double (&v)[N], bool publish)
     {
-        static_assert(N <= K, "reducer width");
+        static_assert(N >= 1 && N <= K, "reducer width");
         const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
 #pragma unroll
         for (int k = 0; k < N; k++) {
             double s = wave_sum(v[k]);
             if (lane == 0) s_red[wv * K + k] = s;
         }
+        if (publish) __builtin_amdgcn_s_waitcnt(0);  // this wave's stores are complete
         __syncthreads();
-        const int sl = slot;
-        slot = (slot + 1) & 3;
-        if (tid == 0) {
-#pragma unroll
-            for (int k = 0; k < N; k++) {
+        ++ep;
+        // diagnostic stamps of syncs PHASE_EP0 .. PHASE_EP0 + 3: arrival and
+        // exit time of every workgroup
+        const bool stamp = A.phase && tid == 0 && ep - PHASE_EP0 < 4u;
+        unsigned long long *st = stamp ? A.phase + 16 + (size_t)(ep - PHASE_EP0) * 2 * A.G + 2 * blockIdx.x : nullptr;
+        if (stamp) st[0] = wall_clock64();
+        if (tid < 64) {
+            const unsigned G = (unsigned)A.G;
+            // each workgroup's words fill one 64-byte line of their own (stores
+            // of different workgroups never share a line)
+            unsigned long long *W = A.slots + (size_t)(ep & 1u) * G * SYNC_K * 2;
+            const unsigned long long tag = (unsigned long long)ep << 32;
+            if (lane < 2 * N) {
+                const int k = lane >> 1;
                 double s = 0.0;
                 for (int w = 0; w < TPB / 64; w++) s += s_red[w * K + k];
-                A.partials[((size_t)sl * A.G + blockIdx.x) * 8 + k] = s;
+                const unsigned long long bits = (unsigned long long)__double_as_longlong(s);
+                const unsigned long long half = (lane & 1) ? (bits >> 32) : (bits & 0xffffffffull);
+                __hip_atomic_store(W + ((size_t)blockIdx.x * SYNC_K + k) * 2 + (lane & 1), tag | half,
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
-        }
-        if (!barrier(A)) return false;
-        if (tid < 64) {
+            double acc[N];
+            long long spins = 0;
+            int ab = 0;
+            for (;;) {
+                bool ok = true;
 #pragma unroll
-            for (int k = 0; k < N; k++) {
-                double s = 0.0;
-                for (int g = tid; g < A.G; g += 64) s += A.partials[((size_t)sl * A.G + g) * 8 + k];
-                s = wave_sum(s);
-                if (tid == 0) s_bc[k] = s;
+                for (int k = 0; k < N; k++) acc[k] = 0.0;
+                for (unsigned g = lane; g < G; g += 64) {
+#pragma unroll
+                    for (int k = 0; k < N; k++) {
+                        const unsigned long long *w = W + ((size_t)g * SYNC_K + k) * 2;
+                        const unsigned long long lo = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        const unsigned long long hi = __hip_atomic_load(w + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        ok = ok && (lo >> 32) == ep && (hi >> 32) == ep;
+                        acc[k] += __longlong_as_double((long long)((hi << 32) | (lo & 0xffffffffull)));
+                    }
+                }
+                if (__all(ok)) break;
+                if (__builtin_amdgcn_readfirstlane(ld_agent(A.ctl))) { ab = 1; break; }
+                if (++spins > A.spin_limit) {
+                    if (lane == 0) __hip_atomic_store(A.ctl, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ab = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
             }
+            if (!ab) {
+#pragma unroll
+                for (int k = 0; k < N; k++) {
+                    const double s = wave_sum(acc[k]);
+                    if (lane == 0) s_bc[k] = s;
+                }
+                if (publish) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            }
+            if (lane == 0) *s_flag = ab;
         }
+        if (stamp) st[1] = wall_clock64();
         __syncthreads();
+        if (*s_flag) return false;
 #pragma unroll
         for (int k = 0; k < N; k++) v[k] = s_bc[k];
-        __syncthreads();
         return true;
+    }
+
+    // Grid barrier after publishing vector entries.
+    __device__ bool barrier(const SolveArgs &A)
+    {
+        double z[1] = {0.0};
+        return allreduce(A, z, true);
     }
 };
 
@@ -269,8 +329,8 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
     auto tile_row = [&](int i) { return tr0 + i * GPB + grp; };
     auto row_of = [&](int i) { return (size_t)(rbase + i * TPB); };
     auto publish = [&](size_t row, double v) {
-        if (A.has64) A.g64[row] = v;
-        if (A.haslow) A.g32[row] = (float)v;
+        if (A.has64) st_agent(A.g64 + row, v);
+        if (A.haslow) st_agent(A.g32 + row, (float)v);
     };
     auto finish = [&](int status, int iter, int repl, double rt2, double g2) {
 #pragma clang loop unroll(full)
@@ -299,7 +359,7 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
             v1[0] += bv * bv;
         }
     }
-    if (!R.allreduce(A, v1)) return;
+    if (!R.allreduce(A, v1, true)) return;
     const double bnorm2 = v1[0];
     if (blockIdx.x == 0 && tid == 0) A.out[0] = sqrt(bnorm2);
     const double tol2 = A.tol * A.tol * bnorm2;
@@ -317,8 +377,8 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
 #pragma clang loop unroll(full)
             for (int i = 0; i < RPT; i++)
                 if (tile_row(i) < tr1) {
-                    A.h64[row_of(i)] = xr[i];
-                    A.q64[row_of(i)] = pr[i];
+                    st_agent(A.h64 + row_of(i), xr[i]);
+                    st_agent(A.q64 + row_of(i), pr[i]);
                 }
             if (!R.barrier(A)) return;
             double t1[1] = {0.0};
@@ -337,7 +397,7 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
                     t1[0] += rv * rv;
                 }
             }
-            if (!R.allreduce(A, t1)) return;
+            if (!R.allreduce(A, t1, true)) return;
             clk.lap(4);
             ++repl;
             if (!isfinite(t1[0])) { finish(MF_E_NONFINITE, iter, repl, t1[0], gamma); return; }
@@ -363,7 +423,7 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
             }
         }
         clk.lap(0);
-        if (!R.allreduce(A, red)) return;
+        if (!R.allreduce(A, red, false)) return;
         clk.lap(1);
         const double delta = red[0];
         const double denom = iter ? delta - beta * gamma / aprev : delta;
@@ -389,7 +449,7 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
         aprev = alpha;
         ++iter;
         clk.lap(2);
-        if (!R.allreduce(A, g1)) return;
+        if (!R.allreduce(A, g1, true)) return;
         clk.lap(3);
         gamma = g1[0];
     }
@@ -436,8 +496,8 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
     auto tile_row = [&](int i) { return tr0 + i * GPB + grp; };
     auto row_of = [&](int i) { return (size_t)(rbase + i * TPB); };
     auto publish = [&](size_t row, double v) {
-        if (A.has64) A.g64[row] = v;
-        if (A.haslow) A.g32[row] = (float)v;
+        if (A.has64) st_agent(A.g64 + row, v);
+        if (A.haslow) st_agent(A.g32 + row, (float)v);
     };
     auto finish = [&](int status, int iter, int repl, double rt2, double g2) {
 #pragma clang loop unroll(full)
@@ -474,7 +534,7 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
             v1[0] += rr[i] * rr[i];
         }
     }
-    if (!R.allreduce(A, v1)) return;
+    if (!R.allreduce(A, v1, true)) return;
     const double bnorm2 = v1[0];
     if (blockIdx.x == 0 && tid == 0) A.out[0] = sqrt(bnorm2);
     const double tol2 = A.tol * A.tol * bnorm2;
@@ -488,7 +548,7 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
 #pragma clang loop unroll(full)
         for (int i = 0; i < RPT; i++)
             if (tile_row(i) < tr1) a1[0] += rh[i] * s_v[i * TPB + tid];
-        if (!R.allreduce(A, a1)) return;
+        if (!R.allreduce(A, a1, false)) return;
         if (!(a1[0] != 0.0) || !isfinite(a1[0])) { finish(MF_E_BREAKDOWN, iter, repl, -1.0, -1.0); return; }
         const double alpha = rho / a1[0];
 #pragma clang loop unroll(full)
@@ -507,7 +567,7 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
                 a2[0] += t * sr[i];
                 a2[1] += t * t;
             }
-        if (!R.allreduce(A, a2)) return;
+        if (!R.allreduce(A, a2, false)) return;
         const double omega = a2[1] > 0.0 ? a2[0] / a2[1] : 0.0;
         double a3[2] = {0.0, 0.0};
 #pragma clang loop unroll(full)
@@ -518,7 +578,7 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
                 a3[0] += rr[i] * rh[i];
                 a3[1] += rr[i] * rr[i];
             }
-        if (!R.allreduce(A, a3)) return;
+        if (!R.allreduce(A, a3, false)) return;
         double rho_new = a3[0], g2 = a3[1];
         ++iter;
         if (!finite_all(rho_new, g2)) { finish(MF_E_NONFINITE, iter, repl, -1.0, g2); return; }
@@ -527,7 +587,7 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
         if (conv || due) {
 #pragma clang loop unroll(full)
             for (int i = 0; i < RPT; i++)
-                if (tile_row(i) < tr1) A.h64[row_of(i)] = xr[i];
+                if (tile_row(i) < tr1) st_agent(A.h64 + row_of(i), xr[i]);
             if (!R.barrier(A)) return;
 #pragma unroll 1
             for (int i = 0; i < RPT; i++) {
@@ -547,7 +607,7 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
                     t2[0] += rr[i] * rr[i];
                     t2[1] += rr[i] * rh[i];
                 }
-            if (!R.allreduce(A, t2)) return;
+            if (!R.allreduce(A, t2, false)) return;
             ++repl;
             if (!isfinite(t2[0])) { finish(MF_E_NONFINITE, iter, repl, t2[0], g2); return; }
             if (t2[0] <= tol2) { finish(MF_OK, iter, repl, t2[0], g2); return; }
@@ -606,7 +666,7 @@ static KernelChoice pick(int B, bool idx8, int method, long long rows_per_wg)
 
 void Workspace::release()
 {
-    void *ps[] = {g64, h64, q64, g32, partials, bar, ctl, out, r_outer, d, tmp, red};
+    void *ps[] = {g64, h64, q64, g32, slots, ctl, out, r_outer, d, tmp, red};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     *this = Workspace();
@@ -624,8 +684,7 @@ static int ensure_ws(mf_matrix m, int G)
     MF_HIP(hipMalloc((void **)&w.h64, sizeof(double) * ncol));
     MF_HIP(hipMalloc((void **)&w.q64, sizeof(double) * ncol));
     MF_HIP(hipMalloc((void **)&w.g32, sizeof(float) * ncol));
-    MF_HIP(hipMalloc((void **)&w.partials, sizeof(double) * 4 * 8 * w.G));
-    MF_HIP(hipMalloc((void **)&w.bar, sizeof(unsigned) * BAR_WORDS));
+    MF_HIP(hipMalloc((void **)&w.slots, sizeof(unsigned long long) * 2 * SYNC_K * 2 * w.G));
     MF_HIP(hipMalloc((void **)&w.ctl, sizeof(int) * 8));
     MF_HIP(hipMalloc((void **)&w.out, sizeof(double) * 4));
     MF_HIP(hipMalloc((void **)&w.r_outer, sizeof(double) * ncol));
@@ -647,9 +706,12 @@ struct InnerResult {
 
 // Zeroes the barrier counters (global + 8 groups, one 128-byte line apart)
 // and the control words with a kernel (no memset nodes).
-__global__ void k_reset_ctl(unsigned *bar, int *ctl)
+// Clears the sync words (epoch 0 never occurs in a launch) and the control
+// words before every persistent launch (a fill kernel, so the launch
+// sequence can be captured in a graph).
+__global__ void k_reset_ctl(unsigned long long *slots, int nslots, int *ctl)
 {
-    for (int i = threadIdx.x; i < BAR_WORDS; i += blockDim.x) bar[i] = 0u;
+    for (int i = threadIdx.x; i < nslots; i += blockDim.x) slots[i] = 0ull;
     if (threadIdx.x < 8) ctl[threadIdx.x] = 0;
 }
 
@@ -681,8 +743,7 @@ static int launch_inner(mf_matrix m, const Layout &L, int method, const double *
     A.g32 = w.g32;
     A.h64 = w.h64;
     A.q64 = w.q64;
-    A.partials = w.partials;
-    A.bar = w.bar;
+    A.slots = w.slots;
     A.ctl = w.ctl;
     A.out = w.out;
     A.tol = tol;
@@ -697,7 +758,7 @@ static int launch_inner(mf_matrix m, const Layout &L, int method, const double *
     if (phase_profile && !d_phase) MF_HIP(hipMalloc((void **)&d_phase, phase_words * sizeof(unsigned long long)));
     A.phase = phase_profile ? d_phase : nullptr;
     if (A.phase) MF_HIP(hipMemsetAsync(d_phase, 0, phase_words * sizeof(unsigned long long), st));
-    k_reset_ctl<<<1, 256, 0, st>>>(w.bar, w.ctl);
+    k_reset_ctl<<<1, 1024, 0, st>>>(w.slots, 2 * SYNC_K * 2 * G, w.ctl);
     MF_HIP(hipGetLastError());
     void *args[] = {&A};
     // Cooperative launch guarantees that all G workgroups are resident.  Some

@@ -62,7 +62,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--grid", type=int, default=128)
-    ap.add_argument("--rr", type=float, default=1e-2, help="residual replacement factor (mixed)")
+    ap.add_argument("--rr", type=float, default=1e-4, help="residual replacement factor (mixed)")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -321,10 +321,6 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
         s_t0[q] = A.L.tile_ptr[min(tr, A.tilem)];
     }
     __syncthreads();
-    auto ctx = [&](int i) {
-        const int q = i * GPB + grp;
-        return row_ctx_from<IDX8>(A.L, s_t0[q], s_t0[q + 1] - s_t0[q], s_h[2 * q], s_h[2 * q + 1], lane);
-    };
     int rbase = tr0 * B + tid;  // row of step i is rbase + i * TPB
     auto tile_row = [&](int i) { return tr0 + i * GPB + grp; };
     auto row_of = [&](int i) { return (size_t)(rbase + i * TPB); };
@@ -365,13 +361,17 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
     const double tol2 = A.tol * A.tol * bnorm2;
     if (bnorm2 == 0.0) { finish(MF_OK, 0, 0, 0.0, 0.0); return; }
     double gamma = bnorm2, ref2 = bnorm2, gprev = 1.0, aprev = 1.0;
+    bool restart = false;   // next iteration starts a new direction (beta = 0)
+    bool replaced = false;  // r was replaced before this iteration
+    bool force = false;     // a breakdown asks for a true-residual pass + restart
+    double rs_repl = 0.0, ps_repl = 0.0;
     int iter = 0, repl = 0;
     PhaseClock clk(A.phase);
     for (;;) {
         rbase = opaque(rbase);
         if (!isfinite(gamma)) { finish(MF_E_NONFINITE, iter, repl, -1.0, gamma); return; }
         const bool conv = gamma <= tol2;
-        const bool due = A.rr_factor > 0.0 && gamma <= A.rr_factor * A.rr_factor * ref2;
+        const bool due = force || (A.rr_factor > 0.0 && gamma <= A.rr_factor * A.rr_factor * ref2);
         if (conv || due) {
             // true residual b - A x and s = A p in one FP64-accumulated pass
 #pragma clang loop unroll(full)
@@ -381,20 +381,26 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
                     st_agent(A.q64 + row_of(i), pr[i]);
                 }
             if (!R.barrier(A)) return;
-            double t1[1] = {0.0};
+            // (r, r), (r, s), (p, s) of the replaced r and refreshed s = A p
+            double t1[3] = {0.0, 0.0, 0.0};
 #pragma unroll 1
             for (int i = 0; i < RPT; i++) {
                 const int tr = tile_row(i);
                 if (tr < tr1) {
                     const double *g[2] = {A.h64, A.q64};
                     double acc[2] = {0.0, 0.0};
-                    tile_row_spmv_f64_c<B, IDX8, 2>(A.L, ctx(i), lane, lanebase, g, acc);
+                    const int q = i * GPB + grp;
+                    tile_row_spmv_f64_c<B, IDX8, 2>(
+                        A.L, row_ctx_from<IDX8>(A.L, s_t0[q], s_t0[q + 1] - s_t0[q], s_h[2 * q], s_h[2 * q + 1], lane),
+                        lane, lanebase, g, acc);
                     const size_t row = (size_t)tr * B + lane;
                     const double rv = A.b[row] - acc[0];
                     s_r[i * TPB + tid] = rv;
                     s_s[i * TPB + tid] = acc[1];
                     publish(row, rv);
                     t1[0] += rv * rv;
+                    t1[1] += rv * acc[1];
+                    t1[2] += pr[i] * acc[1];
                 }
             }
             if (!R.allreduce(A, t1, true)) return;
@@ -402,11 +408,23 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
             ++repl;
             if (!isfinite(t1[0])) { finish(MF_E_NONFINITE, iter, repl, t1[0], gamma); return; }
             if (t1[0] <= tol2) { finish(MF_OK, iter, repl, t1[0], gamma); return; }
+            // reliable update: keep the direction p.  The next step's
+            // (p', A p') with p' = r + beta p, A p' = A r + beta s is then
+            // (A r, r) + 2 beta (r, s) + beta^2 (p, s) (A symmetric), not the
+            // recurrence's form.  When the recurrence had drifted by more
+            // than a factor 2 in norm, restart from the true residual.
+            restart = force || t1[0] > 4.0 * gamma;
+            force = false;
+            replaced = true;
+            rs_repl = t1[1];
+            ps_repl = t1[2];
             gamma = ref2 = t1[0];
             continue;
         }
         if (iter >= A.maxit) { finish(MF_E_MAXIT, iter, repl, -1.0, gamma); return; }
-        const double beta = iter ? gamma / gprev : 0.0;
+        const bool fresh = iter == 0 || restart, after_repl = replaced && !fresh;
+        restart = replaced = false;
+        const double beta = fresh ? 0.0 : gamma / gprev;
         // w = A r; s = w + beta s; (w, r)
         double red[1] = {0.0};
 #pragma unroll 1
@@ -426,9 +444,17 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
         if (!R.allreduce(A, red, false)) return;
         clk.lap(1);
         const double delta = red[0];
-        const double denom = iter ? delta - beta * gamma / aprev : delta;
+        const double denom = fresh        ? delta
+                             : after_repl ? delta + beta * (2.0 * rs_repl + beta * ps_repl)
+                                          : delta - beta * gamma / aprev;
         if (!isfinite(denom)) { finish(MF_E_NONFINITE, iter, repl, -1.0, gamma); return; }
-        if (!(denom > 0.0)) { finish(MF_E_BREAKDOWN, iter, repl, -1.0, gamma); return; }
+        if (!(denom > 0.0)) {
+            // (p, A p) <= 0 from rounding: recompute the residual and restart;
+            // a breakdown right after a restart is final
+            if (fresh) { finish(MF_E_BREAKDOWN, iter, repl, -1.0, gamma); return; }
+            force = true;
+            continue;
+        }
         const double alpha = gamma / denom;
 #pragma clang loop unroll(full)
         for (int i = 0; i < RPT; i++)
@@ -488,10 +514,6 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
         s_t0[q] = A.L.tile_ptr[min(tr, A.tilem)];
     }
     __syncthreads();
-    auto ctx = [&](int i) {
-        const int q = i * GPB + grp;
-        return row_ctx_from<IDX8>(A.L, s_t0[q], s_t0[q + 1] - s_t0[q], s_h[2 * q], s_h[2 * q + 1], lane);
-    };
     int rbase = tr0 * B + tid;  // row of step i is rbase + i * TPB
     auto tile_row = [&](int i) { return tr0 + i * GPB + grp; };
     auto row_of = [&](int i) { return (size_t)(rbase + i * TPB); };
@@ -540,6 +562,7 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
     const double tol2 = A.tol * A.tol * bnorm2;
     if (bnorm2 == 0.0) { finish(MF_OK, 0, 0, 0.0, 0.0); return; }
     double rho = bnorm2, ref2 = bnorm2;
+    bool restart = false;
     int iter = 0, repl = 0;
     for (;;) {
         rbase = opaque(rbase);
@@ -595,7 +618,10 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
                 if (tr < tr1) {
                     const double *g[1] = {A.h64};
                     double acc[1] = {0.0};
-                    tile_row_spmv_f64_c<B, IDX8, 1>(A.L, ctx(i), lane, lanebase, g, acc);
+                    const int q = i * GPB + grp;
+                    tile_row_spmv_f64_c<B, IDX8, 1>(
+                        A.L, row_ctx_from<IDX8>(A.L, s_t0[q], s_t0[q + 1] - s_t0[q], s_h[2 * q], s_h[2 * q + 1], lane),
+                        lane, lanebase, g, acc);
                     s_t[i * TPB + tid] = A.b[(size_t)tr * B + lane] - acc[0];
                 }
             }
@@ -611,12 +637,14 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
             ++repl;
             if (!isfinite(t2[0])) { finish(MF_E_NONFINITE, iter, repl, t2[0], g2); return; }
             if (t2[0] <= tol2) { finish(MF_OK, iter, repl, t2[0], g2); return; }
+            restart = t2[0] > 4.0 * g2;  // recurrence drifted: new direction p = r
             ref2 = t2[0];
             rho_new = t2[1];
         }
         if (iter >= A.maxit) { finish(MF_E_MAXIT, iter, repl, -1.0, g2); return; }
         if (omega == 0.0 || rho == 0.0) { finish(MF_E_BREAKDOWN, iter, repl, -1.0, g2); return; }
-        const double beta = (rho_new / rho) * (alpha / omega);
+        const double beta = restart ? 0.0 : (rho_new / rho) * (alpha / omega);
+        restart = false;
 #pragma clang loop unroll(full)
         for (int i = 0; i < RPT; i++)
             if (tile_row(i) < tr1) {

@@ -64,6 +64,8 @@ def main():
     ap.add_argument("--grid", type=int, default=128)
     ap.add_argument("--rr", type=float, default=1e-4, help="residual replacement factor (mixed)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib", action="store_true",
+                    help="also run a plain 512 MiB device copy (known bytes) to calibrate PMC byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -132,18 +134,41 @@ def main():
     log(f"fp64: {ms64:.3f} ms, {it64} it; mixed: {msmx:.3f} ms, {itmx} it, "
         f"{repsmx[0]['replacements']} true-residual passes")
 
+    if args.calib:
+        src = torch.ones(64 << 20, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {src.numel() * 8} bytes read and written per launch, 3 launches")
+        del src, dst
+
     # standalone SpMV kernel
     spmv64 = A.spmv_bench(50, mixed=False)
     spmvmx = A.spmv_bench(50, mixed=True)
     b64, bmx = info["bytes_alg_fp64"], info["bytes_alg_mixed"]
 
-    # dominant kernel: the persistent CG launch; one SpMV's algorithmic bytes
-    # per Krylov iteration (true-residual passes not counted)
-    ach = itmx * bmx / (msmx * 1e-3)
+    # dominant kernel: the persistent CG launch.  Unit = one CG iteration,
+    # algorithmic bytes B_alg + 56 n (the SpMV plus x, r, d read and written
+    # and q re-read by a kernel that streams its vectors); true-residual
+    # passes are not counted.
+    iter_bytes = bmx + 56.0 * n
+    ach = itmx * iter_bytes / (msmx * 1e-3)
+    traffic = None
+    pmc_file = os.path.join(ROOT, "profiles", "k_cg_poisson128_pmc.json")
+    if g == 128 and os.path.exists(pmc_file):
+        with open(pmc_file) as f:
+            pmc = json.load(f)
+        # calibrated FETCH_SIZE + WRITE_SIZE of the mixed launch, per iteration
+        traffic = round(pmc["mixed"]["traffic_bytes_per_iteration"] * itmx)
     roof = {"bound": "hbm", "achieved": round(ach / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK, 4), "traffic": None,
+            "frac": round(ach / HBM_PEAK, 4), "traffic": traffic,
             "kernel": "k_cg<16,true,512,16> (persistent mixed CG, one launch per solve)",
-            "bytes_per_unit": bmx, "units_per_launch": itmx, "launch_ms": round(msmx, 4)}
+            "bytes_per_unit": iter_bytes, "unit_def": "one CG iteration: B_alg + 56 n bytes",
+            "units_per_launch": itmx, "launch_ms": round(msmx, 4),
+            "spmv_bytes_only_GBps": round(itmx * bmx / (msmx * 1e-3) / 1e9, 1),
+            "traffic_source": "profiles/k_cg_poisson128_pmc.json (rocprofv3 FETCH_SIZE x2.000 + WRITE_SIZE x1.000)"
+            if traffic is not None else None}
 
     cpu = None
     if not args.no_cpu:
@@ -183,7 +208,7 @@ def main():
             "mixed": {"ms": round(spmvmx, 5), "GB/s": round(bmx / spmvmx / 1e6, 1),
                       "frac": round(bmx / (spmvmx * 1e-3) / HBM_PEAK, 4), "bytes_alg": bmx},
         },
-        "solver_GBps": {"fp64": round(it64 * b64 / ms64 / 1e6, 1), "mixed": round(ach / 1e9, 1)},
+        "solver_GBps": {"fp64": round(it64 * (b64 + 56.0 * n) / ms64 / 1e6, 1), "mixed": round(ach / 1e9, 1)},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

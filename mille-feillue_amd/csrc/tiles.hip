@@ -461,26 +461,38 @@ __global__ __launch_bounds__(256) void k_slab_sizes(const int *__restrict__ tile
                                                     const uint8_t *__restrict__ codes, int iw,
                                                     unsigned long long *units, unsigned long long *stats)
 {
+    // tiles and nonzeros per code: per-thread counts, summed in LDS, then one
+    // global add per block and code
+    __shared__ unsigned long long s_stats[10];
+    if (threadIdx.x < 10) s_stats[threadIdx.x] = 0ull;
+    __syncthreads();
     const int tr = blockIdx.x * 256 + threadIdx.x;
-    if (tr >= tilem) return;
-    int len[4][B];
-    for (int c = 0; c < 4; c++)
-        for (int r = 0; r < B; r++) len[c][r] = 0;
-    for (int t = tile_ptr[tr]; t < tile_ptr[tr + 1]; t++) {
-        const int code = codes ? codes[t] : 0, c = code_to_cls(code);
-        const int s = tile_start[t], e = tile_start[t + 1];
-        for (int k = s; k < e; k++) len[c][rowtmp[k]]++;
-        atomicAdd(&stats[code], 1ull);
-        atomicAdd(&stats[5 + code], (unsigned long long)(e - s));
-    }
     unsigned long long u = 0;
-    for (int c = 0; c < 4; c++) {
-        int Lc = 0;
-        for (int r = 0; r < B; r++) Lc = max(Lc, len[c][r]);
-        u += (unsigned long long)((Lc + 3) >> 2) * B * 4 * (cls_width(c) + iw) / 8;
+    if (tr < tilem) {
+        int len[4][B];
+        unsigned long long cnt[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int c = 0; c < 4; c++)
+            for (int r = 0; r < B; r++) len[c][r] = 0;
+        for (int t = tile_ptr[tr]; t < tile_ptr[tr + 1]; t++) {
+            const int code = codes ? codes[t] : 0, c = code_to_cls(code);
+            const int s = tile_start[t], e = tile_start[t + 1];
+            for (int k = s; k < e; k++) len[c][rowtmp[k]]++;
+            cnt[code]++;
+            cnt[5 + code] += (unsigned long long)(e - s);
+        }
+        for (int k = 0; k < 10; k++)
+            if (cnt[k]) atomicAdd(&s_stats[k], cnt[k]);
+        for (int c = 0; c < 4; c++) {
+            int Lc = 0;
+            for (int r = 0; r < B; r++) Lc = max(Lc, len[c][r]);
+            u += (unsigned long long)((Lc + 3) >> 2) * B * 4 * (cls_width(c) + iw) / 8;
+        }
+        units[tr] = u;
     }
-    units[tr] = u;
+    __syncthreads();
+    if (threadIdx.x < 10 && s_stats[threadIdx.x]) atomicAdd(&stats[threadIdx.x], s_stats[threadIdx.x]);
 }
+
 
 template <int B>
 __global__ __launch_bounds__(256) void k_slab_pack(const int *__restrict__ tile_ptr, int tilem,

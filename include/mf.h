@@ -185,6 +185,31 @@ int mf_spmv(mf_matrix m, int mixed, const double *x, double *y);
  * returns the mean kernel time in ms (device events). */
 int mf_spmv_bench(mf_matrix m, int mixed, int reps, double *ms);
 
+/* Batched solver object: nsys independent square SPD systems (CSR), each
+ * with nrhs (1..8) right-hand sides, solved by CG in one persistent launch
+ * captured in a hipGraph.  Tiles are built on the GPU per system and joined
+ * into one block-diagonal layout; opts->block picks the tile size (8, 16 or
+ * 32; 0 = 16), opts->grid the target number of workgroups (0 = 2 per CU),
+ * opts->precision/tau/allow_mask the per-tile classes (every system reaches
+ * opts->tol in true relative residual through FP64 true-residual passes). */
+typedef struct mf_batch_s *mf_batch;
+typedef struct {
+    int nsys, nrhs, B, G;          /* systems, columns, tile size, workgroups */
+    long long rows, tilenum;       /* padded rows and tiles of the joined layout */
+    double bytes_alg;              /* per iteration: matrix once + 16 n per column */
+    double ms_setup;
+    long long nnz_per_code[5];
+} mf_batch_info_t;
+int mf_batch_create(int nsys, const int *n, const int *const *rowptr, const int *const *colidx,
+                    const double *const *val, int nrhs, const mf_opts *o, mf_batch *out);
+/* b[s][k*n_s + i] in, x[s][k*n_s + i] out (host); per-system iterations and
+ * max-over-columns true relative residual; rep->ms_solve = graph launch time
+ * (right-hand sides already in HBM). */
+int mf_batch_solve(mf_batch bt, const double *const *b, double *const *x, int *iters, double *relres,
+                   mf_report *rep);
+int mf_batch_info(mf_batch bt, mf_batch_info_t *info);
+void mf_batch_destroy(mf_batch bt);
+
 /* Batched: nsys independent square systems, each with nrhs right-hand sides
  * (column-major: b[s][k*n_s + i]), solved by one persistent launch captured
  * in a hipGraph.  Matrices are given as CSR.  x likewise.  Per-system

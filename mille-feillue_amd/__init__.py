@@ -88,6 +88,7 @@ EXPORTS = (
     "mf_solve_batched", "mf_mc_classify", "mf_compact_rows", "mf_partition_build", "mf_free",
     "mf_gen_laplace2d", "mf_gen_poisson3d", "mf_gen_convdiff3d", "mf_gen_random_spd",
     "mf_gen_stencil_lognormal", "mf_gen_uniform", "mf_gen_normal",
+    "mf_batch_create", "mf_batch_solve", "mf_batch_info", "mf_batch_destroy",
 )
 
 _L = None
@@ -140,6 +141,11 @@ def lib():
     L.mf_gen_normal.argtypes = [C.c_ulonglong, C.c_int, _d]
     L.mf_solve_batched.argtypes = [C.c_int, _i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_d), C.c_int,
                                    C.POINTER(_d), C.POINTER(_d), C.POINTER(Opts), _i, _d, C.POINTER(Report)]
+    L.mf_batch_create.argtypes = [C.c_int, _i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_d), C.c_int,
+                                  C.POINTER(Opts), C.POINTER(C.c_void_p)]
+    L.mf_batch_solve.argtypes = [C.c_void_p, C.POINTER(_d), C.POINTER(_d), _i, _d, C.POINTER(Report)]
+    L.mf_batch_info.argtypes = [C.c_void_p, C.POINTER(BatchInfo)]
+    L.mf_batch_destroy.argtypes = [C.c_void_p]
     _L = L
     return L
 
@@ -181,6 +187,67 @@ def ref_tiles_to_dict(t):
         "Tile_csr_Col": _copy(t.Tile_csr_Col, t.csrsize, np.uint8),
         "csr_compressedIdx": _copy(t.csr_compressedIdx, csz, np.uint8),
     }
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [("nsys", C.c_int), ("nrhs", C.c_int), ("B", C.c_int), ("G", C.c_int),
+                ("rows", C.c_longlong), ("tilenum", C.c_longlong), ("bytes_alg", C.c_double),
+                ("ms_setup", C.c_double), ("nnz_per_code", C.c_longlong * 5)]
+
+
+class Batch:
+    """Many independent SPD systems (CSR triples) with nrhs right-hand sides
+    each, solved by one persistent CG launch captured in a hipGraph."""
+
+    def __init__(self, systems, nrhs=1, **opts):
+        L = lib()
+        self.nsys = len(systems)
+        self.nrhs = nrhs
+        self._rp = [np.ascontiguousarray(s[0], np.int32) for s in systems]
+        self._ci = [np.ascontiguousarray(s[1], np.int32) for s in systems]
+        self._va = [np.ascontiguousarray(s[2], np.float64) for s in systems]
+        self.n = np.array([len(r) - 1 for r in self._rp], np.int32)
+        rp = (_i * self.nsys)(*[_p(a, C.c_int) for a in self._rp])
+        ci = (_i * self.nsys)(*[_p(a, C.c_int) for a in self._ci])
+        va = (_d * self.nsys)(*[_p(a, C.c_double) for a in self._va])
+        o = default_opts(**opts)
+        h = C.c_void_p()
+        _check(L.mf_batch_create(self.nsys, _p(self.n, C.c_int), rp, ci, va, nrhs, C.byref(o), C.byref(h)))
+        self.h = h
+
+    def info(self):
+        i = BatchInfo()
+        _check(lib().mf_batch_info(self.h, C.byref(i)))
+        d = {k: getattr(i, k) for k, _ in BatchInfo._fields_ if k != "nnz_per_code"}
+        d["nnz_per_code"] = list(i.nnz_per_code)
+        return d
+
+    def solve(self, b):
+        """b: list of arrays of shape (nrhs, n_s) (or (n_s,) for nrhs = 1).
+        Returns (x list of the same shapes, per-system iterations, per-system
+        true relative residual, report)."""
+        bs = [np.ascontiguousarray(np.reshape(bb, (self.nrhs, -1)), np.float64) for bb in b]
+        xs = [np.zeros_like(bb) for bb in bs]
+        bp = (_d * self.nsys)(*[_p(a, C.c_double) for a in bs])
+        xp = (_d * self.nsys)(*[_p(a, C.c_double) for a in xs])
+        it = np.zeros(self.nsys, np.int32)
+        rel = np.zeros(self.nsys, np.float64)
+        rep = Report()
+        _check(lib().mf_batch_solve(self.h, bp, xp, _p(it, C.c_int), _p(rel, C.c_double), C.byref(rep)))
+        if self.nrhs == 1:
+            xs = [x[0] for x in xs]
+        return xs, it, rel, rep.as_dict()
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mf_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def default_opts(**kw):

@@ -1,16 +1,903 @@
-// Batched persistent solver (many independent systems x several RHS).
+// Batched persistent CG: many independent SPD systems of uneven size, each
+// with K right-hand sides, solved by one launch captured in a hipGraph.
+//
+// Every system keeps its own compute layouts (built on the GPU exactly as for
+// a single matrix); the batch concatenates them into one block-diagonal
+// layout (tile rows, tile columns and slab offsets rebased), once in FP64 and
+// once with per-tile precision classes.  Vectors live in HBM, the K columns
+// of a row interleaved ([row][K]), so one slab read and one gather index feed
+// K FMAs: the matrix is streamed once per iteration for all right-hand sides.
+// System s runs on a group of W_s workgroups (W_s sized to its work); the
+// group's reductions are block-local when W_s = 1 and otherwise use the
+// tagged-word sync of the single-system kernels restricted to the group.
+// Each right-hand side has its own CG scalars; a column that has converged
+// is frozen (alpha = beta = 0) while the others continue, and the system
+// finishes when an FP64 true-residual pass over the FP64 layout shows every
+// column at the tolerance.  With lossy tiles the same passes (reliable
+// updates against the exact operator) carry the solve to 1e-10.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
 
 #include "mf_matrix.h"
 
+namespace mf {
+
+constexpr int BK_MAXV = 24;  // values per group allreduce: 3 K with K <= 8
+
+struct BatchArgs {
+    DevL L;    // inner layout (mixed classes, or FP64)
+    DevL L64;  // FP64 layout: true residuals
+    int G, nrhs;
+    const int *wg_sys, *wg_tr0, *wg_tr1;  // per workgroup: system, tile rows [tr0, tr1)
+    const int *sys_wg0, *sys_nwg;         // per system: first workgroup, group size
+    const double *b;                      // [rows][K]
+    double *x, *r, *p, *s;                // [rows][K]
+    float *r32;                           // FP32 copy of r for low-precision tiles
+    unsigned long long *slots;            // sync words [2][G][2 * BK_MAXV]
+    int *ctl;                             // [0] abort word
+    int *sys_out;                         // [4 s + 0] status, +1 iterations, +2 replacements
+    double *sys_relres;                   // max over columns of the true relative residual
+    double tol, rr_factor;
+    int maxit, haslow;
+    long long spin_limit;
+};
+
+__device__ __forceinline__ void st_ag(double *p, double v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_ag(float *p, float v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Group allreduce over the W workgroups of one system (see the file comment
+// of solver.hip for the protocol of the tagged words; a group of one only
+// reduces across its waves).
+template <int TPB>
+struct GroupSync {
+    double *s_red;  // [TPB/64][BK_MAXV]
+    double *s_bc;   // [BK_MAXV]
+    int *s_flag;
+    int gi, gn, gw0;  // index in group, group size, first workgroup of the group
+    unsigned ep = 0;
+
+    template <int N>
+    __device__ bool allreduce(const BatchArgs &A, double (&v)[N], bool publish)
+    {
+        static_assert(N <= BK_MAXV, "group reducer width");
+        const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+        for (int k = 0; k < N; k++) {
+            const double s = wave_sum(v[k]);
+            if (lane == 0) s_red[wv * BK_MAXV + k] = s;
+        }
+        if (publish && gn > 1) __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        ++ep;
+        if (tid < 64) {
+            double part = 0.0;
+            if (lane < N)
+                for (int w = 0; w < TPB / 64; w++) part += s_red[w * BK_MAXV + lane];
+            int ab = 0;
+            if (gn == 1) {
+                if (lane < N) s_bc[lane] = part;
+            } else {
+                unsigned long long *W = A.slots + (size_t)(ep & 1u) * A.G * 2 * BK_MAXV;
+                const unsigned long long tag = (unsigned long long)ep << 32;
+                if (lane < N) {
+                    const unsigned long long bits = (unsigned long long)__double_as_longlong(part);
+                    unsigned long long *mine = W + (size_t)(gw0 + gi) * 2 * BK_MAXV + 2 * lane;
+                    __hip_atomic_store(mine, tag | (bits & 0xffffffffull), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(mine + 1, tag | (bits >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                long long spins = 0;
+                double acc = 0.0;
+                for (;;) {
+                    bool ok = true;
+                    acc = 0.0;
+                    if (lane < N)
+                        for (int q = 0; q < gn; q++) {
+                            const unsigned long long *w = W + (size_t)(gw0 + q) * 2 * BK_MAXV + 2 * lane;
+                            const unsigned long long lo = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            const unsigned long long hi = __hip_atomic_load(w + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            ok = ok && (lo >> 32) == ep && (hi >> 32) == ep;
+                            acc += __longlong_as_double((long long)((hi << 32) | (lo & 0xffffffffull)));
+                        }
+                    if (__all(ok)) break;
+                    if (__builtin_amdgcn_readfirstlane(ld_agent(A.ctl))) { ab = 1; break; }
+                    if (++spins > A.spin_limit) {
+                        if (lane == 0) __hip_atomic_store(A.ctl, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        ab = 1;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                if (lane < N) s_bc[lane] = acc;
+                if (!ab && publish) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            }
+            if (lane == 0) *s_flag = ab;
+        }
+        __syncthreads();
+        if (*s_flag) return false;
+#pragma unroll
+        for (int k = 0; k < N; k++) v[k] = s_bc[k];
+        return true;
+    }
+};
+
+// K consecutive entries of a [row][K] vector
+template <int K>
+__device__ __forceinline__ void ldK(const double *__restrict__ g, uint32_t col, double (&o)[K])
+{
+    if constexpr (K == 1) {
+        o[0] = g[col];
+    } else {
+        const double2 *q = reinterpret_cast<const double2 *>(g + (size_t)col * K);
+#pragma unroll
+        for (int j = 0; j < K / 2; j++) {
+            const double2 d = q[j];
+            o[2 * j] = d.x;
+            o[2 * j + 1] = d.y;
+        }
+    }
+}
+template <int K>
+__device__ __forceinline__ void ldK(const float *__restrict__ g, uint32_t col, float (&o)[K])
+{
+    if constexpr (K % 4 == 0) {
+        const float4 *q = reinterpret_cast<const float4 *>(g + (size_t)col * K);
+#pragma unroll
+        for (int j = 0; j < K / 4; j++) {
+            const float4 d = q[j];
+            o[4 * j] = d.x;
+            o[4 * j + 1] = d.y;
+            o[4 * j + 2] = d.z;
+            o[4 * j + 3] = d.w;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < K; j++) o[j] = g[(size_t)col * K + j];
+    }
+}
+
+// One class slab of this lane's row applied to K columns.
+template <int B, bool IDX8, int CLS, int K, typename AccT, typename GT>
+__device__ __forceinline__ void slabK(const DevL &L, int2 h, int tcr, int t0, int r, int lanebase,
+                                      const GT *__restrict__ g, AccT (&acc)[K])
+{
+    const int nq = (h.y + 3) >> 2;
+    const uint8_t *vals = L.blob + (size_t)(unsigned)h.x * 8;
+    const uint8_t *idx = vals + (size_t)nq * B * 4 * cls_width(CLS);
+    for (int j = 0; j < nq; j++) {
+        Quad<CLS> v;
+        IdxQuad<IDX8> id;
+        v.load(vals, (size_t)j * B + r);
+        id.load(idx, (size_t)j * B + r);
+        GT xg[4][K];
+        static_for<4>([&](auto e) { ldK<K>(g, gather_col<B, IDX8>(id.template get<e>(), tcr, t0, L, lanebase), xg[e]); });
+        static_for<4>([&](auto e) {
+            const AccT a = (AccT)v.template get<e>();
+#pragma unroll
+            for (int k = 0; k < K; k++) acc[k] = fma_t(a, (AccT)xg[e][k], acc[k]);
+        });
+    }
+}
+
+template <bool IDX8, int B>
+__device__ __forceinline__ int row_tcr(const DevL &L, int t0, int nt, int r)
+{
+    return (IDX8 && r < nt) ? L.stc[t0 + r] * B : 0;
+}
+
+// y = A_row g for K columns: FP64 tiles in FP64 over g64, the other classes
+// in FP32 over g32, as the single-vector SpMV.
+template <int B, bool IDX8, int K>
+__device__ __forceinline__ void row_spmvK(const DevL &L, int tr, int r, int lanebase, const double *__restrict__ g64,
+                                          const float *__restrict__ g32, double (&y)[K])
+{
+    const int4 *h = reinterpret_cast<const int4 *>(L.trh + (size_t)tr * 4);
+    const int4 h01 = h[0], h23 = h[1];
+    const int t0 = L.tile_ptr[tr], nt = L.tile_ptr[tr + 1] - t0;
+    const int tcr = row_tcr<IDX8, B>(L, t0, nt, r);
+    double a64[K];
+    float a32[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        a64[k] = 0.0;
+        a32[k] = 0.0f;
+    }
+    if (h01.y) slabK<B, IDX8, 0, K>(L, make_int2(h01.x, h01.y), tcr, t0, r, lanebase, g64, a64);
+    if (h01.w) slabK<B, IDX8, 1, K>(L, make_int2(h01.z, h01.w), tcr, t0, r, lanebase, g32, a32);
+    if (h23.y) slabK<B, IDX8, 2, K>(L, make_int2(h23.x, h23.y), tcr, t0, r, lanebase, g32, a32);
+    if (h23.w) slabK<B, IDX8, 3, K>(L, make_int2(h23.z, h23.w), tcr, t0, r, lanebase, g32, a32);
+#pragma unroll
+    for (int k = 0; k < K; k++) y[k] = a64[k] + (double)a32[k];
+}
+
+// A x and A p on the FP64 layout (class 0 only) for K columns each.
+template <int B, bool IDX8, int K>
+__device__ __forceinline__ void row_spmv64_xp(const DevL &L, int tr, int r, int lanebase, const double *__restrict__ gx,
+                                              const double *__restrict__ gp, double (&ax)[K], double (&ap)[K])
+{
+    const int2 h = reinterpret_cast<const int2 *>(L.trh)[(size_t)tr * 4];
+    const int t0 = L.tile_ptr[tr], nt = L.tile_ptr[tr + 1] - t0;
+    const int tcr = row_tcr<IDX8, B>(L, t0, nt, r);
+#pragma unroll
+    for (int k = 0; k < K; k++) ax[k] = ap[k] = 0.0;
+    const int nq = (h.y + 3) >> 2;
+    const uint8_t *vals = L.blob + (size_t)(unsigned)h.x * 8;
+    const uint8_t *idx = vals + (size_t)nq * B * 4 * 8;
+    for (int j = 0; j < nq; j++) {
+        Quad<0> v;
+        IdxQuad<IDX8> id;
+        v.load(vals, (size_t)j * B + r);
+        id.load(idx, (size_t)j * B + r);
+        static_for<4>([&](auto e) {
+            const uint32_t c = gather_col<B, IDX8>(id.template get<e>(), tcr, t0, L, lanebase);
+            double vx[K], vp[K];
+            ldK<K>(gx, c, vx);
+            ldK<K>(gp, c, vp);
+            const double a = v.template get<e>();
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                ax[k] = __builtin_fma(a, vx[k], ax[k]);
+                ap[k] = __builtin_fma(a, vp[k], ap[k]);
+            }
+        });
+    }
+}
+
+template <int B, bool IDX8, int K, int TPB>
+__global__ __launch_bounds__(TPB) void k_cg_batched(BatchArgs A)
+{
+    constexpr int GPB = TPB / B;
+    __shared__ double s_red[(TPB / 64) * BK_MAXV];
+    __shared__ double s_bc[BK_MAXV];
+    __shared__ int s_flag;
+    // per-column CG scalars: gamma, previous gamma, previous alpha, reference
+    // ||r||^2 of the last replacement, ||b||^2, (r, s) and (p, s) of the last
+    // replacement, alpha, beta
+    __shared__ double sh_g[K], sh_gp[K], sh_ap[K], sh_ref[K], sh_b2[K], sh_rs[K], sh_ps[K], sh_al[K], sh_be[K];
+    __shared__ int sh_mode[K];  // bit 0 restart, bit 1 replaced, bit 2 fresh, bit 3 after replacement, bit 4 frozen
+    __shared__ int sh_bd;       // breakdown flags: bit 0 in a fresh step, bit 1 otherwise
+    const int g = blockIdx.x, tid = threadIdx.x;
+    const int sys = A.wg_sys[g], gw0 = A.sys_wg0[sys], gn = A.sys_nwg[sys];
+    const int tr0 = A.wg_tr0[g], tr1 = A.wg_tr1[g];
+    const int grp = tid / B, lane = tid % B, lanebase = (tid & 63) & ~(B - 1);
+    GroupSync<TPB> S{s_red, s_bc, &s_flag, g - gw0, gn, gw0};
+    const int KA = A.nrhs;
+    const double tol2 = A.tol * A.tol, rr2 = A.rr_factor * A.rr_factor;
+    auto finish = [&](int status, int iter, int repl, double relres) {
+        if (g == gw0 && tid == 0) {
+            A.sys_out[4 * sys] = status;
+            A.sys_out[4 * sys + 1] = iter;
+            A.sys_out[4 * sys + 2] = repl;
+            A.sys_relres[sys] = relres;
+        }
+    };
+
+    double v[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) v[k] = 0.0;
+    for (int tr = tr0 + grp; tr < tr1; tr += GPB) {
+        const size_t row = (size_t)tr * B + lane;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const size_t e = row * K + k;
+            const double bv = A.b[e];
+            st_ag(A.x + e, 0.0);
+            st_ag(A.p + e, 0.0);
+            A.s[e] = 0.0;
+            st_ag(A.r + e, bv);
+            if (A.haslow) st_ag(A.r32 + e, (float)bv);
+            v[k] += bv * bv;
+        }
+    }
+    if (!S.allreduce(A, v, true)) return;
+    if (tid < K) {
+        sh_b2[tid] = sh_g[tid] = sh_ref[tid] = v[tid];
+        sh_gp[tid] = sh_ap[tid] = 1.0;
+        sh_mode[tid] = 1;
+    }
+    __syncthreads();
+    int iter = 0, repl = 0;
+    bool force = false;
+    for (;;) {
+        bool conv = true, due = A.rr_factor > 0.0, finite = true;
+        for (int k = 0; k < KA; k++) {
+            const double gk = sh_g[k];
+            finite = finite && isfinite(gk);
+            conv = conv && gk <= tol2 * sh_b2[k];
+            due = due && gk <= rr2 * sh_ref[k];
+        }
+        if (!finite) { finish(MF_E_NONFINITE, iter, repl, -1.0); return; }
+        if (conv || due || force) {
+            // r = b - A x and s = A p on the FP64 layout; (r,r), (r,s), (p,s)
+            if (!S.allreduce(A, v, true)) return;  // x and p of every row visible
+            double t[3 * K];
+#pragma unroll
+            for (int k = 0; k < 3 * K; k++) t[k] = 0.0;
+            for (int tr = tr0 + grp; tr < tr1; tr += GPB) {
+                const size_t row = (size_t)tr * B + lane;
+                double ax[K], ap[K];
+                row_spmv64_xp<B, IDX8, K>(A.L64, tr, lane, lanebase, A.x, A.p, ax, ap);
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    const size_t e = row * K + k;
+                    const double rv = A.b[e] - ax[k], pv = A.p[e];
+                    st_ag(A.r + e, rv);
+                    if (A.haslow) st_ag(A.r32 + e, (float)rv);
+                    A.s[e] = ap[k];
+                    t[k] += rv * rv;
+                    t[K + k] += rv * ap[k];
+                    t[2 * K + k] += pv * ap[k];
+                }
+            }
+            if (!S.allreduce(A, t, true)) return;
+            ++repl;
+            bool done = true;
+            double rel = 0.0;
+            for (int k = 0; k < KA; k++) {
+                done = done && t[k] <= tol2 * sh_b2[k];
+                if (sh_b2[k] > 0.0) rel = fmax(rel, sqrt(t[k] / sh_b2[k]));
+            }
+            if (!isfinite(rel)) { finish(MF_E_NONFINITE, iter, repl, rel); return; }
+            if (done) { finish(MF_OK, iter, repl, rel); return; }
+            if (tid < K) {
+                const bool restart = force || t[tid] > 4.0 * sh_g[tid];
+                sh_mode[tid] = (restart ? 1 : 0) | 2;
+                sh_rs[tid] = t[K + tid];
+                sh_ps[tid] = t[2 * K + tid];
+                sh_g[tid] = sh_ref[tid] = t[tid];
+            }
+            force = false;
+            __syncthreads();
+            continue;
+        }
+        if (iter >= A.maxit) {
+            double rel = 0.0;
+            for (int k = 0; k < KA; k++)
+                if (sh_b2[k] > 0.0) rel = fmax(rel, sqrt(sh_g[k] / sh_b2[k]));
+            finish(MF_E_MAXIT, iter, repl, rel);
+            return;
+        }
+        if (tid < K) {
+            const int m = sh_mode[tid];
+            const bool fresh = iter == 0 || (m & 1);
+            const bool frozen = tid >= KA || sh_g[tid] <= tol2 * sh_b2[tid];
+            sh_be[tid] = (fresh || frozen) ? 0.0 : sh_g[tid] / sh_gp[tid];
+            sh_mode[tid] = (fresh ? 4 : 0) | ((!fresh && (m & 2)) ? 8 : 0) | (frozen ? 16 : 0);
+        }
+        if (tid == 0) sh_bd = 0;
+        __syncthreads();
+        // w = A r; s = w + beta s; (w, r)
+        double d[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) d[k] = 0.0;
+        for (int tr = tr0 + grp; tr < tr1; tr += GPB) {
+            const size_t row = (size_t)tr * B + lane;
+            double w[K];
+            row_spmvK<B, IDX8, K>(A.L, tr, lane, lanebase, A.r, A.r32, w);
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const size_t e = row * K + k;
+                A.s[e] = w[k] + sh_be[k] * A.s[e];
+                d[k] += w[k] * A.r[e];
+            }
+        }
+        if (!S.allreduce(A, d, false)) return;
+        if (tid < K) {
+            const int m = sh_mode[tid];
+            const double be = sh_be[tid], gk = sh_g[tid];
+            const double denom = (m & 4)   ? d[tid]
+                                 : (m & 8) ? d[tid] + be * (2.0 * sh_rs[tid] + be * sh_ps[tid])
+                                           : d[tid] - be * gk / sh_ap[tid];
+            if (m & 16) {
+                sh_al[tid] = 0.0;
+            } else if (!(denom > 0.0) || !isfinite(denom)) {
+                sh_al[tid] = 0.0;
+                atomicOr(&sh_bd, (m & 4) ? 1 : 2);
+            } else {
+                sh_al[tid] = gk / denom;
+            }
+        }
+        __syncthreads();
+        if (sh_bd) {
+            // rounding broke (p, A p) > 0: recompute the residual and restart;
+            // a breakdown in a fresh step is final
+            if (sh_bd & 1) { finish(MF_E_BREAKDOWN, iter, repl, -1.0); return; }
+            force = true;
+            continue;
+        }
+        double gg[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) gg[k] = 0.0;
+        for (int tr = tr0 + grp; tr < tr1; tr += GPB) {
+            const size_t row = (size_t)tr * B + lane;
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const size_t e = row * K + k;
+                const double rv0 = A.r[e];
+                const double pv = rv0 + sh_be[k] * A.p[e];
+                const double xv = A.x[e] + sh_al[k] * pv;
+                const double rv = rv0 - sh_al[k] * A.s[e];
+                st_ag(A.p + e, pv);
+                st_ag(A.x + e, xv);
+                st_ag(A.r + e, rv);
+                if (A.haslow) st_ag(A.r32 + e, (float)rv);
+                gg[k] += rv * rv;
+            }
+        }
+        if (!S.allreduce(A, gg, true)) return;
+        if (tid < K) {
+            if (!(sh_mode[tid] & 16)) {
+                sh_gp[tid] = sh_g[tid];
+                sh_ap[tid] = sh_al[tid];
+            }
+            sh_g[tid] = gg[tid];
+            sh_mode[tid] = 0;
+        }
+        __syncthreads();
+        ++iter;
+    }
+}
+
+__global__ void k_batch_reset(unsigned long long *slots, long long nslots, int *ctl, int *sys_out, int nsys)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nslots) slots[i] = 0ull;
+    if (i < 8) ctl[i] = 0;
+    if (i < 4ll * nsys) sys_out[i] = (i % 4 == 0) ? MF_E_TIMEOUT : 0;
+}
+
+__global__ void k_add_int(const int *__restrict__ src, int n, int add, int *__restrict__ dst)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i] + add;
+}
+
+__global__ void k_rebase_trh(const int2 *__restrict__ src, int n, unsigned add8, int2 *__restrict__ dst)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const int2 h = src[i];
+        dst[i] = make_int2((int)((unsigned)h.x + add8), h.y);
+    }
+}
+
+}  // namespace mf
+
 using namespace mf;
 
-extern "C" int mf_solve_batched(int nsys, const int *n, const int *const *rowptr,
-                                const int *const *colidx, const double *const *val, int nrhs,
-                                const double *const *b, double *const *x, const mf_opts *o,
-                                int *iters, double *relres, mf_report *rep)
+// One concatenated (block-diagonal) compute layout of the batch.
+struct BatchLayout {
+    int2 *trh = nullptr;
+    int *stc = nullptr;
+    uint8_t *blob = nullptr;
+    size_t blob_bytes = 0;
+    bool has64 = false, haslow = false;
+    long long nnz_code[5] = {0, 0, 0, 0, 0};
+    void release()
+    {
+        if (trh) (void)hipFree(trh);
+        if (stc) (void)hipFree(stc);
+        if (blob) (void)hipFree(blob);
+        *this = BatchLayout();
+    }
+};
+
+struct mf_batch_s {
+    int nsys = 0, nrhs = 0, K = 0, B = 16, TPB = 256;
+    bool mixed = false, idx8 = true;
+    mf_opts o;
+    std::vector<int> n, tilem, trbase, nnz;
+    std::vector<long long> tbase;
+    std::vector<double> tau;
+    int tilem_total = 0;
+    long long tilenum_total = 0;
+    size_t rows = 0;
+    int *d_tile_ptr = nullptr;
+    BatchLayout L64, Lmix;
+    int G = 0;
+    std::vector<int> sys_nwg;
+    int *d_wg = nullptr;  // wg_sys, wg_tr0, wg_tr1 [3][G], then sys_wg0, sys_nwg [2][nsys]
+    double *d_vec = nullptr;  // b, x, r, p, s: 5 x rows x K
+    float *d_r32 = nullptr;
+    unsigned long long *d_slots = nullptr;
+    int *d_ctl = nullptr, *d_sys_out = nullptr;
+    double *d_relres = nullptr;
+    double *h_stage = nullptr;  // pinned [rows][K]
+    const void *fn = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    BatchArgs args{};
+    double ms_setup = 0.0;
+    double bytes_alg = 0.0;  // per iteration: matrix once + K (16 n) vector bytes
+};
+
+static void batch_release(mf_batch_s *bt)
 {
-    (void)nsys; (void)n; (void)rowptr; (void)colidx; (void)val; (void)nrhs; (void)b; (void)x;
-    (void)o; (void)iters; (void)relres; (void)rep;
-    return fail(MF_E_UNSUPPORTED, "batched solver not built yet");
+    if (bt->gexec) (void)hipGraphExecDestroy(bt->gexec);
+    if (bt->graph) (void)hipGraphDestroy(bt->graph);
+    bt->L64.release();
+    bt->Lmix.release();
+    void *ps[] = {bt->d_tile_ptr, bt->d_wg, bt->d_vec, bt->d_r32, bt->d_slots, bt->d_ctl, bt->d_sys_out, bt->d_relres};
+    for (void *p : ps)
+        if (p) (void)hipFree(p);
+    if (bt->h_stage) (void)hipHostFree(bt->h_stage);
 }
+
+typedef const void *KernelFn;
+
+template <int B, bool IDX8>
+static KernelFn batched_fn(int K)
+{
+    if (K == 1) return (KernelFn)k_cg_batched<B, IDX8, 1, 256>;
+    return (KernelFn)k_cg_batched<B, IDX8, 8, 256>;
+}
+
+static KernelFn pick_batched(int B, bool idx8, int K)
+{
+    switch (B) {
+    case 8: return idx8 ? batched_fn<8, true>(K) : batched_fn<8, false>(K);
+    case 16: return idx8 ? batched_fn<16, true>(K) : batched_fn<16, false>(K);
+    default: return idx8 ? batched_fn<32, true>(K) : batched_fn<32, false>(K);
+    }
+}
+
+// Concatenates one layout per system into a block-diagonal layout.
+static int concat_layouts(mf_batch_s *bt, const std::vector<mf_matrix> &mats, bool mix, BatchLayout &out)
+{
+    hipStream_t st = stream();
+    std::vector<size_t> base(bt->nsys + 1, 0);
+    for (int s = 0; s < bt->nsys; s++) {
+        const Layout &L = mix ? mats[s]->Lmix : mats[s]->L64;
+        base[s + 1] = base[s] + (L.blob_bytes + 255) / 256 * 256;
+        out.has64 = out.has64 || L.has64;
+        out.haslow = out.haslow || L.haslow;
+        for (int c = 0; c < 5; c++) out.nnz_code[c] += L.nnz_per_code[c];
+    }
+    out.blob_bytes = base[bt->nsys];
+    if (out.blob_bytes / 8 > 0xFFFFFFFFull) return fail(MF_E_UNSUPPORTED, "batched layout larger than 32 GB");
+    MF_HIP(hipMalloc((void **)&out.trh, sizeof(int2) * 4 * ((size_t)bt->tilem_total + 1)));
+    MF_HIP(hipMalloc((void **)&out.stc, sizeof(int) * ((size_t)bt->tilenum_total + 1)));
+    MF_HIP(hipMalloc((void **)&out.blob, std::max<size_t>(out.blob_bytes, 64)));
+    MF_HIP(hipMemsetAsync(out.trh, 0, sizeof(int2) * 4 * ((size_t)bt->tilem_total + 1), st));
+    MF_HIP(hipMemsetAsync(out.stc, 0, sizeof(int) * ((size_t)bt->tilenum_total + 1), st));
+    for (int s = 0; s < bt->nsys; s++) {
+        const Layout &L = mix ? mats[s]->Lmix : mats[s]->L64;
+        const mf_matrix m = mats[s];
+        const int n4 = 4 * m->tilem;
+        k_rebase_trh<<<(n4 + 255) / 256, 256, 0, st>>>(L.trh, n4, (unsigned)(base[s] / 8), out.trh + (size_t)4 * bt->trbase[s]);
+        if (m->tilenum)
+            k_add_int<<<(m->tilenum + 255) / 256, 256, 0, st>>>(L.stc, m->tilenum, bt->trbase[s],
+                                                               out.stc + bt->tbase[s]);
+        if (L.blob_bytes)
+            MF_HIP(hipMemcpyAsync(out.blob + base[s], L.blob, L.blob_bytes, hipMemcpyDeviceToDevice, st));
+    }
+    MF_HIP(hipGetLastError());
+    return MF_OK;
+}
+
+static int batch_assign(mf_batch_s *bt, int g_target)
+{
+    // work of a system ~ its slab bytes plus K vector passes over its rows
+    std::vector<double> work(bt->nsys);
+    double total = 0.0;
+    for (int s = 0; s < bt->nsys; s++) {
+        work[s] = 12.0 * bt->nnz[s] + 48.0 * bt->K * bt->tilem[s] * bt->B;
+        total += work[s];
+    }
+    const int gpb = bt->TPB / bt->B;
+    bt->sys_nwg.assign(bt->nsys, 1);
+    int G = 0;
+    for (int s = 0; s < bt->nsys; s++) {
+        int w = (int)std::lround(g_target * work[s] / total);
+        w = std::max(1, std::min(w, std::max(1, bt->tilem[s] / gpb)));
+        bt->sys_nwg[s] = w;
+        G += w;
+    }
+    std::vector<int> h(3 * (size_t)G + 2 * (size_t)bt->nsys);
+    int g = 0;
+    for (int s = 0; s < bt->nsys; s++) {
+        const int w = bt->sys_nwg[s];
+        h[3 * (size_t)G + s] = g;
+        h[3 * (size_t)G + bt->nsys + s] = w;
+        for (int q = 0; q < w; q++, g++) {
+            h[g] = s;
+            h[(size_t)G + g] = bt->trbase[s] + (int)((long long)bt->tilem[s] * q / w);
+            h[2 * (size_t)G + g] = bt->trbase[s] + (int)((long long)bt->tilem[s] * (q + 1) / w);
+        }
+    }
+    if (bt->d_wg) MF_HIP(hipFree(bt->d_wg));
+    if (bt->d_slots) MF_HIP(hipFree(bt->d_slots));
+    bt->d_wg = nullptr;
+    bt->d_slots = nullptr;
+    MF_HIP(hipMalloc((void **)&bt->d_wg, sizeof(int) * h.size()));
+    MF_HIP(hipMemcpy(bt->d_wg, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice));
+    MF_HIP(hipMalloc((void **)&bt->d_slots, sizeof(unsigned long long) * 2 * 2 * BK_MAXV * (size_t)G));
+    bt->G = G;
+    return MF_OK;
+}
+
+// Builds the launch arguments and captures reset + persistent launch in a
+// graph (the reset is a kernel: no memset nodes).
+static int batch_capture(mf_batch_s *bt)
+{
+    hipStream_t st = stream();
+    BatchArgs &A = bt->args;
+    const BatchLayout &Li = bt->mixed ? bt->Lmix : bt->L64;
+    A.L = DevL{bt->d_tile_ptr, Li.trh, Li.stc, Li.blob};
+    A.L64 = DevL{bt->d_tile_ptr, bt->L64.trh, bt->L64.stc, bt->L64.blob};
+    A.G = bt->G;
+    A.nrhs = bt->nrhs;
+    A.wg_sys = bt->d_wg;
+    A.wg_tr0 = bt->d_wg + bt->G;
+    A.wg_tr1 = bt->d_wg + 2 * bt->G;
+    A.sys_wg0 = bt->d_wg + 3 * bt->G;
+    A.sys_nwg = bt->d_wg + 3 * bt->G + bt->nsys;
+    const size_t vn = bt->rows * bt->K;
+    A.b = bt->d_vec;
+    A.x = bt->d_vec + vn;
+    A.r = bt->d_vec + 2 * vn;
+    A.p = bt->d_vec + 3 * vn;
+    A.s = bt->d_vec + 4 * vn;
+    A.r32 = bt->d_r32;
+    A.slots = bt->d_slots;
+    A.ctl = bt->d_ctl;
+    A.sys_out = bt->d_sys_out;
+    A.sys_relres = bt->d_relres;
+    A.tol = bt->o.tol;
+    A.rr_factor = bt->o.rr_factor;
+    A.maxit = bt->o.maxit;
+    A.haslow = Li.haslow ? 1 : 0;
+    A.spin_limit = bt->o.spin_limit;
+    bt->fn = pick_batched(bt->B, bt->idx8, bt->K);
+    int per_cu = 0;
+    MF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bt->fn, bt->TPB, 0));
+    if ((long long)per_cu * device_cu_count() < bt->G)
+        return fail(MF_E_UNSUPPORTED, "batched grid does not fit the device");
+    if (bt->gexec) (void)hipGraphExecDestroy(bt->gexec);
+    if (bt->graph) (void)hipGraphDestroy(bt->graph);
+    bt->gexec = nullptr;
+    bt->graph = nullptr;
+    const long long nslots = 2ll * 2 * BK_MAXV * bt->G;
+    const long long nr = std::max<long long>(nslots, 4ll * bt->nsys);
+    MF_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    k_batch_reset<<<(unsigned)((nr + 255) / 256), 256, 0, st>>>(bt->d_slots, nslots, bt->d_ctl, bt->d_sys_out, bt->nsys);
+    void *argv[] = {&bt->args};
+    hipError_t le = hipLaunchKernel(bt->fn, dim3(bt->G), dim3(bt->TPB), argv, 0, st);
+    hipGraph_t graph = nullptr;
+    hipError_t ce = hipStreamEndCapture(st, &graph);
+    if (le != hipSuccess) return hip_fail(le, "batched launch (capture)");
+    if (ce != hipSuccess) return hip_fail(ce, "hipStreamEndCapture");
+    bt->graph = graph;
+    MF_HIP(hipGraphInstantiate(&bt->gexec, graph, nullptr, nullptr, 0));
+    return MF_OK;
+}
+
+static int batch_build(mf_batch_s *bt, const int *const *rowptr, const int *const *colidx, const double *const *val)
+{
+    hipStream_t st = stream();
+    std::vector<mf_matrix> mats(bt->nsys, nullptr);
+    auto cleanup = [&]() {
+        for (mf_matrix m : mats)
+            if (m) mf_matrix_destroy(m);
+    };
+    int rc = MF_OK;
+    bool force16 = false;
+    for (int pass = 0; pass < 2; pass++) {
+        bool all8 = true;
+        for (int s = 0; s < bt->nsys && !rc; s++) {
+            if (!mats[s]) rc = mf_matrix_create(bt->n[s], bt->n[s], rowptr[s][bt->n[s]], rowptr[s], colidx[s], val[s],
+                                                 nullptr, bt->B, &mats[s]);
+            if (!rc) rc = build_layout(mats[s], mats[s]->L64, 0.0, 0, force16);
+            if (!rc && bt->mixed) rc = build_layout(mats[s], mats[s]->Lmix, bt->tau[s], bt->o.allow_mask, force16);
+            if (!rc) all8 = all8 && mats[s]->L64.idx8 && (!bt->mixed || mats[s]->Lmix.idx8);
+        }
+        if (rc) { cleanup(); return rc; }
+        if (all8 || force16) break;
+        force16 = true;
+    }
+    bt->idx8 = !force16;
+    bt->tilem.resize(bt->nsys);
+    bt->trbase.assign(bt->nsys + 1, 0);
+    bt->tbase.assign(bt->nsys + 1, 0);
+    bt->nnz.resize(bt->nsys);
+    for (int s = 0; s < bt->nsys; s++) {
+        bt->tilem[s] = mats[s]->tilem;
+        bt->nnz[s] = mats[s]->nnz;
+        bt->trbase[s + 1] = bt->trbase[s] + mats[s]->tilem;
+        bt->tbase[s + 1] = bt->tbase[s] + mats[s]->tilenum;
+    }
+    bt->tilem_total = bt->trbase[bt->nsys];
+    bt->tilenum_total = bt->tbase[bt->nsys];
+    if ((long long)bt->tilem_total * bt->B * bt->K * 8 >= (1ll << 32) / 2) {
+        cleanup();
+        return fail(MF_E_UNSUPPORTED, "batched vectors of 2 GB or more");
+    }
+    bt->rows = (size_t)bt->tilem_total * bt->B;
+    rc = hipMalloc((void **)&bt->d_tile_ptr, sizeof(int) * ((size_t)bt->tilem_total + 1)) == hipSuccess ? MF_OK
+                                                                                                  : fail(MF_E_NOMEM, "tile_ptr");
+    for (int s = 0; s < bt->nsys && !rc; s++)
+        k_add_int<<<(mats[s]->tilem + 1 + 255) / 256, 256, 0, st>>>(mats[s]->d_tile_ptr, mats[s]->tilem + 1,
+                                                                     (int)bt->tbase[s], bt->d_tile_ptr + bt->trbase[s]);
+    if (!rc) rc = concat_layouts(bt, mats, false, bt->L64);
+    if (!rc && bt->mixed) rc = concat_layouts(bt, mats, true, bt->Lmix);
+    if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = fail(MF_E_HIP, "batched layout copy");
+    cleanup();
+    if (rc) return rc;
+    // algorithmic bytes per iteration: the matrix once (tile formula, each
+    // system as in the single-matrix case) plus 16 n bytes per column
+    const BatchLayout &Li = bt->mixed ? bt->Lmix : bt->L64;
+    const double wv[5] = {8, 0, 1, 2, 4};
+    double vt = 0.0;
+    for (int c = 0; c < 5; c++) vt += Li.nnz_code[c] * (wv[c] + 1);
+    bt->bytes_alg = vt + (double)bt->tilenum_total * (8 + bt->B) + 4.0 * (bt->tilem_total + bt->nsys) +
+                    16.0 * bt->K * (double)bt->rows;
+    const size_t vn = bt->rows * bt->K;
+    MF_HIP(hipMalloc((void **)&bt->d_vec, sizeof(double) * 5 * vn));
+    MF_HIP(hipMalloc((void **)&bt->d_r32, sizeof(float) * vn));
+    MF_HIP(hipMemset(bt->d_vec, 0, sizeof(double) * 5 * vn));
+    MF_HIP(hipMemset(bt->d_r32, 0, sizeof(float) * vn));
+    MF_HIP(hipMalloc((void **)&bt->d_ctl, sizeof(int) * 8));
+    MF_HIP(hipMalloc((void **)&bt->d_sys_out, sizeof(int) * 4 * bt->nsys));
+    MF_HIP(hipMalloc((void **)&bt->d_relres, sizeof(double) * bt->nsys));
+    MF_HIP(hipHostMalloc((void **)&bt->h_stage, sizeof(double) * vn, hipHostMallocDefault));
+    return MF_OK;
+}
+
+static int batch_run(mf_batch_s *bt, float *ms)
+{
+    hipStream_t st = stream();
+    hipEvent_t e0, e1;
+    MF_HIP(hipEventCreate(&e0));
+    MF_HIP(hipEventCreate(&e1));
+    MF_HIP(hipEventRecord(e0, st));
+    MF_HIP(hipGraphLaunch(bt->gexec, st));
+    MF_HIP(hipEventRecord(e1, st));
+    MF_HIP(hipEventSynchronize(e1));
+    MF_HIP(hipEventElapsedTime(ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return MF_OK;
+}
+
+extern "C" {
+
+int mf_batch_create(int nsys, const int *n, const int *const *rowptr, const int *const *colidx,
+                    const double *const *val, int nrhs, const mf_opts *opts, mf_batch *out)
+{
+    clear_error();
+    *out = nullptr;
+    if (nsys <= 0 || !n || !rowptr || !colidx || !val) return fail(MF_E_ARG, "empty batch");
+    if (nrhs < 1 || nrhs > 8) return fail(MF_E_ARG, "nrhs must be 1..8");
+    int rc = ensure_device();
+    if (rc) return rc;
+    mf_batch_s *bt = new mf_batch_s();
+    if (opts) bt->o = *opts;
+    else mf_opts_default(&bt->o);
+    if (bt->o.method != MF_CG) {
+        delete bt;
+        return fail(MF_E_UNSUPPORTED, "the batched solver runs CG");
+    }
+    const double t0 = now_ms();
+    bt->nsys = nsys;
+    bt->nrhs = nrhs;
+    bt->K = nrhs == 1 ? 1 : 8;
+    bt->mixed = bt->o.precision == MF_MIXED;
+    bt->n.assign(n, n + nsys);
+    bt->tau.assign(nsys, bt->o.tau);
+    bt->B = bt->o.block == 8 || bt->o.block == 32 ? bt->o.block : 16;
+    rc = batch_build(bt, rowptr, colidx, val);
+    if (!rc) rc = batch_assign(bt, bt->o.grid > 0 ? bt->o.grid : 2 * device_cu_count());
+    if (!rc) rc = batch_capture(bt);
+    if (rc) {
+        batch_release(bt);
+        delete bt;
+        return rc;
+    }
+    bt->ms_setup = now_ms() - t0;
+    *out = bt;
+    return MF_OK;
+}
+
+void mf_batch_destroy(mf_batch bt)
+{
+    if (!bt) return;
+    batch_release(bt);
+    delete bt;
+}
+
+int mf_batch_solve(mf_batch bt, const double *const *b, double *const *x, int *iters, double *relres, mf_report *rep)
+{
+    clear_error();
+    if (!bt || !b || !x) return fail(MF_E_ARG, "null argument");
+    hipStream_t st = stream();
+    const int K = bt->K;
+    const size_t vn = bt->rows * K;
+    std::memset(bt->h_stage, 0, sizeof(double) * vn);
+    for (int s = 0; s < bt->nsys; s++) {
+        const size_t r0 = (size_t)bt->trbase[s] * bt->B;
+        for (int k = 0; k < bt->nrhs; k++)
+            for (int i = 0; i < bt->n[s]; i++) bt->h_stage[(r0 + i) * K + k] = b[s][(size_t)k * bt->n[s] + i];
+    }
+    MF_HIP(hipMemcpyAsync(bt->d_vec, bt->h_stage, sizeof(double) * vn, hipMemcpyHostToDevice, st));
+    MF_HIP(hipStreamSynchronize(st));
+    float ms = 0.0f;
+    int rc = batch_run(bt, &ms);
+    if (rc) return rc;
+    std::vector<int> so(4 * (size_t)bt->nsys);
+    std::vector<double> rel(bt->nsys);
+    int ab = 0;
+    MF_HIP(hipMemcpy(so.data(), bt->d_sys_out, sizeof(int) * so.size(), hipMemcpyDeviceToHost));
+    MF_HIP(hipMemcpy(rel.data(), bt->d_relres, sizeof(double) * rel.size(), hipMemcpyDeviceToHost));
+    MF_HIP(hipMemcpy(&ab, bt->d_ctl, sizeof(int), hipMemcpyDeviceToHost));
+    MF_HIP(hipMemcpy(bt->h_stage, bt->args.x, sizeof(double) * vn, hipMemcpyDeviceToHost));
+    for (int s = 0; s < bt->nsys; s++) {
+        const size_t r0 = (size_t)bt->trbase[s] * bt->B;
+        for (int k = 0; k < bt->nrhs; k++)
+            for (int i = 0; i < bt->n[s]; i++) x[s][(size_t)k * bt->n[s] + i] = bt->h_stage[(r0 + i) * K + k];
+    }
+    int status = ab ? MF_E_TIMEOUT : MF_OK, itmax = 0, repl = 0;
+    for (int s = 0; s < bt->nsys; s++) {
+        if (iters) iters[s] = so[4 * s + 1];
+        if (relres) relres[s] = rel[s];
+        itmax = std::max(itmax, so[4 * s + 1]);
+        repl += so[4 * s + 2];
+        if (status == MF_OK && so[4 * s] != MF_OK) status = so[4 * s];
+    }
+    if (rep) {
+        std::memset(rep, 0, sizeof(*rep));
+        rep->status = status;
+        rep->iterations = itmax;
+        rep->outer = 1;
+        rep->replacements = repl;
+        double worst = 0.0;
+        for (double r : rel) worst = std::max(worst, r);
+        rep->relres_true = worst;
+        rep->ms_solve = ms;
+        rep->ms_setup = bt->ms_setup;
+        rep->grid = bt->G;
+        rep->block = bt->TPB;
+        rep->rows_per_thread = 0;
+    }
+    return status == MF_E_TIMEOUT ? fail(MF_E_TIMEOUT, "group sync timeout") : MF_OK;
+}
+
+int mf_batch_info(mf_batch bt, mf_batch_info_t *info)
+{
+    if (!bt || !info) return fail(MF_E_ARG, "null argument");
+    info->nsys = bt->nsys;
+    info->nrhs = bt->nrhs;
+    info->B = bt->B;
+    info->G = bt->G;
+    info->rows = (long long)bt->rows;
+    info->tilenum = bt->tilenum_total;
+    info->bytes_alg = bt->bytes_alg;
+    info->ms_setup = bt->ms_setup;
+    const BatchLayout &Li = bt->mixed ? bt->Lmix : bt->L64;
+    for (int c = 0; c < 5; c++) info->nnz_per_code[c] = Li.nnz_code[c];
+    return MF_OK;
+}
+
+int mf_solve_batched(int nsys, const int *n, const int *const *rowptr, const int *const *colidx,
+                     const double *const *val, int nrhs, const double *const *b, double *const *x, const mf_opts *o,
+                     int *iters, double *relres, mf_report *rep)
+{
+    mf_batch bt = nullptr;
+    int rc = mf_batch_create(nsys, n, rowptr, colidx, val, nrhs, o, &bt);
+    if (rc) return rc;
+    rc = mf_batch_solve(bt, b, x, iters, relres, rep);
+    mf_batch_destroy(bt);
+    return rc;
+}
+
+}  // extern "C"

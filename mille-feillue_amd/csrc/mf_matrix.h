@@ -76,7 +76,8 @@ struct mf_matrix_s {
 
 namespace mf {
 int build_reference_tiles(mf_matrix m);              // tiles.hip
-int build_layout(mf_matrix m, Layout &L, double tau, int allow_mask);  // tiles.hip
+int build_layout(mf_matrix m, Layout &L, double tau, int allow_mask,
+                 bool force_idx16 = false);  // tiles.hip
 int encode_cols_device(mf_matrix m, uint8_t *d_out);  // tiles.hip
 int classes_device(mf_matrix m, double tau, int allow_mask, uint8_t *d_codes);  // tiles.hip
 hipStream_t stream();

@@ -722,7 +722,7 @@ void Layout::release()
 }
 
 template <int B>
-static int build_layout_B(mf_matrix m, Layout &L, double tau, int allow_mask)
+static int build_layout_B(mf_matrix m, Layout &L, double tau, int allow_mask, bool force_idx16)
 {
     hipStream_t st = stream();
     L.release();
@@ -751,7 +751,7 @@ static int build_layout_B(mf_matrix m, Layout &L, double tau, int allow_mask)
     MF_HIP(hipStreamSynchronize(st));
     // one-byte index: slot and column fit 8 bits and every slot has a lane of
     // the group to hold its tile column (B = 8: 8 slots, 16: 16, 32: 8)
-    const bool idx8 = hmax <= std::min(B, 1 << (8 - log2_of<B>()));
+    const bool idx8 = !force_idx16 && hmax <= std::min(B, 1 << (8 - log2_of<B>()));
     if (!idx8 && hmax > (1 << (16 - log2_of<B>()))) {
         (void)hipFree(codes); (void)hipFree(units); (void)hipFree(stats); (void)hipFree(err); (void)hipFree(maxnt);
         return fail(MF_E_FORMAT, "more tiles in one tile row than the two-byte slot index holds");
@@ -815,14 +815,14 @@ static int build_layout_B(mf_matrix m, Layout &L, double tau, int allow_mask)
     return MF_OK;
 }
 
-int build_layout(mf_matrix m, Layout &L, double tau, int allow_mask)
+int build_layout(mf_matrix m, Layout &L, double tau, int allow_mask, bool force_idx16)
 {
     double t0 = now_ms();
     int rc;
     switch (m->B) {
-    case 8: rc = build_layout_B<8>(m, L, tau, allow_mask); break;
-    case 16: rc = build_layout_B<16>(m, L, tau, allow_mask); break;
-    case 32: rc = build_layout_B<32>(m, L, tau, allow_mask); break;
+    case 8: rc = build_layout_B<8>(m, L, tau, allow_mask, force_idx16); break;
+    case 16: rc = build_layout_B<16>(m, L, tau, allow_mask, force_idx16); break;
+    case 32: rc = build_layout_B<32>(m, L, tau, allow_mask, force_idx16); break;
     default: return fail(MF_E_ARG, "tile size");
     }
     m->ms_layout += now_ms() - t0;

@@ -199,6 +199,9 @@ typedef struct {
     double bytes_alg;              /* per iteration: matrix once + 16 n per column */
     double ms_setup;
     long long nnz_per_code[5];
+    int mixed;                     /* inner layout has per-tile classes */
+    double tau;                    /* their rounding threshold */
+    char tuned[512];               /* the autotuner's trials and choice */
 } mf_batch_info_t;
 int mf_batch_create(int nsys, const int *n, const int *const *rowptr, const int *const *colidx,
                     const double *const *val, int nrhs, const mf_opts *o, mf_batch *out);
@@ -208,6 +211,11 @@ int mf_batch_create(int nsys, const int *n, const int *const *rowptr, const int 
 int mf_batch_solve(mf_batch bt, const double *const *b, double *const *x, int *iters, double *relres,
                    mf_report *rep);
 int mf_batch_info(mf_batch bt, mf_batch_info_t *info);
+/* Autotuner: trial solves of b pick the workgroups per system (weighted by
+ * measured iterations), the tile size (8/16/32) and, when o->precision is
+ * MF_MIXED, the tile threshold; the fastest converged choice stays in the
+ * batch.  rep describes the last trial. */
+int mf_batch_tune(mf_batch bt, const double *const *b, mf_report *rep);
 void mf_batch_destroy(mf_batch bt);
 
 /* Batched: nsys independent square systems, each with nrhs right-hand sides

@@ -88,7 +88,7 @@ EXPORTS = (
     "mf_solve_batched", "mf_mc_classify", "mf_compact_rows", "mf_partition_build", "mf_free",
     "mf_gen_laplace2d", "mf_gen_poisson3d", "mf_gen_convdiff3d", "mf_gen_random_spd",
     "mf_gen_stencil_lognormal", "mf_gen_uniform", "mf_gen_normal",
-    "mf_batch_create", "mf_batch_solve", "mf_batch_info", "mf_batch_destroy",
+    "mf_batch_create", "mf_batch_solve", "mf_batch_info", "mf_batch_destroy", "mf_batch_tune",
 )
 
 _L = None
@@ -146,6 +146,7 @@ def lib():
     L.mf_batch_solve.argtypes = [C.c_void_p, C.POINTER(_d), C.POINTER(_d), _i, _d, C.POINTER(Report)]
     L.mf_batch_info.argtypes = [C.c_void_p, C.POINTER(BatchInfo)]
     L.mf_batch_destroy.argtypes = [C.c_void_p]
+    L.mf_batch_tune.argtypes = [C.c_void_p, C.POINTER(_d), C.POINTER(Report)]
     _L = L
     return L
 
@@ -192,7 +193,8 @@ def ref_tiles_to_dict(t):
 class BatchInfo(C.Structure):
     _fields_ = [("nsys", C.c_int), ("nrhs", C.c_int), ("B", C.c_int), ("G", C.c_int),
                 ("rows", C.c_longlong), ("tilenum", C.c_longlong), ("bytes_alg", C.c_double),
-                ("ms_setup", C.c_double), ("nnz_per_code", C.c_longlong * 5)]
+                ("ms_setup", C.c_double), ("nnz_per_code", C.c_longlong * 5), ("mixed", C.c_int),
+                ("tau", C.c_double), ("tuned", C.c_char * 512)]
 
 
 class Batch:
@@ -218,9 +220,22 @@ class Batch:
     def info(self):
         i = BatchInfo()
         _check(lib().mf_batch_info(self.h, C.byref(i)))
-        d = {k: getattr(i, k) for k, _ in BatchInfo._fields_ if k != "nnz_per_code"}
+        d = {k: getattr(i, k) for k, _ in BatchInfo._fields_ if k not in ("nnz_per_code", "tuned")}
         d["nnz_per_code"] = list(i.nnz_per_code)
+        d["tuned"] = i.tuned.decode()
         return d
+
+    def _ptrs(self, b):
+        bs = [np.ascontiguousarray(np.reshape(bb, (self.nrhs, -1)), np.float64) for bb in b]
+        return bs, (_d * self.nsys)(*[_p(a, C.c_double) for a in bs])
+
+    def tune(self, b):
+        """Autotunes on right-hand sides b (trial solves); returns the report
+        of the last trial."""
+        bs, bp = self._ptrs(b)
+        rep = Report()
+        _check(lib().mf_batch_tune(self.h, bp, C.byref(rep)))
+        return rep.as_dict()
 
     def solve(self, b):
         """b: list of arrays of shape (nrhs, n_s) (or (n_s,) for nrhs = 1).

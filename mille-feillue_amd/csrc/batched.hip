@@ -22,6 +22,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "mf_matrix.h"
@@ -34,8 +35,8 @@ struct BatchArgs {
     DevL L;    // inner layout (mixed classes, or FP64)
     DevL L64;  // FP64 layout: true residuals
     int G, nrhs;
-    const int *wg_sys, *wg_tr0, *wg_tr1;  // per workgroup: system, tile rows [tr0, tr1)
-    const int *sys_wg0, *sys_nwg;         // per system: first workgroup, group size
+    const int *wg_task;                   // [G + 1]: tasks wg_task[g] .. wg_task[g + 1] - 1 of workgroup g
+    const int *task;                      // [5 t]: system, tile rows tr0, tr1, group's first workgroup, group size
     const double *b;                      // [rows][K]
     double *x, *r, *p, *s;                // [rows][K]
     float *r32;                           // FP32 copy of r for low-precision tiles
@@ -46,6 +47,7 @@ struct BatchArgs {
     double tol, rr_factor;
     int maxit, haslow;
     long long spin_limit;
+    unsigned long long *prof;  // optional per workgroup: [4 g] spmv, update, sync, pass (100 MHz ticks)
 };
 
 __device__ __forceinline__ void st_ag(double *p, double v)
@@ -254,24 +256,34 @@ __device__ __forceinline__ void row_spmv64_xp(const DevL &L, int tr, int r, int 
     }
 }
 
+// Per-column CG scalars of the system at hand, in LDS: gamma, previous
+// gamma, previous alpha, reference ||r||^2 of the last replacement, ||b||^2,
+// (r, s) and (p, s) of the last replacement, alpha, beta; mode bits: 0
+// restart, 1 replaced, 2 fresh, 3 after replacement, 4 frozen; breakdown
+// flags: bit 0 in a fresh step, bit 1 otherwise.
+template <int K, int TPB>
+struct BatchShared {
+    double red[(TPB / 64) * BK_MAXV];
+    double bc[BK_MAXV];
+    int flag;
+    double g[K], gp[K], ap[K], ref[K], b2[K], rs[K], ps[K], al[K], be[K];
+    int mode[K];
+    int bd;
+};
+
+// CG on one system (tile rows tr0 .. tr1 - 1 of it in this workgroup);
+// returns false when the group sync gave up (abort word raised).
 template <int B, bool IDX8, int K, int TPB>
-__global__ __launch_bounds__(TPB) void k_cg_batched(BatchArgs A)
+__device__ bool cg_system(const BatchArgs &A, BatchShared<K, TPB> &sh, int sys, int tr0, int tr1, int gw0, int gn)
 {
     constexpr int GPB = TPB / B;
-    __shared__ double s_red[(TPB / 64) * BK_MAXV];
-    __shared__ double s_bc[BK_MAXV];
-    __shared__ int s_flag;
-    // per-column CG scalars: gamma, previous gamma, previous alpha, reference
-    // ||r||^2 of the last replacement, ||b||^2, (r, s) and (p, s) of the last
-    // replacement, alpha, beta
-    __shared__ double sh_g[K], sh_gp[K], sh_ap[K], sh_ref[K], sh_b2[K], sh_rs[K], sh_ps[K], sh_al[K], sh_be[K];
-    __shared__ int sh_mode[K];  // bit 0 restart, bit 1 replaced, bit 2 fresh, bit 3 after replacement, bit 4 frozen
-    __shared__ int sh_bd;       // breakdown flags: bit 0 in a fresh step, bit 1 otherwise
+    double *sh_g = sh.g, *sh_gp = sh.gp, *sh_ap = sh.ap, *sh_ref = sh.ref, *sh_b2 = sh.b2, *sh_rs = sh.rs,
+           *sh_ps = sh.ps, *sh_al = sh.al, *sh_be = sh.be;
+    int *sh_mode = sh.mode;
+    int &sh_bd = sh.bd;
     const int g = blockIdx.x, tid = threadIdx.x;
-    const int sys = A.wg_sys[g], gw0 = A.sys_wg0[sys], gn = A.sys_nwg[sys];
-    const int tr0 = A.wg_tr0[g], tr1 = A.wg_tr1[g];
     const int grp = tid / B, lane = tid % B, lanebase = (tid & 63) & ~(B - 1);
-    GroupSync<TPB> S{s_red, s_bc, &s_flag, g - gw0, gn, gw0};
+    GroupSync<TPB> S{sh.red, sh.bc, &sh.flag, g - gw0, gn, gw0};
     const int KA = A.nrhs;
     const double tol2 = A.tol * A.tol, rr2 = A.rr_factor * A.rr_factor;
     auto finish = [&](int status, int iter, int repl, double relres) {
@@ -284,23 +296,22 @@ __global__ __launch_bounds__(TPB) void k_cg_batched(BatchArgs A)
     };
 
     double v[K];
-#pragma unroll
-    for (int k = 0; k < K; k++) v[k] = 0.0;
-    for (int tr = tr0 + grp; tr < tr1; tr += GPB) {
-        const size_t row = (size_t)tr * B + lane;
-#pragma unroll
-        for (int k = 0; k < K; k++) {
-            const size_t e = row * K + k;
+    {
+        const int km = tid % K;
+        double acc = 0.0;
+        for (size_t e = (size_t)tr0 * B * K + tid; e < (size_t)tr1 * B * K; e += TPB) {
             const double bv = A.b[e];
             st_ag(A.x + e, 0.0);
             st_ag(A.p + e, 0.0);
             A.s[e] = 0.0;
             st_ag(A.r + e, bv);
             if (A.haslow) st_ag(A.r32 + e, (float)bv);
-            v[k] += bv * bv;
+            acc += bv * bv;
         }
+#pragma unroll
+        for (int k = 0; k < K; k++) v[k] = k == km ? acc : 0.0;
     }
-    if (!S.allreduce(A, v, true)) return;
+    if (!S.allreduce(A, v, true)) return false;
     if (tid < K) {
         sh_b2[tid] = sh_g[tid] = sh_ref[tid] = v[tid];
         sh_gp[tid] = sh_ap[tid] = 1.0;
@@ -309,6 +320,15 @@ __global__ __launch_bounds__(TPB) void k_cg_batched(BatchArgs A)
     __syncthreads();
     int iter = 0, repl = 0;
     bool force = false;
+    unsigned long long *pf = A.prof && tid == 0 ? A.prof + 4 * (size_t)g : nullptr;
+    unsigned long long tk = pf ? wall_clock64() : 0;
+    auto lap = [&](int k) {
+        if (pf) {
+            const unsigned long long now = wall_clock64();
+            pf[k] += now - tk;
+            tk = now;
+        }
+    };
     for (;;) {
         bool conv = true, due = A.rr_factor > 0.0, finite = true;
         for (int k = 0; k < KA; k++) {
@@ -317,10 +337,10 @@ __global__ __launch_bounds__(TPB) void k_cg_batched(BatchArgs A)
             conv = conv && gk <= tol2 * sh_b2[k];
             due = due && gk <= rr2 * sh_ref[k];
         }
-        if (!finite) { finish(MF_E_NONFINITE, iter, repl, -1.0); return; }
+        if (!finite) { finish(MF_E_NONFINITE, iter, repl, -1.0); return true; }
         if (conv || due || force) {
             // r = b - A x and s = A p on the FP64 layout; (r,r), (r,s), (p,s)
-            if (!S.allreduce(A, v, true)) return;  // x and p of every row visible
+            if (!S.allreduce(A, v, true)) return false;  // x and p of every row visible
             double t[3 * K];
 #pragma unroll
             for (int k = 0; k < 3 * K; k++) t[k] = 0.0;
@@ -340,7 +360,8 @@ __global__ __launch_bounds__(TPB) void k_cg_batched(BatchArgs A)
                     t[2 * K + k] += pv * ap[k];
                 }
             }
-            if (!S.allreduce(A, t, true)) return;
+            if (!S.allreduce(A, t, true)) return false;
+            lap(3);
             ++repl;
             bool done = true;
             double rel = 0.0;
@@ -348,8 +369,8 @@ __global__ __launch_bounds__(TPB) void k_cg_batched(BatchArgs A)
                 done = done && t[k] <= tol2 * sh_b2[k];
                 if (sh_b2[k] > 0.0) rel = fmax(rel, sqrt(t[k] / sh_b2[k]));
             }
-            if (!isfinite(rel)) { finish(MF_E_NONFINITE, iter, repl, rel); return; }
-            if (done) { finish(MF_OK, iter, repl, rel); return; }
+            if (!isfinite(rel)) { finish(MF_E_NONFINITE, iter, repl, rel); return true; }
+            if (done) { finish(MF_OK, iter, repl, rel); return true; }
             if (tid < K) {
                 const bool restart = force || t[tid] > 4.0 * sh_g[tid];
                 sh_mode[tid] = (restart ? 1 : 0) | 2;
@@ -366,7 +387,7 @@ __global__ __launch_bounds__(TPB) void k_cg_batched(BatchArgs A)
             for (int k = 0; k < KA; k++)
                 if (sh_b2[k] > 0.0) rel = fmax(rel, sqrt(sh_g[k] / sh_b2[k]));
             finish(MF_E_MAXIT, iter, repl, rel);
-            return;
+            return true;
         }
         if (tid < K) {
             const int m = sh_mode[tid];
@@ -392,7 +413,9 @@ __global__ __launch_bounds__(TPB) void k_cg_batched(BatchArgs A)
                 d[k] += w[k] * A.r[e];
             }
         }
-        if (!S.allreduce(A, d, false)) return;
+        lap(0);
+        if (!S.allreduce(A, d, false)) return false;
+        lap(2);
         if (tid < K) {
             const int m = sh_mode[tid];
             const double be = sh_be[tid], gk = sh_g[tid];
@@ -412,30 +435,49 @@ __global__ __launch_bounds__(TPB) void k_cg_batched(BatchArgs A)
         if (sh_bd) {
             // rounding broke (p, A p) > 0: recompute the residual and restart;
             // a breakdown in a fresh step is final
-            if (sh_bd & 1) { finish(MF_E_BREAKDOWN, iter, repl, -1.0); return; }
+            if (sh_bd & 1) { finish(MF_E_BREAKDOWN, iter, repl, -1.0); return true; }
             force = true;
             continue;
         }
+        // p, x, r updates stream the workgroup's elements [row][K] with unit
+        // stride; every thread keeps one column (TPB is a multiple of K)
         double gg[K];
+        {
+            const int km = tid % K;
+            const double al = sh_al[km], be = sh_be[km];
+            const size_t e1 = (size_t)tr1 * B * K;
+            double acc = 0.0;
+            constexpr int U = 4;
+            for (size_t e0 = (size_t)tr0 * B * K + tid; e0 < e1; e0 += (size_t)U * TPB) {
+                double rv0[U], pv0[U], xv0[U], sv0[U];
 #pragma unroll
-        for (int k = 0; k < K; k++) gg[k] = 0.0;
-        for (int tr = tr0 + grp; tr < tr1; tr += GPB) {
-            const size_t row = (size_t)tr * B + lane;
+                for (int u = 0; u < U; u++) {
+                    const size_t e = min(e0 + (size_t)u * TPB, e1 - 1);
+                    rv0[u] = A.r[e];
+                    pv0[u] = A.p[e];
+                    xv0[u] = A.x[e];
+                    sv0[u] = A.s[e];
+                }
 #pragma unroll
-            for (int k = 0; k < K; k++) {
-                const size_t e = row * K + k;
-                const double rv0 = A.r[e];
-                const double pv = rv0 + sh_be[k] * A.p[e];
-                const double xv = A.x[e] + sh_al[k] * pv;
-                const double rv = rv0 - sh_al[k] * A.s[e];
-                st_ag(A.p + e, pv);
-                st_ag(A.x + e, xv);
-                st_ag(A.r + e, rv);
-                if (A.haslow) st_ag(A.r32 + e, (float)rv);
-                gg[k] += rv * rv;
+                for (int u = 0; u < U; u++) {
+                    const size_t e = e0 + (size_t)u * TPB;
+                    if (e < e1) {
+                        const double pv = rv0[u] + be * pv0[u];
+                        const double rv = rv0[u] - al * sv0[u];
+                        st_ag(A.p + e, pv);
+                        st_ag(A.x + e, xv0[u] + al * pv);
+                        st_ag(A.r + e, rv);
+                        if (A.haslow) st_ag(A.r32 + e, (float)rv);
+                        acc += rv * rv;
+                    }
+                }
             }
+#pragma unroll
+            for (int k = 0; k < K; k++) gg[k] = k == km ? acc : 0.0;
         }
-        if (!S.allreduce(A, gg, true)) return;
+        lap(1);
+        if (!S.allreduce(A, gg, true)) return false;
+        lap(2);
         if (tid < K) {
             if (!(sh_mode[tid] & 16)) {
                 sh_gp[tid] = sh_g[tid];
@@ -446,6 +488,17 @@ __global__ __launch_bounds__(TPB) void k_cg_batched(BatchArgs A)
         }
         __syncthreads();
         ++iter;
+    }
+}
+
+template <int B, bool IDX8, int K, int TPB>
+__global__ __launch_bounds__(TPB) void k_cg_batched(BatchArgs A)
+{
+    __shared__ BatchShared<K, TPB> sh;
+    for (int t = A.wg_task[blockIdx.x]; t < A.wg_task[blockIdx.x + 1]; t++) {
+        const int *q = A.task + 5 * (size_t)t;
+        if (!cg_system<B, IDX8, K, TPB>(A, sh, q[0], q[1], q[2], q[3], q[4])) return;
+        __syncthreads();  // LDS scalars are reused by the next system
     }
 }
 
@@ -498,6 +551,10 @@ struct mf_batch_s {
     bool mixed = false, idx8 = true;
     mf_opts o;
     std::vector<int> n, tilem, trbase, nnz;
+    std::vector<std::vector<int>> h_rp, h_ci;  // host copies of the systems (rebuilds)
+    std::vector<std::vector<double>> h_va;
+    std::vector<int> weights;                  // iteration weights of the assignment
+    std::string tuned;                         // what the tuner chose
     std::vector<long long> tbase;
     std::vector<double> tau;
     int tilem_total = 0;
@@ -507,13 +564,14 @@ struct mf_batch_s {
     BatchLayout L64, Lmix;
     int G = 0;
     std::vector<int> sys_nwg;
-    int *d_wg = nullptr;  // wg_sys, wg_tr0, wg_tr1 [3][G], then sys_wg0, sys_nwg [2][nsys]
+    int *d_wg = nullptr;  // wg_task [G + 1], then task records [5][tasks]
     double *d_vec = nullptr;  // b, x, r, p, s: 5 x rows x K
     float *d_r32 = nullptr;
     unsigned long long *d_slots = nullptr;
     int *d_ctl = nullptr, *d_sys_out = nullptr;
     double *d_relres = nullptr;
     double *h_stage = nullptr;  // pinned [rows][K]
+    unsigned long long *d_prof = nullptr;  // optional phase counters
     const void *fn = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -522,16 +580,16 @@ struct mf_batch_s {
     double bytes_alg = 0.0;  // per iteration: matrix once + K (16 n) vector bytes
 };
 
+static void batch_release_data(mf_batch_s *bt);
+
 static void batch_release(mf_batch_s *bt)
 {
     if (bt->gexec) (void)hipGraphExecDestroy(bt->gexec);
     if (bt->graph) (void)hipGraphDestroy(bt->graph);
-    bt->L64.release();
-    bt->Lmix.release();
-    void *ps[] = {bt->d_tile_ptr, bt->d_wg, bt->d_vec, bt->d_r32, bt->d_slots, bt->d_ctl, bt->d_sys_out, bt->d_relres};
-    for (void *p : ps)
-        if (p) (void)hipFree(p);
-    if (bt->h_stage) (void)hipHostFree(bt->h_stage);
+    batch_release_data(bt);
+    if (bt->d_wg) (void)hipFree(bt->d_wg);
+    if (bt->d_slots) (void)hipFree(bt->d_slots);
+    if (bt->d_prof) (void)hipFree(bt->d_prof);
 }
 
 typedef const void *KernelFn;
@@ -586,36 +644,61 @@ static int concat_layouts(mf_batch_s *bt, const std::vector<mf_matrix> &mats, bo
     return MF_OK;
 }
 
-static int batch_assign(mf_batch_s *bt, int g_target)
+// Work assignment: a system whose work exceeds 1.5 workgroup shares gets a
+// group of its own (one task per workgroup of the group, tile rows split
+// evenly); the other systems are packed onto the remaining workgroups,
+// largest first onto the least loaded one, and run one after another.
+// weights: per-system iteration counts of an earlier solve (work per
+// iteration x iterations), or empty (work per iteration only).
+static int batch_assign(mf_batch_s *bt, int g_target, const std::vector<int> &iters)
 {
-    // work of a system ~ its slab bytes plus K vector passes over its rows
     std::vector<double> work(bt->nsys);
     double total = 0.0;
     for (int s = 0; s < bt->nsys; s++) {
         work[s] = 12.0 * bt->nnz[s] + 48.0 * bt->K * bt->tilem[s] * bt->B;
+        if (!iters.empty()) work[s] *= std::max(1, iters[s]);
         total += work[s];
     }
+    g_target = std::max(1, g_target);
     const int gpb = bt->TPB / bt->B;
+    const double share = total / g_target;
     bt->sys_nwg.assign(bt->nsys, 1);
-    int G = 0;
+    std::vector<int> big, small;
+    int gbig = 0;
     for (int s = 0; s < bt->nsys; s++) {
-        int w = (int)std::lround(g_target * work[s] / total);
-        w = std::max(1, std::min(w, std::max(1, bt->tilem[s] / gpb)));
+        int w = work[s] > 1.5 * share ? (int)std::ceil(work[s] / share) : 1;
+        w = std::min(w, std::max(1, bt->tilem[s] / gpb));
         bt->sys_nwg[s] = w;
-        G += w;
-    }
-    std::vector<int> h(3 * (size_t)G + 2 * (size_t)bt->nsys);
-    int g = 0;
-    for (int s = 0; s < bt->nsys; s++) {
-        const int w = bt->sys_nwg[s];
-        h[3 * (size_t)G + s] = g;
-        h[3 * (size_t)G + bt->nsys + s] = w;
-        for (int q = 0; q < w; q++, g++) {
-            h[g] = s;
-            h[(size_t)G + g] = bt->trbase[s] + (int)((long long)bt->tilem[s] * q / w);
-            h[2 * (size_t)G + g] = bt->trbase[s] + (int)((long long)bt->tilem[s] * (q + 1) / w);
+        if (w > 1) {
+            big.push_back(s);
+            gbig += w;
+        } else {
+            small.push_back(s);
         }
     }
+    const int gsmall = small.empty() ? 0 : std::max(1, std::min<int>((int)small.size(), g_target - gbig));
+    const int G = gbig + gsmall;
+    std::vector<std::vector<int>> tasks(G);  // per workgroup: task records (5 ints)
+    int g = 0;
+    for (int s : big) {
+        const int w = bt->sys_nwg[s];
+        for (int q = 0; q < w; q++, g++)
+            tasks[g] = {s, bt->trbase[s] + (int)((long long)bt->tilem[s] * q / w),
+                        bt->trbase[s] + (int)((long long)bt->tilem[s] * (q + 1) / w), g - q, w};
+    }
+    std::sort(small.begin(), small.end(), [&](int a, int c) { return work[a] > work[c] || (work[a] == work[c] && a < c); });
+    std::vector<double> load(gsmall, 0.0);
+    for (int s : small) {
+        int best = 0;
+        for (int q = 1; q < gsmall; q++)
+            if (load[q] < load[best]) best = q;
+        load[best] += work[s];
+        const int wg = gbig + best;
+        tasks[wg].insert(tasks[wg].end(), {s, bt->trbase[s], bt->trbase[s] + bt->tilem[s], wg, 1});
+    }
+    std::vector<int> h(G + 1, 0);
+    for (int q = 0; q < G; q++) h[q + 1] = h[q] + (int)tasks[q].size() / 5;
+    for (int q = 0; q < G; q++) h.insert(h.end(), tasks[q].begin(), tasks[q].end());
     if (bt->d_wg) MF_HIP(hipFree(bt->d_wg));
     if (bt->d_slots) MF_HIP(hipFree(bt->d_slots));
     bt->d_wg = nullptr;
@@ -638,11 +721,8 @@ static int batch_capture(mf_batch_s *bt)
     A.L64 = DevL{bt->d_tile_ptr, bt->L64.trh, bt->L64.stc, bt->L64.blob};
     A.G = bt->G;
     A.nrhs = bt->nrhs;
-    A.wg_sys = bt->d_wg;
-    A.wg_tr0 = bt->d_wg + bt->G;
-    A.wg_tr1 = bt->d_wg + 2 * bt->G;
-    A.sys_wg0 = bt->d_wg + 3 * bt->G;
-    A.sys_nwg = bt->d_wg + 3 * bt->G + bt->nsys;
+    A.wg_task = bt->d_wg;
+    A.task = bt->d_wg + bt->G + 1;
     const size_t vn = bt->rows * bt->K;
     A.b = bt->d_vec;
     A.x = bt->d_vec + vn;
@@ -659,6 +739,14 @@ static int batch_capture(mf_batch_s *bt)
     A.maxit = bt->o.maxit;
     A.haslow = Li.haslow ? 1 : 0;
     A.spin_limit = bt->o.spin_limit;
+    A.prof = nullptr;
+    if (std::getenv("MF_PHASE_PROFILE")) {
+        if (bt->d_prof) (void)hipFree(bt->d_prof);
+        bt->d_prof = nullptr;
+        MF_HIP(hipMalloc((void **)&bt->d_prof, sizeof(unsigned long long) * 4 * bt->G));
+        MF_HIP(hipMemset(bt->d_prof, 0, sizeof(unsigned long long) * 4 * bt->G));
+        A.prof = bt->d_prof;
+    }
     bt->fn = pick_batched(bt->B, bt->idx8, bt->K);
     int per_cu = 0;
     MF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bt->fn, bt->TPB, 0));
@@ -683,9 +771,27 @@ static int batch_capture(mf_batch_s *bt)
     return MF_OK;
 }
 
-static int batch_build(mf_batch_s *bt, const int *const *rowptr, const int *const *colidx, const double *const *val)
+static void batch_release_data(mf_batch_s *bt)
+{
+    bt->L64.release();
+    bt->Lmix.release();
+    void *ps[] = {bt->d_tile_ptr, bt->d_vec, bt->d_r32, bt->d_ctl, bt->d_sys_out, bt->d_relres};
+    for (void *p : ps)
+        if (p) (void)hipFree(p);
+    if (bt->h_stage) (void)hipHostFree(bt->h_stage);
+    bt->d_tile_ptr = nullptr;
+    bt->d_vec = nullptr;
+    bt->d_r32 = nullptr;
+    bt->d_ctl = bt->d_sys_out = nullptr;
+    bt->d_relres = nullptr;
+    bt->h_stage = nullptr;
+}
+
+// (Re)builds layouts and vectors for the current tile size and thresholds.
+static int batch_build(mf_batch_s *bt)
 {
     hipStream_t st = stream();
+    batch_release_data(bt);
     std::vector<mf_matrix> mats(bt->nsys, nullptr);
     auto cleanup = [&]() {
         for (mf_matrix m : mats)
@@ -696,8 +802,8 @@ static int batch_build(mf_batch_s *bt, const int *const *rowptr, const int *cons
     for (int pass = 0; pass < 2; pass++) {
         bool all8 = true;
         for (int s = 0; s < bt->nsys && !rc; s++) {
-            if (!mats[s]) rc = mf_matrix_create(bt->n[s], bt->n[s], rowptr[s][bt->n[s]], rowptr[s], colidx[s], val[s],
-                                                 nullptr, bt->B, &mats[s]);
+            if (!mats[s]) rc = mf_matrix_create(bt->n[s], bt->n[s], bt->h_rp[s][bt->n[s]], bt->h_rp[s].data(),
+                                                 bt->h_ci[s].data(), bt->h_va[s].data(), nullptr, bt->B, &mats[s]);
             if (!rc) rc = build_layout(mats[s], mats[s]->L64, 0.0, 0, force16);
             if (!rc && bt->mixed) rc = build_layout(mats[s], mats[s]->Lmix, bt->tau[s], bt->o.allow_mask, force16);
             if (!rc) all8 = all8 && mats[s]->L64.idx8 && (!bt->mixed || mats[s]->Lmix.idx8);
@@ -754,6 +860,23 @@ static int batch_build(mf_batch_s *bt, const int *const *rowptr, const int *cons
     return MF_OK;
 }
 
+// Workgroups of the batched kernel that can be resident at once.
+static int batch_capacity(mf_batch_s *bt)
+{
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pick_batched(bt->B, bt->idx8, bt->K), bt->TPB, 0) !=
+        hipSuccess)
+        per_cu = 1;
+    return std::max(1, per_cu) * device_cu_count();
+}
+
+// grid target: opts.grid when set, else the resident capacity (never more)
+static int batch_target(mf_batch_s *bt)
+{
+    const int cap = batch_capacity(bt);
+    return bt->o.grid > 0 ? std::min(bt->o.grid, cap) : cap;
+}
+
 static int batch_run(mf_batch_s *bt, float *ms)
 {
     hipStream_t st = stream();
@@ -796,8 +919,25 @@ int mf_batch_create(int nsys, const int *n, const int *const *rowptr, const int 
     bt->n.assign(n, n + nsys);
     bt->tau.assign(nsys, bt->o.tau);
     bt->B = bt->o.block == 8 || bt->o.block == 32 ? bt->o.block : 16;
-    rc = batch_build(bt, rowptr, colidx, val);
-    if (!rc) rc = batch_assign(bt, bt->o.grid > 0 ? bt->o.grid : 2 * device_cu_count());
+    bt->h_rp.resize(nsys);
+    bt->h_ci.resize(nsys);
+    bt->h_va.resize(nsys);
+    bt->weights.resize(nsys);
+    for (int s = 0; s < nsys && !rc; s++) {
+        if (n[s] <= 0 || !rowptr[s]) rc = fail(MF_E_ARG, "empty system");
+        else {
+            const int nz = rowptr[s][n[s]];
+            bt->h_rp[s].assign(rowptr[s], rowptr[s] + n[s] + 1);
+            bt->h_ci[s].assign(colidx[s], colidx[s] + nz);
+            bt->h_va[s].assign(val[s], val[s] + nz);
+            // CG iterations grow like the square root of the condition number,
+            // about n^(1/d) for d-dimensional stencils: sqrt(n) as a first
+            // guess until a solve has measured them
+            bt->weights[s] = (int)std::lround(std::sqrt((double)n[s]));
+        }
+    }
+    if (!rc) rc = batch_build(bt);
+    if (!rc) rc = batch_assign(bt, batch_target(bt), bt->weights);
     if (!rc) rc = batch_capture(bt);
     if (rc) {
         batch_release(bt);
@@ -816,10 +956,11 @@ void mf_batch_destroy(mf_batch bt)
     delete bt;
 }
 
-int mf_batch_solve(mf_batch bt, const double *const *b, double *const *x, int *iters, double *relres, mf_report *rep)
+}  // extern "C"
+
+// Right-hand sides into the device layout ([row][K], padded rows zero).
+static int batch_stage(mf_batch_s *bt, const double *const *b)
 {
-    clear_error();
-    if (!bt || !b || !x) return fail(MF_E_ARG, "null argument");
     hipStream_t st = stream();
     const int K = bt->K;
     const size_t vn = bt->rows * K;
@@ -831,45 +972,188 @@ int mf_batch_solve(mf_batch bt, const double *const *b, double *const *x, int *i
     }
     MF_HIP(hipMemcpyAsync(bt->d_vec, bt->h_stage, sizeof(double) * vn, hipMemcpyHostToDevice, st));
     MF_HIP(hipStreamSynchronize(st));
-    float ms = 0.0f;
-    int rc = batch_run(bt, &ms);
-    if (rc) return rc;
+    return MF_OK;
+}
+
+struct BatchOutcome {
+    int status = MF_OK, itmax = 0, repl = 0;
+    double worst = 0.0;
+    std::vector<int> iters;
+    std::vector<double> relres;
+};
+
+static int batch_outcome(mf_batch_s *bt, BatchOutcome &oc)
+{
     std::vector<int> so(4 * (size_t)bt->nsys);
-    std::vector<double> rel(bt->nsys);
+    oc.relres.assign(bt->nsys, 0.0);
+    oc.iters.assign(bt->nsys, 0);
     int ab = 0;
     MF_HIP(hipMemcpy(so.data(), bt->d_sys_out, sizeof(int) * so.size(), hipMemcpyDeviceToHost));
-    MF_HIP(hipMemcpy(rel.data(), bt->d_relres, sizeof(double) * rel.size(), hipMemcpyDeviceToHost));
+    MF_HIP(hipMemcpy(oc.relres.data(), bt->d_relres, sizeof(double) * bt->nsys, hipMemcpyDeviceToHost));
     MF_HIP(hipMemcpy(&ab, bt->d_ctl, sizeof(int), hipMemcpyDeviceToHost));
+    oc.status = ab ? MF_E_TIMEOUT : MF_OK;
+    oc.itmax = oc.repl = 0;
+    oc.worst = 0.0;
+    for (int s = 0; s < bt->nsys; s++) {
+        oc.iters[s] = so[4 * s + 1];
+        oc.itmax = std::max(oc.itmax, so[4 * s + 1]);
+        oc.repl += so[4 * s + 2];
+        oc.worst = std::max(oc.worst, oc.relres[s]);
+        if (oc.status == MF_OK && so[4 * s] != MF_OK) oc.status = so[4 * s];
+    }
+    return MF_OK;
+}
+
+static void batch_report(mf_batch_s *bt, const BatchOutcome &oc, float ms, mf_report *rep)
+{
+    if (!rep) return;
+    std::memset(rep, 0, sizeof(*rep));
+    rep->status = oc.status;
+    rep->iterations = oc.itmax;
+    rep->outer = 1;
+    rep->replacements = oc.repl;
+    rep->relres_true = oc.worst;
+    rep->ms_solve = ms;
+    rep->ms_setup = bt->ms_setup;
+    rep->grid = bt->G;
+    rep->block = bt->TPB;
+    rep->rows_per_thread = 0;
+}
+
+extern "C" {
+
+int mf_batch_solve(mf_batch bt, const double *const *b, double *const *x, int *iters, double *relres, mf_report *rep)
+{
+    clear_error();
+    if (!bt || !b || !x) return fail(MF_E_ARG, "null argument");
+    int rc = batch_stage(bt, b);
+    float ms = 0.0f;
+    if (!rc) rc = batch_run(bt, &ms);
+    BatchOutcome oc;
+    if (!rc) rc = batch_outcome(bt, oc);
+    if (rc) return rc;
+    const int K = bt->K;
+    const size_t vn = bt->rows * K;
     MF_HIP(hipMemcpy(bt->h_stage, bt->args.x, sizeof(double) * vn, hipMemcpyDeviceToHost));
     for (int s = 0; s < bt->nsys; s++) {
         const size_t r0 = (size_t)bt->trbase[s] * bt->B;
         for (int k = 0; k < bt->nrhs; k++)
             for (int i = 0; i < bt->n[s]; i++) x[s][(size_t)k * bt->n[s] + i] = bt->h_stage[(r0 + i) * K + k];
+        if (iters) iters[s] = oc.iters[s];
+        if (relres) relres[s] = oc.relres[s];
     }
-    int status = ab ? MF_E_TIMEOUT : MF_OK, itmax = 0, repl = 0;
-    for (int s = 0; s < bt->nsys; s++) {
-        if (iters) iters[s] = so[4 * s + 1];
-        if (relres) relres[s] = rel[s];
-        itmax = std::max(itmax, so[4 * s + 1]);
-        repl += so[4 * s + 2];
-        if (status == MF_OK && so[4 * s] != MF_OK) status = so[4 * s];
+    batch_report(bt, oc, ms, rep);
+    if (bt->args.prof) {
+        std::vector<unsigned long long> pf(4 * (size_t)bt->G);
+        MF_HIP(hipMemcpy(pf.data(), bt->args.prof, sizeof(unsigned long long) * pf.size(), hipMemcpyDeviceToHost));
+        double sum[4] = {0, 0, 0, 0}, tmax = 0;
+        int gmax = 0;
+        for (int g = 0; g < bt->G; g++) {
+            double t = 0;
+            for (int k = 0; k < 4; k++) {
+                sum[k] += pf[4 * g + k] / 100.0;
+                t += pf[4 * g + k] / 100.0;
+            }
+            if (t > tmax) { tmax = t; gmax = g; }
+        }
+        std::fprintf(stderr, "[mf batch phase] mean us per workgroup: spmv %.0f update %.0f sync %.0f pass %.0f | "
+                     "busiest workgroup %d: spmv %.0f update %.0f sync %.0f pass %.0f\n",
+                     sum[0] / bt->G, sum[1] / bt->G, sum[2] / bt->G, sum[3] / bt->G, gmax, pf[4 * gmax] / 100.0,
+                     pf[4 * gmax + 1] / 100.0, pf[4 * gmax + 2] / 100.0, pf[4 * gmax + 3] / 100.0);
+        MF_HIP(hipMemset(bt->args.prof, 0, sizeof(unsigned long long) * pf.size()));
     }
-    if (rep) {
-        std::memset(rep, 0, sizeof(*rep));
-        rep->status = status;
-        rep->iterations = itmax;
-        rep->outer = 1;
-        rep->replacements = repl;
-        double worst = 0.0;
-        for (double r : rel) worst = std::max(worst, r);
-        rep->relres_true = worst;
-        rep->ms_solve = ms;
-        rep->ms_setup = bt->ms_setup;
-        rep->grid = bt->G;
-        rep->block = bt->TPB;
-        rep->rows_per_thread = 0;
+    return oc.status == MF_E_TIMEOUT ? fail(MF_E_TIMEOUT, "group sync timeout") : MF_OK;
+}
+
+// Autotuner.  Trial solves of the caller's right-hand sides, each timed with
+// device events:
+//  1. the current configuration, to measure every system's iterations;
+//  2. workgroups re-assigned by (work per iteration) x (iterations), so that
+//     the systems needing many iterations get more workgroups;
+//  3. tile sizes 8, 16 and 32 (layouts rebuilt on the GPU);
+//  4. when the caller asked for mixed precision: tile thresholds tau = 2^-24
+//     (FP32-rounded tiles) and 2^-11 (FP16-rounded), kept only if every
+//     column still reaches the tolerance.
+// The fastest converged configuration stays captured in the batch's graph.
+int mf_batch_tune(mf_batch bt, const double *const *b, mf_report *rep)
+{
+    clear_error();
+    if (!bt || !b) return fail(MF_E_ARG, "null argument");
+    struct Cfg {
+        int B;
+        bool mixed;
+        double tau;
+    };
+    auto apply = [&](const Cfg &c, bool rebuild) -> int {
+        bt->B = c.B;
+        bt->mixed = c.mixed;
+        bt->tau.assign(bt->nsys, c.tau);
+        int rc = rebuild ? batch_build(bt) : MF_OK;
+        if (!rc) rc = batch_assign(bt, batch_target(bt), bt->weights);
+        if (!rc) rc = batch_capture(bt);
+        return rc;
+    };
+    auto trial = [&](float *ms, BatchOutcome &oc) -> int {
+        int rc = batch_stage(bt, b);
+        if (!rc) rc = batch_run(bt, ms);
+        if (!rc) rc = batch_outcome(bt, oc);
+        return rc;
+    };
+    const bool want_mixed = bt->o.precision == MF_MIXED;
+    Cfg cur{bt->B, bt->mixed, bt->tau.empty() ? 0.0 : bt->tau[0]};
+    float ms = 0.0f;
+    BatchOutcome oc;
+    int rc = trial(&ms, oc);
+    if (rc) return rc;
+    if (oc.status != MF_OK) return fail(oc.status, "tuning trial did not converge");
+    bt->weights = oc.iters;
+    rc = apply(cur, false);
+    if (!rc) rc = trial(&ms, oc);
+    if (rc) return rc;
+    Cfg best = cur;
+    float best_ms = ms;
+    bt->weights = oc.iters;
+    char buf[256];
+    std::string log;
+    std::snprintf(buf, sizeof buf, "B=%d %s: %.2f ms", cur.B, cur.mixed ? "mixed" : "fp64", ms);
+    log += buf;
+    std::vector<Cfg> cands;
+    for (int B : {8, 16, 32})
+        if (B != cur.B) cands.push_back({B, cur.mixed, cur.tau});
+    for (const Cfg &c : cands) {
+        rc = apply(c, true);
+        if (!rc) rc = trial(&ms, oc);
+        if (rc) return rc;
+        std::snprintf(buf, sizeof buf, "; B=%d: %.2f ms%s", c.B, ms, oc.status ? " (not converged)" : "");
+        log += buf;
+        if (oc.status == MF_OK && oc.worst <= bt->o.tol && ms < best_ms) {
+            best = c;
+            best_ms = ms;
+        }
     }
-    return status == MF_E_TIMEOUT ? fail(MF_E_TIMEOUT, "group sync timeout") : MF_OK;
+    if (want_mixed) {
+        for (double tau : {std::ldexp(1.0, -24), std::ldexp(1.0, -11)}) {
+            const Cfg c{best.B, true, tau};
+            rc = apply(c, true);
+            if (!rc) rc = trial(&ms, oc);
+            if (rc) return rc;
+            std::snprintf(buf, sizeof buf, "; tau=2^%d: %.2f ms%s", (int)std::lround(std::log2(tau)), ms,
+                          oc.status ? " (not converged)" : "");
+            log += buf;
+            if (oc.status == MF_OK && oc.worst <= bt->o.tol && ms < best_ms) {
+                best = c;
+                best_ms = ms;
+            }
+        }
+    }
+    rc = apply(best, true);
+    if (!rc) rc = trial(&ms, oc);
+    if (rc) return rc;
+    std::snprintf(buf, sizeof buf, " -> B=%d %s tau=%g, %d workgroups", best.B, best.mixed ? "mixed" : "fp64",
+                  best.tau, bt->G);
+    bt->tuned = log + buf;
+    batch_report(bt, oc, ms, rep);
+    return MF_OK;
 }
 
 int mf_batch_info(mf_batch bt, mf_batch_info_t *info)
@@ -885,6 +1169,9 @@ int mf_batch_info(mf_batch bt, mf_batch_info_t *info)
     info->ms_setup = bt->ms_setup;
     const BatchLayout &Li = bt->mixed ? bt->Lmix : bt->L64;
     for (int c = 0; c < 5; c++) info->nnz_per_code[c] = Li.nnz_code[c];
+    info->mixed = bt->mixed ? 1 : 0;
+    info->tau = bt->tau.empty() ? 0.0 : bt->tau[0];
+    std::snprintf(info->tuned, sizeof info->tuned, "%s", bt->tuned.c_str());
     return MF_OK;
 }
 

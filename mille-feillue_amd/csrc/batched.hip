@@ -516,6 +516,20 @@ __global__ void k_add_int(const int *__restrict__ src, int n, int add, int *__re
     if (i < n) dst[i] = src[i] + add;
 }
 
+// Four-byte slab indices hold the matrix column itself: shift every index of
+// one system's slabs (blob at hand = that system's copy) by its first row.
+template <int B>
+__global__ void k_rebase_cols(const int2 *__restrict__ trh, int tilem, uint8_t *__restrict__ blob, unsigned add)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 4 * tilem) return;
+    const int2 h = trh[i];
+    const int c = i & 3, nq = (h.y + 3) >> 2;
+    if (!nq) return;
+    uint32_t *idx = reinterpret_cast<uint32_t *>(blob + (size_t)(unsigned)h.x * 8 + (size_t)nq * B * 4 * cls_width(c));
+    for (int e = 0; e < nq * B * 4; e++) idx[e] += add;
+}
+
 __global__ void k_rebase_trh(const int2 *__restrict__ src, int n, unsigned add8, int2 *__restrict__ dst)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -637,8 +651,16 @@ static int concat_layouts(mf_batch_s *bt, const std::vector<mf_matrix> &mats, bo
         if (m->tilenum)
             k_add_int<<<(m->tilenum + 255) / 256, 256, 0, st>>>(L.stc, m->tilenum, bt->trbase[s],
                                                                out.stc + bt->tbase[s]);
-        if (L.blob_bytes)
+        if (L.blob_bytes) {
             MF_HIP(hipMemcpyAsync(out.blob + base[s], L.blob, L.blob_bytes, hipMemcpyDeviceToDevice, st));
+            if (!L.idx8 && bt->trbase[s]) {
+                const unsigned add = (unsigned)bt->trbase[s] * bt->B;
+                const unsigned nb = (unsigned)((n4 + 255) / 256);
+                if (bt->B == 8) k_rebase_cols<8><<<nb, 256, 0, st>>>(L.trh, m->tilem, out.blob + base[s], add);
+                else if (bt->B == 16) k_rebase_cols<16><<<nb, 256, 0, st>>>(L.trh, m->tilem, out.blob + base[s], add);
+                else k_rebase_cols<32><<<nb, 256, 0, st>>>(L.trh, m->tilem, out.blob + base[s], add);
+            }
+        }
     }
     MF_HIP(hipGetLastError());
     return MF_OK;
@@ -798,21 +820,21 @@ static int batch_build(mf_batch_s *bt)
             if (m) mf_matrix_destroy(m);
     };
     int rc = MF_OK;
-    bool force16 = false;
+    bool wide = false;
     for (int pass = 0; pass < 2; pass++) {
         bool all8 = true;
         for (int s = 0; s < bt->nsys && !rc; s++) {
             if (!mats[s]) rc = mf_matrix_create(bt->n[s], bt->n[s], bt->h_rp[s][bt->n[s]], bt->h_rp[s].data(),
                                                  bt->h_ci[s].data(), bt->h_va[s].data(), nullptr, bt->B, &mats[s]);
-            if (!rc) rc = build_layout(mats[s], mats[s]->L64, 0.0, 0, force16);
-            if (!rc && bt->mixed) rc = build_layout(mats[s], mats[s]->Lmix, bt->tau[s], bt->o.allow_mask, force16);
+            if (!rc) rc = build_layout(mats[s], mats[s]->L64, 0.0, 0, wide);
+            if (!rc && bt->mixed) rc = build_layout(mats[s], mats[s]->Lmix, bt->tau[s], bt->o.allow_mask, wide);
             if (!rc) all8 = all8 && mats[s]->L64.idx8 && (!bt->mixed || mats[s]->Lmix.idx8);
         }
         if (rc) { cleanup(); return rc; }
-        if (all8 || force16) break;
-        force16 = true;
+        if (all8 || wide) break;
+        wide = true;
     }
-    bt->idx8 = !force16;
+    bt->idx8 = !wide;
     bt->tilem.resize(bt->nsys);
     bt->trbase.assign(bt->nsys + 1, 0);
     bt->tbase.assign(bt->nsys + 1, 0);

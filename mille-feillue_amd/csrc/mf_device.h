@@ -18,10 +18,11 @@ namespace mf {
 // slab of ceil(L_c / 4) quads: entry k of row r sits at element
 // ((k / 4) * B + r) * 4 + k % 4, so one lane reads four consecutive entries of
 // its row with one load and the B lanes of a tile row read one contiguous
-// block.  All value quads come first, then the index quads (slot << log2(B) |
-// column in tile; one byte when every tile row has at most B tiles, else
-// two).  Short rows are padded with value 0 / index 0.  Every slab size is a
-// multiple of 16 bytes.  trh[tr * 4 + c] = {slab offset / 8, L_c}.  Class c:
+// block.  All value quads come first, then the index quads: one byte (slot
+// << log2(B) | column in tile) when every tile row has few enough tiles for
+// the slots of a tile row to fit the lanes of its group, else four bytes (the
+// matrix column itself).  Short rows are padded with value 0 / index 0.
+// Every slab size is a multiple of 16 bytes.  trh[tr * 4 + c] = {slab offset / 8, L_c}.  Class c:
 // 0 FP64, 1 FP32, 2 FP16, 3 FP8 E4M3 (OCP).
 struct DevL {
     const int *tile_ptr;
@@ -74,13 +75,14 @@ __device__ __forceinline__ RowCtx row_ctx(const DevL &L, int tr, int r)
     return c;
 }
 
-// Gather-vector index of one slab entry.
+// Gather-vector index of one slab entry (as gather_col, from a row context).
 template <int B, bool IDX8>
 __device__ __forceinline__ int gather_index(uint32_t id, const RowCtx &c, const DevL &L, int lanebase)
 {
+    (void)L;
+    if (!IDX8) return (int)id;
     const int slot = (int)(id >> log2_of<B>()), col = (int)(id & (B - 1));
-    const int tc = IDX8 ? __shfl(c.tc_reg, lanebase + slot, 64) : L.stc[c.t0 + slot];
-    return tc * B + col;
+    return __shfl(c.tc_reg, lanebase + slot, 64) * B + col;
 }
 
 // One quad (four consecutive entries of one lane's row) of values and of
@@ -128,10 +130,10 @@ template <> struct IdxQuad<true> {
     template <int E> __device__ uint32_t get() const { return (a >> (8 * E)) & 0xffu; }
 };
 template <> struct IdxQuad<false> {
-    uint2 a;
-    __device__ void load(const uint8_t *p, size_t q) { a = reinterpret_cast<const uint2 *>(p)[q]; }
-    __device__ void zero() { a = make_uint2(0u, 0u); }
-    template <int E> __device__ uint32_t get() const { return ((E < 2 ? a.x : a.y) >> (16 * (E & 1))) & 0xffffu; }
+    uint4 a;
+    __device__ void load(const uint8_t *p, size_t q) { a = reinterpret_cast<const uint4 *>(p)[q]; }
+    __device__ void zero() { a = make_uint4(0u, 0u, 0u, 0u); }
+    template <int E> __device__ uint32_t get() const { return E == 0 ? a.x : E == 1 ? a.y : E == 2 ? a.z : a.w; }
 };
 
 template <int N, typename F> __device__ __forceinline__ void static_for(F &&f)
@@ -162,17 +164,18 @@ template <typename T> __device__ __forceinline__ T ld_elem(const T *__restrict__
     return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(g) + (idx * (uint32_t)sizeof(T)));
 }
 
-// Element of the gathered vector for one slab index (slot << log2 B | col):
-// tcb is the lane's copy of tile column * B of slot (lane - lanebase) (one-byte
-// index form), fetched from the slot's lane with ds_bpermute; in the two-byte
-// form the tile column comes from stc.
+// Element of the gathered vector for one slab index.  One-byte form (slot <<
+// log2 B | col): tcb is the lane's copy of tile column * B of slot
+// (lane - lanebase), fetched from the slot's lane with ds_bpermute.  Four-byte
+// form: the index is the column.
 template <int B, bool IDX8>
 __device__ __forceinline__ uint32_t gather_col(uint32_t id, int tcb, int t0, const DevL &L, int lanebase)
 {
+    (void)t0;
+    (void)L;
+    if (!IDX8) return id;
     const uint32_t slot = id >> log2_of<B>(), col = id & (B - 1);
-    const int base = IDX8 ? __builtin_amdgcn_ds_bpermute((int)((lanebase + slot) << 2), tcb)
-                          : L.stc[t0 + (int)slot] * B;
-    return (uint32_t)base + col;
+    return (uint32_t)__builtin_amdgcn_ds_bpermute((int)((lanebase + slot) << 2), tcb) + col;
 }
 
 // Tile-row headers staged in LDS: hd[2 q] = {off0, L0, off1, L1} and

@@ -27,7 +27,7 @@ struct Layout {
     int2 *trh = nullptr;
     int *stc = nullptr;
     uint8_t *blob = nullptr;
-    bool idx8 = true;           // one-byte slab indices
+    bool idx8 = true;           // one-byte slab indices (else four-byte matrix columns)
     size_t blob_bytes = 0;
     long long tiles_per_code[5] = {0, 0, 0, 0, 0};
     long long nnz_per_code[5] = {0, 0, 0, 0, 0};
@@ -77,7 +77,7 @@ struct mf_matrix_s {
 namespace mf {
 int build_reference_tiles(mf_matrix m);              // tiles.hip
 int build_layout(mf_matrix m, Layout &L, double tau, int allow_mask,
-                 bool force_idx16 = false);  // tiles.hip
+                 bool force_wide = false);  // tiles.hip
 int encode_cols_device(mf_matrix m, uint8_t *d_out);  // tiles.hip
 int classes_device(mf_matrix m, double tau, int allow_mask, uint8_t *d_codes);  // tiles.hip
 hipStream_t stream();

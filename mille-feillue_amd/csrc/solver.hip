@@ -271,7 +271,11 @@ struct Reducer {
 };
 
 // tile rows whose loads are interleaved inside the persistent kernels' SpMV
-__host__ __device__ constexpr int spmv_rows(int rpt) { return rpt < 4 ? rpt : 4; }
+// (four-byte indices take twice the registers: two rows)
+__host__ __device__ constexpr int spmv_rows(int rpt, bool idx8)
+{
+    return rpt < (idx8 ? 4 : 2) ? rpt : (idx8 ? 4 : 2);
+}
 
 __device__ __forceinline__ bool finite_all(double a, double b) { return isfinite(a) && isfinite(b); }
 
@@ -298,7 +302,7 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
     __shared__ double s_bc[4];
     __shared__ int s_flag;
     Reducer<TPB, 4> R{s_red, s_bc, &s_flag};
-    constexpr int GPB = TPB / B, NR = spmv_rows(RPT);
+    constexpr int GPB = TPB / B, NR = spmv_rows(RPT, IDX8);
     const int tid = threadIdx.x;
     const int chunk = xcd_chunk(blockIdx.x, gridDim.x);
     const int grp = tid / B, lane = tid % B, lanebase = (tid & 63) & ~(B - 1);
@@ -491,7 +495,7 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
     __shared__ double s_bc[4];
     __shared__ int s_flag;
     Reducer<TPB, 4> R{s_red, s_bc, &s_flag};
-    constexpr int GPB = TPB / B, NR = spmv_rows(RPT);
+    constexpr int GPB = TPB / B, NR = spmv_rows(RPT, IDX8);
     const int tid = threadIdx.x;
     const int chunk = xcd_chunk(blockIdx.x, gridDim.x);
     const int grp = tid / B, lane = tid % B, lanebase = (tid & 63) & ~(B - 1);

@@ -539,9 +539,8 @@ __global__ __launch_bounds__(256) void k_slab_pack(const int *__restrict__ tile_
                 case 2: reinterpret_cast<uint16_t *>(vals)[e] = f64_to_f16(v); break;
                 default: vals[e] = f64_to_e4m3(v); break;
                 }
-                const unsigned id = ((unsigned)slot << log2_of<B>()) | bcol[k];
-                if (iw == 1) idx[e] = (uint8_t)id;
-                else reinterpret_cast<uint16_t *>(idx)[e] = (uint16_t)id;
+                if (iw == 1) idx[e] = (uint8_t)(((unsigned)slot << log2_of<B>()) | bcol[k]);
+                else reinterpret_cast<uint32_t *>(idx)[e] = (uint32_t)tile_columnidx[t] * B + bcol[k];
             }
             slot++;
         }
@@ -722,7 +721,7 @@ void Layout::release()
 }
 
 template <int B>
-static int build_layout_B(mf_matrix m, Layout &L, double tau, int allow_mask, bool force_idx16)
+static int build_layout_B(mf_matrix m, Layout &L, double tau, int allow_mask, bool force_wide)
 {
     hipStream_t st = stream();
     L.release();
@@ -750,13 +749,10 @@ static int build_layout_B(mf_matrix m, Layout &L, double tau, int allow_mask, bo
     MF_HIP(hipMemcpyAsync(&hmax, maxnt, sizeof(int), hipMemcpyDeviceToHost, st));
     MF_HIP(hipStreamSynchronize(st));
     // one-byte index: slot and column fit 8 bits and every slot has a lane of
-    // the group to hold its tile column (B = 8: 8 slots, 16: 16, 32: 8)
-    const bool idx8 = !force_idx16 && hmax <= std::min(B, 1 << (8 - log2_of<B>()));
-    if (!idx8 && hmax > (1 << (16 - log2_of<B>()))) {
-        (void)hipFree(codes); (void)hipFree(units); (void)hipFree(stats); (void)hipFree(err); (void)hipFree(maxnt);
-        return fail(MF_E_FORMAT, "more tiles in one tile row than the two-byte slot index holds");
-    }
-    const int iw = idx8 ? 1 : 2;
+    // the group to hold its tile column (B = 8: 8 slots, 16: 16, 32: 8);
+    // otherwise a four-byte index holds the matrix column itself
+    const bool idx8 = !force_wide && hmax <= std::min(B, 1 << (8 - log2_of<B>()));
+    const int iw = idx8 ? 1 : 4;
     k_slab_sizes<B><<<nbr, 256, 0, st>>>(m->d_tile_ptr, tilem, m->d_tile_start, m->d_rowtmp, codes, iw,
                                          units, stats);
     int rc = scan_exclusive<unsigned long long>(units, units, tilem + 1, st);
@@ -815,14 +811,14 @@ static int build_layout_B(mf_matrix m, Layout &L, double tau, int allow_mask, bo
     return MF_OK;
 }
 
-int build_layout(mf_matrix m, Layout &L, double tau, int allow_mask, bool force_idx16)
+int build_layout(mf_matrix m, Layout &L, double tau, int allow_mask, bool force_wide)
 {
     double t0 = now_ms();
     int rc;
     switch (m->B) {
-    case 8: rc = build_layout_B<8>(m, L, tau, allow_mask, force_idx16); break;
-    case 16: rc = build_layout_B<16>(m, L, tau, allow_mask, force_idx16); break;
-    case 32: rc = build_layout_B<32>(m, L, tau, allow_mask, force_idx16); break;
+    case 8: rc = build_layout_B<8>(m, L, tau, allow_mask, force_wide); break;
+    case 16: rc = build_layout_B<16>(m, L, tau, allow_mask, force_wide); break;
+    case 32: rc = build_layout_B<32>(m, L, tau, allow_mask, force_wide); break;
     default: return fail(MF_E_ARG, "tile size");
     }
     m->ms_layout += now_ms() - t0;

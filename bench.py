@@ -9,9 +9,10 @@ x resident in HBM (device buffers owned by torch, passed to mf_solve_device).
 Reported: time to tolerance of the per-tile mixed-precision solve (the value),
 the same persistent kernel run FP64-only, their ratio, the standalone tiled
 SpMV kernel's bandwidth, the roofline of the dominant kernel (the persistent
-CG kernel), and the oracle's tile-SpMV CG timed on the host cores.
+CG kernel), the batched solve of 512 systems x 8 right-hand sides, and the
+oracle's tile-SpMV CG timed on the host cores.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batched]
 """
 import argparse
 import json
@@ -56,6 +57,45 @@ def cpu_baseline(rp, ci, va, b, gpu_iters):
     }
 
 
+def batched_run(mf, nsys=512, nrhs=8):
+    """512 independent SPD systems, n log-uniform in [1k, 64k] (library
+    stream, seed 0), alternating 2D 5-point and 3D 7-point stencils with
+    lognormal(sigma = 1.5) cell coefficients (grids padded up to n), 8
+    right-hand sides N(0, 1) each; one graph-captured persistent launch,
+    workgroups and tile size picked by the autotuner."""
+    t0 = time.perf_counter()
+    u = mf.gen_uniform(0, nsys, np.log(1000.0), np.log(64000.0))
+    sizes = np.exp(u).astype(int)
+    systems = [mf.gen_stencil_lognormal(2 if i % 2 == 0 else 3, int(sizes[i]), 1.5, 1000 + i) for i in range(nsys)]
+    rng = np.random.default_rng(0)
+    b = [rng.standard_normal((nrhs, len(s[0]) - 1)) for s in systems]
+    bt = mf.Batch(systems, nrhs=nrhs, tol=TOL)
+    t1 = time.perf_counter()
+    bt.tune(b)
+    t2 = time.perf_counter()
+    info = bt.info()
+    reps = []
+    for _ in range(2):
+        xs, it, rel, rep = bt.solve(b)
+        if rep["status"] != 0 or rel.max() > TOL:
+            raise RuntimeError(f"batched solve did not reach tolerance: {rep}")
+        reps.append(rep["ms_solve"])
+    ms = min(reps)
+    n = np.array([len(s[0]) - 1 for s in systems], dtype=np.float64)
+    # algorithmic bytes: each system's share (by rows) of the per-iteration
+    # bytes (matrix once + 16 n per column) times its own iterations
+    alg = float(np.sum(info["bytes_alg"] * n / n.sum() * it))
+    bt.close()
+    log(f"batched: {nsys} systems x {nrhs} rhs, setup {t1 - t0:.1f}s, tune {t2 - t1:.1f}s ({info['tuned']}), "
+        f"solve {ms:.1f} ms, iterations max {it.max()} mean {it.mean():.0f}")
+    return {"ms": round(ms, 2), "systems": nsys, "rhs_per_system": nrhs, "rows": int(n.sum()),
+            "nnz": int(sum(len(s[1]) for s in systems)), "iterations_max": int(it.max()),
+            "iterations_mean": round(float(it.mean()), 1), "relres_max": float(rel.max()),
+            "B": info["B"], "workgroups": info["G"], "tuned": info["tuned"],
+            "bytes_alg_total": alg, "GBps": round(alg / (ms * 1e-3) / 1e9, 1),
+            "frac": round(alg / (ms * 1e-3) / HBM_PEAK, 4), "tile_precision": "fp64"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -64,6 +104,7 @@ def main():
     ap.add_argument("--grid", type=int, default=128)
     ap.add_argument("--rr", type=float, default=1e-4, help="residual replacement factor (mixed)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched (512 systems x 8 RHS) measurement")
     ap.add_argument("--calib", action="store_true",
                     help="also run a plain 512 MiB device copy (known bytes) to calibrate PMC byte counters")
     args = ap.parse_args()
@@ -170,6 +211,8 @@ def main():
             "traffic_source": "profiles/k_cg_poisson128_pmc.json (rocprofv3 FETCH_SIZE x2.000 + WRITE_SIZE x1.000)"
             if traffic is not None else None}
 
+    batched = None if args.no_batched else batched_run(mf)
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
@@ -210,6 +253,7 @@ def main():
         },
         "solver_GBps": {"fp64": round(it64 * (b64 + 56.0 * n) / ms64 / 1e6, 1), "mixed": round(ach / 1e9, 1)},
         "roofline": roof,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -39,7 +39,7 @@ namespace mf {
 // 256 threads = 256/B lane groups, lane = row inside the tile row.  A block
 // covers SPMV_NR * 256/B consecutive tile rows; lane group q handles tile
 // rows base + k * 256/B + q (k < SPMV_NR) together.
-constexpr int SPMV_NR = 4;
+constexpr int SPMV_NR = 2;
 template <int B, bool IDX8>
 __global__ __launch_bounds__(256) void k_spmv(DevL L, int tilem, const double *__restrict__ g64,
                                               const float *__restrict__ g32, double *__restrict__ y)

@@ -150,8 +150,11 @@ typedef struct {
     int allow_mask;     /* mixed: enabled codes (default FP32|FP16|FP8) */
     double rr_factor;   /* residual replacement when ||r|| drops by this factor
                            since the last replacement (0: only at convergence) */
-    int ir_max_outer;   /* FP64 iterative-refinement steps (default 8) */
-    double ir_inner_tol;/* relative tolerance of each inner solve when tiles are lossy */
+    int ir_max_outer;   /* lossy tiles: 0 (default) = one launch whose true-residual
+                           passes use the FP64 layout (reliable updates against A);
+                           > 0 = at most this many FP64 refinement steps around
+                           inner solves on the lossy layout */
+    double ir_inner_tol;/* relative tolerance of each inner solve of the refinement */
     int grid;           /* workgroups (0: one per CU) */
     int block;          /* threads per workgroup (0: auto, 512 or 1024) */
     long long spin_limit; /* grid-barrier polls before MF_E_TIMEOUT */

@@ -98,7 +98,7 @@ void mf_opts_default(mf_opts *o)
     o->tau = 0.0;
     o->allow_mask = (1 << 2) | (1 << 3) | (1 << 4);
     o->rr_factor = 0.0;
-    o->ir_max_outer = 8;
+    o->ir_max_outer = 0;
     o->ir_inner_tol = 1e-6;
     o->grid = 0;
     o->block = 0;

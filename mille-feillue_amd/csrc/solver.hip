@@ -111,6 +111,8 @@ constexpr int SYNC_K = 4;  // values per grid allreduce, at most
 
 struct SolveArgs {
     DevL L;
+    DevL L64;          // FP64 layout for the true-residual passes of a lossy layout
+    int use64;         // 1: passes use L64 (headers read from L64), else L
     int tilem, G, trw;
     const double *b;
     double *x;
@@ -394,9 +396,12 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
                     const double *g[2] = {A.h64, A.q64};
                     double acc[2] = {0.0, 0.0};
                     const int q = i * GPB + grp;
-                    tile_row_spmv_f64_c<B, IDX8, 2>(
-                        A.L, row_ctx_from<IDX8>(A.L, s_t0[q], s_t0[q + 1] - s_t0[q], s_h[2 * q], s_h[2 * q + 1], lane),
-                        lane, lanebase, g, acc);
+                    if (A.use64)
+                        tile_row_spmv_f64_c<B, IDX8, 2>(A.L64, row_ctx<B, IDX8>(A.L64, tr, lane), lane, lanebase, g, acc);
+                    else
+                        tile_row_spmv_f64_c<B, IDX8, 2>(
+                            A.L, row_ctx_from<IDX8>(A.L, s_t0[q], s_t0[q + 1] - s_t0[q], s_h[2 * q], s_h[2 * q + 1], lane),
+                            lane, lanebase, g, acc);
                     const size_t row = (size_t)tr * B + lane;
                     const double rv = A.b[row] - acc[0];
                     s_r[i * TPB + tid] = rv;
@@ -623,9 +628,12 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
                     const double *g[1] = {A.h64};
                     double acc[1] = {0.0};
                     const int q = i * GPB + grp;
-                    tile_row_spmv_f64_c<B, IDX8, 1>(
-                        A.L, row_ctx_from<IDX8>(A.L, s_t0[q], s_t0[q + 1] - s_t0[q], s_h[2 * q], s_h[2 * q + 1], lane),
-                        lane, lanebase, g, acc);
+                    if (A.use64)
+                        tile_row_spmv_f64_c<B, IDX8, 1>(A.L64, row_ctx<B, IDX8>(A.L64, tr, lane), lane, lanebase, g, acc);
+                    else
+                        tile_row_spmv_f64_c<B, IDX8, 1>(
+                            A.L, row_ctx_from<IDX8>(A.L, s_t0[q], s_t0[q + 1] - s_t0[q], s_h[2 * q], s_h[2 * q + 1], lane),
+                            lane, lanebase, g, acc);
                     s_t[i * TPB + tid] = A.b[(size_t)tr * B + lane] - acc[0];
                 }
             }
@@ -748,8 +756,8 @@ __global__ void k_reset_ctl(unsigned long long *slots, int nslots, int *ctl)
 }
 
 // One persistent launch: solves L x = b (x overwritten) on the stream.
-static int launch_inner(mf_matrix m, const Layout &L, int method, const double *d_b, double *d_x,
-                        double tol, const mf_opts &o, bool sync, InnerResult *res, int *grid_out,
+static int launch_inner(mf_matrix m, const Layout &L, const Layout *L64, int method, const double *d_b,
+                        double *d_x, double tol, const mf_opts &o, bool sync, InnerResult *res, int *grid_out,
                         int *block_out, int *rpt_out)
 {
     hipStream_t st = stream();
@@ -766,6 +774,9 @@ static int launch_inner(mf_matrix m, const Layout &L, int method, const double *
     Workspace &w = m->ws;
     SolveArgs A;
     A.L = L.view(m->d_tile_ptr);
+    A.use64 = L64 && L64 != &L ? 1 : 0;
+    A.L64 = A.use64 ? L64->view(m->d_tile_ptr) : A.L;
+    if (A.use64 && L64->idx8 != L.idx8) return fail(MF_E_FORMAT, "layouts with different index forms");
     A.tilem = m->tilem;
     A.G = G;
     A.trw = trw;
@@ -897,7 +908,8 @@ int solve_device(mf_matrix m, const double *d_b, double *d_x, const mf_opts &o, 
     rep->ms_setup = now_ms() - t0;
     const Layout &L = mixed ? m->Lmix : m->L64;
     // lossless layout: the stored operator is A itself, so the kernel's own
-    // true-residual test is final; otherwise refine in FP64 around it
+    // true-residual test is final; a lossy layout either runs its passes on
+    // the FP64 layout (ir_max_outer = 0) or is refined in FP64 around it
     const bool exact = !mixed || o.tau == 0.0;
     rc = ensure_ws(m, std::max(1, o.grid > 0 ? o.grid : device_cu_count()));
     if (rc) return rc;
@@ -911,8 +923,11 @@ int solve_device(mf_matrix m, const double *d_b, double *d_x, const mf_opts &o, 
     int G = 0, TPB = 0, RPT = 0;
     int status = MF_OK, iters = 0, repl = 0, outer = 0;
     double relres = -1.0;
-    if (exact) {
-        rc = launch_inner(m, L, o.method, d_b, d_x, o.tol, o, true, &ir, &G, &TPB, &RPT);
+    if (exact || o.ir_max_outer <= 0) {
+        // lossless: the kernel's own passes are final; lossy without outer
+        // refinement: the passes run on the FP64 layout (reliable updates
+        // against A itself) inside the one launch
+        rc = launch_inner(m, L, exact ? nullptr : &m->L64, o.method, d_b, d_x, o.tol, o, true, &ir, &G, &TPB, &RPT);
         if (rc) return rc;
         status = ir.status;
         iters = ir.iterations;
@@ -931,7 +946,7 @@ int solve_device(mf_matrix m, const double *d_b, double *d_x, const mf_opts &o, 
         status = MF_E_MAXIT;
         for (outer = 0; outer < o.ir_max_outer && rnorm > o.tol * bnorm; ++outer) {
             const double inner_tol = std::max(o.tol * bnorm / rnorm, o.ir_inner_tol);
-            rc = launch_inner(m, L, o.method, w.r_outer, w.d, inner_tol, o, true, &ir, &G, &TPB, &RPT);
+            rc = launch_inner(m, L, nullptr, o.method, w.r_outer, w.d, inner_tol, o, true, &ir, &G, &TPB, &RPT);
             if (rc) return rc;
             iters += ir.iterations;
             repl += ir.replacements;

@@ -78,6 +78,19 @@ def _cg_case(gpu, orc, rp, ci, va, b, **opts):
     return rep
 
 
+def _cg_case_loose(gpu, orc, rp, ci, va, b, **opts):
+    # lossy tiles change the iterations; the answer must still be the FP64 one
+    A = gpu.Matrix(rp, ci, va)
+    x, rep = A.solve(b, method=gpu.MF_CG, tol=TOL, **opts)
+    xo, orep = orc.cg(rp, ci, va, b, TOL, 20000)
+    print(f"cg lossy n={len(b)} gpu it={rep['iterations']} repl={rep['replacements']} oracle it={orep.iterations} "
+          f"relres={rep['relres_true']:.2e}", flush=True)
+    assert rep["status"] == 0
+    assert rep["relres_true"] <= TOL and _true_relres(rp, ci, va, x, b) <= 1.5 * TOL
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-6
+    return rep
+
+
 def test_gpu_cg_fp64_small_configs(gpu, orc):
     rp, ci, va = gpu.gen_laplace2d(256, 256)
     _cg_case(gpu, orc, rp, ci, va, orc.csr_spmv(rp, ci, va, np.ones(len(rp) - 1)))
@@ -152,13 +165,23 @@ def test_gpu_bicgstab_fp64(gpu, orc):
         _bicg_case(gpu, orc, g["rowptr"], g["col"], g["val"], b)
 
 
-@pytest.mark.parametrize("tau", [2.0 ** -24, 2.0 ** -11])
-def test_gpu_bicgstab_mixed_refinement(gpu, orc, tau):
+@pytest.mark.parametrize("tau,outer", [(2.0 ** -24, 0), (2.0 ** -11, 0), (2.0 ** -11, 8)])
+def test_gpu_bicgstab_mixed_lossy(gpu, orc, tau, outer):
+    # lossy tiles: reliable updates on the FP64 layout inside the launch
+    # (outer = 0) or FP64 refinement around inner solves (outer = 8)
     rp, ci, va = gpu.gen_convdiff3d(24, 10.0, 1.0, 0)
     b = orc.csr_spmv(rp, ci, va, gpu.gen_uniform(0, len(rp) - 1, -1, 1))
     rep = _bicg_case(gpu, orc, rp, ci, va, b, check_iters=False, precision=gpu.MF_MIXED, tau=tau,
-                     rr_factor=1e-2)
+                     rr_factor=1e-2, ir_max_outer=outer)
     assert rep["outer"] >= 1
+
+
+@pytest.mark.parametrize("tau", [2.0 ** -24, 2.0 ** -11])
+def test_gpu_cg_mixed_lossy_reliable_updates(gpu, orc, tau):
+    rp, ci, va = gpu.gen_stencil_lognormal(3, 30000, 1.0, 3)
+    b = orc.csr_spmv(rp, ci, va, gpu.gen_uniform(1, len(rp) - 1, -1, 1))
+    rep = _cg_case_loose(gpu, orc, rp, ci, va, b, precision=gpu.MF_MIXED, tau=tau, rr_factor=1e-2)
+    assert rep["outer"] == 1
 
 
 def test_gpu_solver_edge_cases(gpu, orc):

@@ -135,8 +135,17 @@ typedef struct {
     double bytes_alg_fp64;        /* algorithmic bytes of one FP64 SpMV */
     double bytes_alg_mixed;       /* same for the last mixed layout built */
     double bytes_layout_mixed;    /* actual bytes of the mixed device layout */
+    int index_width;              /* slab index bytes of that layout (1, 2 or 4) */
 } mf_matrix_info;
 int mf_matrix_get_info(mf_matrix m, mf_matrix_info *info);
+
+/* Narrowest slab index width (bytes) of this matrix's compute layouts:
+ * 1 (default) = one-byte slot form where every tile row fits it, else the
+ * two-byte tile-row-relative form where every tile lies within 32768/B - 1
+ * tile columns of the diagonal, else the four-byte column; 2 or 4 skip the
+ * narrower forms.  Layouts already built are released.  No reference
+ * counterpart (the reference stores four-byte columns only). */
+int mf_matrix_set_index_width(mf_matrix m, int width);
 
 /* Precision / solver options. */
 enum { MF_CG = 0, MF_BICGSTAB = 1 };
@@ -158,6 +167,8 @@ typedef struct {
     int grid;           /* workgroups (0: one per CU) */
     int block;          /* threads per workgroup (0: auto, 512 or 1024) */
     long long spin_limit; /* grid-barrier polls before MF_E_TIMEOUT */
+    int index_width;    /* narrowest slab index width, as mf_matrix_set_index_width
+                           (0 = that of the matrix; batched: 0 = 1) */
 } mf_opts;
 void mf_opts_default(mf_opts *o);
 
@@ -205,6 +216,7 @@ typedef struct {
     int mixed;                     /* inner layout has per-tile classes */
     double tau;                    /* their rounding threshold */
     char tuned[512];               /* the autotuner's trials and choice */
+    int index_width;               /* slab index bytes shared by the systems (1, 2, 4) */
 } mf_batch_info_t;
 int mf_batch_create(int nsys, const int *n, const int *const *rowptr, const int *const *colidx,
                     const double *const *val, int nrhs, const mf_opts *o, mf_batch *out);

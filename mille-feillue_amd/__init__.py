@@ -46,14 +46,14 @@ class Info(C.Structure):
                 ("tilem", C.c_int), ("tilen", C.c_int), ("tilenum", C.c_int),
                 ("tiles_per_code", C.c_longlong * 5), ("nnz_per_code", C.c_longlong * 5),
                 ("bytes_alg_fp64", C.c_double), ("bytes_alg_mixed", C.c_double),
-                ("bytes_layout_mixed", C.c_double)]
+                ("bytes_layout_mixed", C.c_double), ("index_width", C.c_int)]
 
 
 class Opts(C.Structure):
     _fields_ = [("method", C.c_int), ("precision", C.c_int), ("tol", C.c_double), ("maxit", C.c_int),
                 ("tau", C.c_double), ("allow_mask", C.c_int), ("rr_factor", C.c_double),
                 ("ir_max_outer", C.c_int), ("ir_inner_tol", C.c_double), ("grid", C.c_int),
-                ("block", C.c_int), ("spin_limit", C.c_longlong)]
+                ("block", C.c_int), ("spin_limit", C.c_longlong), ("index_width", C.c_int)]
 
 
 class Report(C.Structure):
@@ -84,7 +84,7 @@ EXPORTS = (
     "Tile_create", "Tile_destroy", "cg_solve_inc", "bicgstab", "mf_status_str", "mf_last_error",
     "mf_init", "mf_matrix_create", "mf_matrix_destroy", "mf_matrix_export", "mf_ref_tiles_free",
     "mf_matrix_encode_cols", "mf_matrix_classes", "mf_matrix_get_info", "mf_opts_default",
-    "mf_solve", "mf_solve_device", "mf_matrix_layout", "mf_spmv", "mf_spmv_bench",
+    "mf_solve", "mf_solve_device", "mf_matrix_layout", "mf_matrix_set_index_width", "mf_spmv", "mf_spmv_bench",
     "mf_solve_batched", "mf_mc_classify", "mf_compact_rows", "mf_partition_build", "mf_free",
     "mf_gen_laplace2d", "mf_gen_poisson3d", "mf_gen_convdiff3d", "mf_gen_random_spd",
     "mf_gen_stencil_lognormal", "mf_gen_uniform", "mf_gen_normal",
@@ -112,6 +112,7 @@ def lib():
     L.mf_matrix_classes.argtypes = [C.c_void_p, C.c_double, C.c_int, _u8]
     L.mf_matrix_get_info.argtypes = [C.c_void_p, C.POINTER(Info)]
     L.mf_matrix_layout.argtypes = [C.c_void_p, C.c_double, C.c_int]
+    L.mf_matrix_set_index_width.argtypes = [C.c_void_p, C.c_int]
     L.mf_spmv.argtypes = [C.c_void_p, C.c_int, _d, _d]
     L.mf_spmv_bench.argtypes = [C.c_void_p, C.c_int, C.c_int, _d]
     L.mf_opts_default.argtypes = [C.POINTER(Opts)]
@@ -194,7 +195,7 @@ class BatchInfo(C.Structure):
     _fields_ = [("nsys", C.c_int), ("nrhs", C.c_int), ("B", C.c_int), ("G", C.c_int),
                 ("rows", C.c_longlong), ("tilenum", C.c_longlong), ("bytes_alg", C.c_double),
                 ("ms_setup", C.c_double), ("nnz_per_code", C.c_longlong * 5), ("mixed", C.c_int),
-                ("tau", C.c_double), ("tuned", C.c_char * 512)]
+                ("tau", C.c_double), ("tuned", C.c_char * 512), ("index_width", C.c_int)]
 
 
 class Batch:
@@ -336,6 +337,10 @@ class Matrix:
 
     def layout(self, tau=0.0, allow_mask=ALLOW_ALL_LOW):
         _check(lib().mf_matrix_layout(self.h, tau, allow_mask))
+
+    def set_index_width(self, width):
+        """Narrowest slab index width (1, 2 or 4 bytes) of the compute layouts."""
+        _check(lib().mf_matrix_set_index_width(self.h, width))
 
     def spmv(self, x, mixed=False):
         info = self.info()

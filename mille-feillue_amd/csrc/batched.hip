@@ -170,7 +170,7 @@ __device__ __forceinline__ void ldK(const float *__restrict__ g, uint32_t col, f
 }
 
 // One class slab of this lane's row applied to K columns.
-template <int B, bool IDX8, int CLS, int K, typename AccT, typename GT>
+template <int B, int IW, int CLS, int K, typename AccT, typename GT>
 __device__ __forceinline__ void slabK(const DevL &L, int2 h, int tcr, int t0, int r, int lanebase,
                                       const GT *__restrict__ g, AccT (&acc)[K])
 {
@@ -179,11 +179,11 @@ __device__ __forceinline__ void slabK(const DevL &L, int2 h, int tcr, int t0, in
     const uint8_t *idx = vals + (size_t)nq * B * 4 * cls_width(CLS);
     for (int j = 0; j < nq; j++) {
         Quad<CLS> v;
-        IdxQuad<IDX8> id;
+        IdxQuad<IW> id;
         v.load(vals, (size_t)j * B + r);
         id.load(idx, (size_t)j * B + r);
         GT xg[4][K];
-        static_for<4>([&](auto e) { ldK<K>(g, gather_col<B, IDX8>(id.template get<e>(), tcr, t0, L, lanebase), xg[e]); });
+        static_for<4>([&](auto e) { ldK<K>(g, gather_col<B, IW>(id.template get<e>(), tcr, t0, L, lanebase), xg[e]); });
         static_for<4>([&](auto e) {
             const AccT a = (AccT)v.template get<e>();
 #pragma unroll
@@ -192,22 +192,23 @@ __device__ __forceinline__ void slabK(const DevL &L, int2 h, int tcr, int t0, in
     }
 }
 
-template <bool IDX8, int B>
-__device__ __forceinline__ int row_tcr(const DevL &L, int t0, int nt, int r)
+// index base of tile row tr (as load_tcr)
+template <int IW, int B>
+__device__ __forceinline__ int row_tcr(const DevL &L, int tr, int t0, int nt, int r)
 {
-    return (IDX8 && r < nt) ? L.stc[t0 + r] * B : 0;
+    return IW == 1 ? (r < nt ? L.stc[t0 + r] * B : 0) : IW == 2 ? tr * B : 0;
 }
 
 // y = A_row g for K columns: FP64 tiles in FP64 over g64, the other classes
 // in FP32 over g32, as the single-vector SpMV.
-template <int B, bool IDX8, int K>
+template <int B, int IW, int K>
 __device__ __forceinline__ void row_spmvK(const DevL &L, int tr, int r, int lanebase, const double *__restrict__ g64,
                                           const float *__restrict__ g32, double (&y)[K])
 {
     const int4 *h = reinterpret_cast<const int4 *>(L.trh + (size_t)tr * 4);
     const int4 h01 = h[0], h23 = h[1];
     const int t0 = L.tile_ptr[tr], nt = L.tile_ptr[tr + 1] - t0;
-    const int tcr = row_tcr<IDX8, B>(L, t0, nt, r);
+    const int tcr = row_tcr<IW, B>(L, tr, t0, nt, r);
     double a64[K];
     float a32[K];
 #pragma unroll
@@ -215,22 +216,22 @@ __device__ __forceinline__ void row_spmvK(const DevL &L, int tr, int r, int lane
         a64[k] = 0.0;
         a32[k] = 0.0f;
     }
-    if (h01.y) slabK<B, IDX8, 0, K>(L, make_int2(h01.x, h01.y), tcr, t0, r, lanebase, g64, a64);
-    if (h01.w) slabK<B, IDX8, 1, K>(L, make_int2(h01.z, h01.w), tcr, t0, r, lanebase, g32, a32);
-    if (h23.y) slabK<B, IDX8, 2, K>(L, make_int2(h23.x, h23.y), tcr, t0, r, lanebase, g32, a32);
-    if (h23.w) slabK<B, IDX8, 3, K>(L, make_int2(h23.z, h23.w), tcr, t0, r, lanebase, g32, a32);
+    if (h01.y) slabK<B, IW, 0, K>(L, make_int2(h01.x, h01.y), tcr, t0, r, lanebase, g64, a64);
+    if (h01.w) slabK<B, IW, 1, K>(L, make_int2(h01.z, h01.w), tcr, t0, r, lanebase, g32, a32);
+    if (h23.y) slabK<B, IW, 2, K>(L, make_int2(h23.x, h23.y), tcr, t0, r, lanebase, g32, a32);
+    if (h23.w) slabK<B, IW, 3, K>(L, make_int2(h23.z, h23.w), tcr, t0, r, lanebase, g32, a32);
 #pragma unroll
     for (int k = 0; k < K; k++) y[k] = a64[k] + (double)a32[k];
 }
 
 // A x and A p on the FP64 layout (class 0 only) for K columns each.
-template <int B, bool IDX8, int K>
+template <int B, int IW, int K>
 __device__ __forceinline__ void row_spmv64_xp(const DevL &L, int tr, int r, int lanebase, const double *__restrict__ gx,
                                               const double *__restrict__ gp, double (&ax)[K], double (&ap)[K])
 {
     const int2 h = reinterpret_cast<const int2 *>(L.trh)[(size_t)tr * 4];
     const int t0 = L.tile_ptr[tr], nt = L.tile_ptr[tr + 1] - t0;
-    const int tcr = row_tcr<IDX8, B>(L, t0, nt, r);
+    const int tcr = row_tcr<IW, B>(L, tr, t0, nt, r);
 #pragma unroll
     for (int k = 0; k < K; k++) ax[k] = ap[k] = 0.0;
     const int nq = (h.y + 3) >> 2;
@@ -238,11 +239,11 @@ __device__ __forceinline__ void row_spmv64_xp(const DevL &L, int tr, int r, int 
     const uint8_t *idx = vals + (size_t)nq * B * 4 * 8;
     for (int j = 0; j < nq; j++) {
         Quad<0> v;
-        IdxQuad<IDX8> id;
+        IdxQuad<IW> id;
         v.load(vals, (size_t)j * B + r);
         id.load(idx, (size_t)j * B + r);
         static_for<4>([&](auto e) {
-            const uint32_t c = gather_col<B, IDX8>(id.template get<e>(), tcr, t0, L, lanebase);
+            const uint32_t c = gather_col<B, IW>(id.template get<e>(), tcr, t0, L, lanebase);
             double vx[K], vp[K];
             ldK<K>(gx, c, vx);
             ldK<K>(gp, c, vp);
@@ -273,7 +274,7 @@ struct BatchShared {
 
 // CG on one system (tile rows tr0 .. tr1 - 1 of it in this workgroup);
 // returns false when the group sync gave up (abort word raised).
-template <int B, bool IDX8, int K, int TPB>
+template <int B, int IW, int K, int TPB>
 __device__ bool cg_system(const BatchArgs &A, BatchShared<K, TPB> &sh, int sys, int tr0, int tr1, int gw0, int gn)
 {
     constexpr int GPB = TPB / B;
@@ -347,7 +348,7 @@ __device__ bool cg_system(const BatchArgs &A, BatchShared<K, TPB> &sh, int sys, 
             for (int tr = tr0 + grp; tr < tr1; tr += GPB) {
                 const size_t row = (size_t)tr * B + lane;
                 double ax[K], ap[K];
-                row_spmv64_xp<B, IDX8, K>(A.L64, tr, lane, lanebase, A.x, A.p, ax, ap);
+                row_spmv64_xp<B, IW, K>(A.L64, tr, lane, lanebase, A.x, A.p, ax, ap);
 #pragma unroll
                 for (int k = 0; k < K; k++) {
                     const size_t e = row * K + k;
@@ -405,7 +406,7 @@ __device__ bool cg_system(const BatchArgs &A, BatchShared<K, TPB> &sh, int sys, 
         for (int tr = tr0 + grp; tr < tr1; tr += GPB) {
             const size_t row = (size_t)tr * B + lane;
             double w[K];
-            row_spmvK<B, IDX8, K>(A.L, tr, lane, lanebase, A.r, A.r32, w);
+            row_spmvK<B, IW, K>(A.L, tr, lane, lanebase, A.r, A.r32, w);
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 const size_t e = row * K + k;
@@ -491,13 +492,13 @@ __device__ bool cg_system(const BatchArgs &A, BatchShared<K, TPB> &sh, int sys, 
     }
 }
 
-template <int B, bool IDX8, int K, int TPB>
+template <int B, int IW, int K, int TPB>
 __global__ __launch_bounds__(TPB) void k_cg_batched(BatchArgs A)
 {
     __shared__ BatchShared<K, TPB> sh;
     for (int t = A.wg_task[blockIdx.x]; t < A.wg_task[blockIdx.x + 1]; t++) {
         const int *q = A.task + 5 * (size_t)t;
-        if (!cg_system<B, IDX8, K, TPB>(A, sh, q[0], q[1], q[2], q[3], q[4])) return;
+        if (!cg_system<B, IW, K, TPB>(A, sh, q[0], q[1], q[2], q[3], q[4])) return;
         __syncthreads();  // LDS scalars are reused by the next system
     }
 }
@@ -562,7 +563,8 @@ struct BatchLayout {
 
 struct mf_batch_s {
     int nsys = 0, nrhs = 0, K = 0, B = 16, TPB = 256;
-    bool mixed = false, idx8 = true;
+    bool mixed = false;
+    int iw = 1;  // slab index width shared by every system
     mf_opts o;
     std::vector<int> n, tilem, trbase, nnz;
     std::vector<std::vector<int>> h_rp, h_ci;  // host copies of the systems (rebuilds)
@@ -608,19 +610,24 @@ static void batch_release(mf_batch_s *bt)
 
 typedef const void *KernelFn;
 
-template <int B, bool IDX8>
+template <int B, int IW>
 static KernelFn batched_fn(int K)
 {
-    if (K == 1) return (KernelFn)k_cg_batched<B, IDX8, 1, 256>;
-    return (KernelFn)k_cg_batched<B, IDX8, 8, 256>;
+    if (K == 1) return (KernelFn)k_cg_batched<B, IW, 1, 256>;
+    return (KernelFn)k_cg_batched<B, IW, 8, 256>;
 }
 
-static KernelFn pick_batched(int B, bool idx8, int K)
+template <int B> static KernelFn batched_fn_iw(int iw, int K)
+{
+    return iw == 1 ? batched_fn<B, 1>(K) : iw == 2 ? batched_fn<B, 2>(K) : batched_fn<B, 4>(K);
+}
+
+static KernelFn pick_batched(int B, int iw, int K)
 {
     switch (B) {
-    case 8: return idx8 ? batched_fn<8, true>(K) : batched_fn<8, false>(K);
-    case 16: return idx8 ? batched_fn<16, true>(K) : batched_fn<16, false>(K);
-    default: return idx8 ? batched_fn<32, true>(K) : batched_fn<32, false>(K);
+    case 8: return batched_fn_iw<8>(iw, K);
+    case 16: return batched_fn_iw<16>(iw, K);
+    default: return batched_fn_iw<32>(iw, K);
     }
 }
 
@@ -653,7 +660,9 @@ static int concat_layouts(mf_batch_s *bt, const std::vector<mf_matrix> &mats, bo
                                                                out.stc + bt->tbase[s]);
         if (L.blob_bytes) {
             MF_HIP(hipMemcpyAsync(out.blob + base[s], L.blob, L.blob_bytes, hipMemcpyDeviceToDevice, st));
-            if (!L.idx8 && bt->trbase[s]) {
+            // four-byte indices are matrix columns: shift them by the system's
+            // first row (the one- and two-byte forms are tile-row relative)
+            if (L.iw == 4 && bt->trbase[s]) {
                 const unsigned add = (unsigned)bt->trbase[s] * bt->B;
                 const unsigned nb = (unsigned)((n4 + 255) / 256);
                 if (bt->B == 8) k_rebase_cols<8><<<nb, 256, 0, st>>>(L.trh, m->tilem, out.blob + base[s], add);
@@ -769,7 +778,7 @@ static int batch_capture(mf_batch_s *bt)
         MF_HIP(hipMemset(bt->d_prof, 0, sizeof(unsigned long long) * 4 * bt->G));
         A.prof = bt->d_prof;
     }
-    bt->fn = pick_batched(bt->B, bt->idx8, bt->K);
+    bt->fn = pick_batched(bt->B, bt->iw, bt->K);
     int per_cu = 0;
     MF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bt->fn, bt->TPB, 0));
     if ((long long)per_cu * device_cu_count() < bt->G)
@@ -820,21 +829,29 @@ static int batch_build(mf_batch_s *bt)
             if (m) mf_matrix_destroy(m);
     };
     int rc = MF_OK;
-    bool wide = false;
-    for (int pass = 0; pass < 2; pass++) {
-        bool all8 = true;
+    // every system takes the narrowest index form that all of them fit: build
+    // at the narrowest width each allows, then rebuild at the widest width
+    // found until they agree (the width only grows: at most three passes)
+    int iw = std::max(1, bt->o.index_width), got = 0;
+    for (int pass = 0; pass < 3 && !got; pass++) {
+        int lo = 4, hi = 1;
         for (int s = 0; s < bt->nsys && !rc; s++) {
             if (!mats[s]) rc = mf_matrix_create(bt->n[s], bt->n[s], bt->h_rp[s][bt->n[s]], bt->h_rp[s].data(),
                                                  bt->h_ci[s].data(), bt->h_va[s].data(), nullptr, bt->B, &mats[s]);
-            if (!rc) rc = build_layout(mats[s], mats[s]->L64, 0.0, 0, wide);
-            if (!rc && bt->mixed) rc = build_layout(mats[s], mats[s]->Lmix, bt->tau[s], bt->o.allow_mask, wide);
-            if (!rc) all8 = all8 && mats[s]->L64.idx8 && (!bt->mixed || mats[s]->Lmix.idx8);
+            if (!rc) rc = build_layout(mats[s], mats[s]->L64, 0.0, 0, iw);
+            if (!rc && bt->mixed) rc = build_layout(mats[s], mats[s]->Lmix, bt->tau[s], bt->o.allow_mask, iw);
+            if (!rc) {
+                const int a = mats[s]->L64.iw, b = bt->mixed ? mats[s]->Lmix.iw : a;
+                lo = std::min(lo, std::min(a, b));
+                hi = std::max(hi, std::max(a, b));
+            }
         }
         if (rc) { cleanup(); return rc; }
-        if (all8 || wide) break;
-        wide = true;
+        if (lo >= hi) got = hi;  // (no systems: lo > hi)
+        iw = hi;
     }
-    bt->idx8 = !wide;
+    if (!got) { cleanup(); return fail(MF_E_FORMAT, "no common slab index width"); }
+    bt->iw = got;
     bt->tilem.resize(bt->nsys);
     bt->trbase.assign(bt->nsys + 1, 0);
     bt->tbase.assign(bt->nsys + 1, 0);
@@ -886,7 +903,7 @@ static int batch_build(mf_batch_s *bt)
 static int batch_capacity(mf_batch_s *bt)
 {
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pick_batched(bt->B, bt->idx8, bt->K), bt->TPB, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pick_batched(bt->B, bt->iw, bt->K), bt->TPB, 0) !=
         hipSuccess)
         per_cu = 1;
     return std::max(1, per_cu) * device_cu_count();
@@ -924,6 +941,8 @@ int mf_batch_create(int nsys, const int *n, const int *const *rowptr, const int 
     *out = nullptr;
     if (nsys <= 0 || !n || !rowptr || !colidx || !val) return fail(MF_E_ARG, "empty batch");
     if (nrhs < 1 || nrhs > 8) return fail(MF_E_ARG, "nrhs must be 1..8");
+    if (opts && (opts->index_width < 0 || opts->index_width == 3 || opts->index_width > 4))
+        return fail(MF_E_ARG, "index width must be 0, 1, 2 or 4");
     int rc = ensure_device();
     if (rc) return rc;
     mf_batch_s *bt = new mf_batch_s();
@@ -1194,6 +1213,7 @@ int mf_batch_info(mf_batch bt, mf_batch_info_t *info)
     info->mixed = bt->mixed ? 1 : 0;
     info->tau = bt->tau.empty() ? 0.0 : bt->tau[0];
     std::snprintf(info->tuned, sizeof info->tuned, "%s", bt->tuned.c_str());
+    info->index_width = bt->iw;
     return MF_OK;
 }
 

@@ -103,6 +103,7 @@ void mf_opts_default(mf_opts *o)
     o->grid = 0;
     o->block = 0;
     o->spin_limit = 1LL << 22;  // a few seconds of polling
+    o->index_width = 0;
 }
 
 int mf_matrix_create(int rowA, int colA, int nnz, const int *rowptr, const int *colidx,
@@ -286,6 +287,19 @@ int mf_matrix_layout(mf_matrix m, double tau, int allow_mask)
     return build_layout(m, m->Lmix, tau, allow_mask);
 }
 
+int mf_matrix_set_index_width(mf_matrix m, int width)
+{
+    clear_error();
+    if (!m) return fail(MF_E_ARG, "null");
+    if (width != 1 && width != 2 && width != 4) return fail(MF_E_ARG, "index width must be 1, 2 or 4");
+    if (width != m->min_iw) {
+        m->min_iw = width;
+        m->L64.release();
+        m->Lmix.release();
+    }
+    return MF_OK;
+}
+
 int mf_matrix_get_info(mf_matrix m, mf_matrix_info *info)
 {
     clear_error();
@@ -305,6 +319,7 @@ int mf_matrix_get_info(mf_matrix m, mf_matrix_info *info)
     }
     info->bytes_alg_mixed = L.bytes_alg;
     info->bytes_layout_mixed = L.bytes_actual;
+    info->index_width = L.iw;
     return MF_OK;
 }
 

@@ -18,10 +18,13 @@ namespace mf {
 // slab of ceil(L_c / 4) quads: entry k of row r sits at element
 // ((k / 4) * B + r) * 4 + k % 4, so one lane reads four consecutive entries of
 // its row with one load and the B lanes of a tile row read one contiguous
-// block.  All value quads come first, then the index quads: one byte (slot
-// << log2(B) | column in tile) when every tile row has few enough tiles for
-// the slots of a tile row to fit the lanes of its group, else four bytes (the
-// matrix column itself).  Short rows are padded with value 0 / index 0.
+// block.  All value quads come first, then the index quads in one of three
+// widths IW: one byte (slot << log2(B) | column in tile) when every tile row
+// has few enough tiles for the slots of a tile row to fit the lanes of its
+// group; else two bytes (int16 column - tr * B, the column relative to the
+// tile row's first row) when every tile lies within 32768 / B - 1 tile
+// columns of the diagonal; else four bytes (the matrix column itself).  Short
+// rows are padded with value 0 / index 0 (in every form a valid column).
 // Every slab size is a multiple of 16 bytes.  trh[tr * 4 + c] = {slab offset / 8, L_c}.  Class c:
 // 0 FP64, 1 FP32, 2 FP16, 3 FP8 E4M3 (OCP).
 struct DevL {
@@ -58,17 +61,17 @@ template <int B> __host__ __device__ constexpr int log2_of() { return B == 8 ? 3
 // Per-tile-row context of one lane group (B lanes, lane r = row r).
 struct RowCtx {
     int t0;      // first tile of the tile row
-    int tc_reg;  // tile column of slot r (one-byte index form)
+    int tc_reg;  // index base: tile column of slot r (IW 1), tr * B (IW 2), 0 (IW 4)
     int4 h01, h23;  // slab headers of classes 0,1 and 2,3
 };
 
-template <int B, bool IDX8>
+template <int B, int IW>
 __device__ __forceinline__ RowCtx row_ctx(const DevL &L, int tr, int r)
 {
     RowCtx c;
     c.t0 = L.tile_ptr[tr];
     const int nt = L.tile_ptr[tr + 1] - c.t0;
-    c.tc_reg = (IDX8 && r < nt) ? L.stc[c.t0 + r] : 0;
+    c.tc_reg = IW == 1 ? (r < nt ? L.stc[c.t0 + r] : 0) : IW == 2 ? tr * B : 0;
     const int4 *h = reinterpret_cast<const int4 *>(L.trh + (size_t)tr * 4);
     c.h01 = h[0];
     c.h23 = h[1];
@@ -76,11 +79,12 @@ __device__ __forceinline__ RowCtx row_ctx(const DevL &L, int tr, int r)
 }
 
 // Gather-vector index of one slab entry (as gather_col, from a row context).
-template <int B, bool IDX8>
+template <int B, int IW>
 __device__ __forceinline__ int gather_index(uint32_t id, const RowCtx &c, const DevL &L, int lanebase)
 {
     (void)L;
-    if (!IDX8) return (int)id;
+    if (IW == 4) return (int)id;
+    if (IW == 2) return c.tc_reg + (int)id;
     const int slot = (int)(id >> log2_of<B>()), col = (int)(id & (B - 1));
     return __shfl(c.tc_reg, lanebase + slot, 64) * B + col;
 }
@@ -122,14 +126,25 @@ template <> struct Quad<3> {
     // non-NaN code (the layout never stores NaN)
     template <int E> __device__ float get() const { return __builtin_amdgcn_cvt_f32_fp8((int)a, E); }
 };
-template <bool IDX8> struct IdxQuad;
-template <> struct IdxQuad<true> {
+template <int IW> struct IdxQuad;
+template <> struct IdxQuad<1> {
     uint32_t a;
     __device__ void load(const uint8_t *p, size_t q) { a = reinterpret_cast<const uint32_t *>(p)[q]; }
     __device__ void zero() { a = 0u; }
     template <int E> __device__ uint32_t get() const { return (a >> (8 * E)) & 0xffu; }
 };
-template <> struct IdxQuad<false> {
+// two-byte form: get() returns the sign-extended int16 as a 32-bit word
+template <> struct IdxQuad<2> {
+    uint2 a;
+    __device__ void load(const uint8_t *p, size_t q) { a = reinterpret_cast<const uint2 *>(p)[q]; }
+    __device__ void zero() { a = make_uint2(0u, 0u); }
+    template <int E> __device__ uint32_t get() const
+    {
+        const uint32_t w = E < 2 ? a.x : a.y;
+        return (uint32_t)(E & 1 ? (int32_t)w >> 16 : (int32_t)(int16_t)(w & 0xffffu));
+    }
+};
+template <> struct IdxQuad<4> {
     uint4 a;
     __device__ void load(const uint8_t *p, size_t q) { a = reinterpret_cast<const uint4 *>(p)[q]; }
     __device__ void zero() { a = make_uint4(0u, 0u, 0u, 0u); }
@@ -146,12 +161,12 @@ template <int N, typename F> __device__ __forceinline__ void static_for(F &&f)
 
 // Context of tile row tr from headers already at hand (t0, nt, slab
 // headers); only the tile-column word of the lane's slot is loaded.
-template <bool IDX8>
-__device__ __forceinline__ RowCtx row_ctx_from(const DevL &L, int t0, int nt, int4 h01, int4 h23, int r)
+template <int B, int IW>
+__device__ __forceinline__ RowCtx row_ctx_from(const DevL &L, int tr, int t0, int nt, int4 h01, int4 h23, int r)
 {
     RowCtx c;
     c.t0 = t0;
-    c.tc_reg = (IDX8 && r < nt) ? L.stc[t0 + r] : 0;
+    c.tc_reg = IW == 1 ? (r < nt ? L.stc[t0 + r] : 0) : IW == 2 ? tr * B : 0;
     c.h01 = h01;
     c.h23 = h23;
     return c;
@@ -166,14 +181,16 @@ template <typename T> __device__ __forceinline__ T ld_elem(const T *__restrict__
 
 // Element of the gathered vector for one slab index.  One-byte form (slot <<
 // log2 B | col): tcb is the lane's copy of tile column * B of slot
-// (lane - lanebase), fetched from the slot's lane with ds_bpermute.  Four-byte
-// form: the index is the column.
-template <int B, bool IDX8>
+// (lane - lanebase), fetched from the slot's lane with ds_bpermute.  Two-byte
+// form: tcb = tr * B plus the sign-extended offset.  Four-byte form: the
+// index is the column.
+template <int B, int IW>
 __device__ __forceinline__ uint32_t gather_col(uint32_t id, int tcb, int t0, const DevL &L, int lanebase)
 {
     (void)t0;
     (void)L;
-    if (!IDX8) return id;
+    if (IW == 4) return id;
+    if (IW == 2) return (uint32_t)tcb + id;
     const uint32_t slot = id >> log2_of<B>(), col = id & (B - 1);
     return (uint32_t)__builtin_amdgcn_ds_bpermute((int)((lanebase + slot) << 2), tcb) + col;
 }
@@ -186,10 +203,13 @@ template <int CLS> __device__ __forceinline__ int2 lds_header(const int4 *hd, in
     return reinterpret_cast<const int2 *>(hd + 2 * q)[CLS];
 }
 
-// tile column * B of slot r of local tile row q (one-byte index form)
-template <int B, bool IDX8> __device__ __forceinline__ int load_tcr(const DevL &L, const int *t0s, int q, int r)
+// Index base of local tile row q (global tile row tr): tile column * B of
+// slot r (one-byte form), tr * B (two-byte form), 0 (four-byte form).
+template <int B, int IW>
+__device__ __forceinline__ int load_tcr(const DevL &L, const int *t0s, int q, int tr, int r)
 {
-    if (!IDX8) return 0;
+    if (IW == 2) return tr * B;
+    if (IW == 4) return 0;
     const int t0 = t0s[q], nt = t0s[q + 1] - t0;
     return r < nt ? L.stc[t0 + r] * B : 0;
 }
@@ -199,7 +219,7 @@ template <int B, bool IDX8> __device__ __forceinline__ int load_tcr(const DevL &
 // gather is issued, then the FMAs run in each row's own entry order, so each
 // row's sum is bit-identical to slab_spmv.  FP64 values go 1 quad x NR/2 rows
 // per step (their quads are 4x wider), the other classes 2 quads x NR rows.
-template <int B, bool IDX8, int CLS, int NR, int QS, typename AccT, typename GT>
+template <int B, int IW, int CLS, int NR, int QS, typename AccT, typename GT>
 __device__ __forceinline__ void slab_spmv_rows(const DevL &L, const int4 *hd, const int *t0s, int q0,
                                                const int (&tcr)[NR], int r, int lanebase,
                                                const GT *__restrict__ g, AccT (&acc)[NR])
@@ -217,7 +237,7 @@ __device__ __forceinline__ void slab_spmv_rows(const DevL &L, const int4 *hd, co
         }
         for (int j0 = 0; j0 < nqmax; j0 += UQ) {
             Quad<CLS> v[NB][UQ];
-            IdxQuad<IDX8> id[NB][UQ];
+            IdxQuad<IW> id[NB][UQ];
 #pragma unroll
             for (int k = 0; k < NB; k++) {
                 const int nq = (h[k].y + 3) >> 2;
@@ -237,12 +257,12 @@ __device__ __forceinline__ void slab_spmv_rows(const DevL &L, const int4 *hd, co
             GT xg[NB][UQ][4];
 #pragma unroll
             for (int k = 0; k < NB; k++) {
-                const int t0 = IDX8 ? 0 : t0s[q0 + (kb + k) * QS];
+                const int t0 = 0;
 #pragma unroll
                 for (int u = 0; u < UQ; u++)
                     static_for<4>([&](auto e) {
                         xg[k][u][e] =
-                            ld_elem(g, gather_col<B, IDX8>(id[k][u].template get<e>(), tcr[kb + k], t0, L, lanebase));
+                            ld_elem(g, gather_col<B, IW>(id[k][u].template get<e>(), tcr[kb + k], t0, L, lanebase));
                     });
             }
             // padding entries (value 0) add +0 or -0 to a sum that starts at +0
@@ -258,16 +278,16 @@ __device__ __forceinline__ void slab_spmv_rows(const DevL &L, const int4 *hd, co
     }
 }
 
-// y of NR tile rows (local rows q0 + k * QS, headers in LDS), the same sums
-// as tile_row_spmv_c.
-template <int B, bool IDX8, int NR, int QS>
-__device__ __forceinline__ void tile_rows_spmv(const DevL &L, const int4 *hd, const int *t0s, int q0, int r,
-                                               int lanebase, const double *__restrict__ g64,
+// y of NR tile rows (local rows q0 + k * QS, global tile rows trq + k * QS,
+// headers in LDS), the same sums as tile_row_spmv_c.
+template <int B, int IW, int NR, int QS>
+__device__ __forceinline__ void tile_rows_spmv(const DevL &L, const int4 *hd, const int *t0s, int q0, int trq,
+                                               int r, int lanebase, const double *__restrict__ g64,
                                                const float *__restrict__ g32, double (&y)[NR])
 {
     int tcr[NR];
 #pragma unroll
-    for (int k = 0; k < NR; k++) tcr[k] = load_tcr<B, IDX8>(L, t0s, q0 + k * QS, r);
+    for (int k = 0; k < NR; k++) tcr[k] = load_tcr<B, IW>(L, t0s, q0 + k * QS, trq + k * QS, r);
     double a64[NR];
     float a32[NR];
 #pragma unroll
@@ -275,17 +295,17 @@ __device__ __forceinline__ void tile_rows_spmv(const DevL &L, const int4 *hd, co
         a64[k] = 0.0;
         a32[k] = 0.0f;
     }
-    slab_spmv_rows<B, IDX8, 0, NR, QS>(L, hd, t0s, q0, tcr, r, lanebase, g64, a64);
-    slab_spmv_rows<B, IDX8, 1, NR, QS>(L, hd, t0s, q0, tcr, r, lanebase, g32, a32);
-    slab_spmv_rows<B, IDX8, 2, NR, QS>(L, hd, t0s, q0, tcr, r, lanebase, g32, a32);
-    slab_spmv_rows<B, IDX8, 3, NR, QS>(L, hd, t0s, q0, tcr, r, lanebase, g32, a32);
+    slab_spmv_rows<B, IW, 0, NR, QS>(L, hd, t0s, q0, tcr, r, lanebase, g64, a64);
+    slab_spmv_rows<B, IW, 1, NR, QS>(L, hd, t0s, q0, tcr, r, lanebase, g32, a32);
+    slab_spmv_rows<B, IW, 2, NR, QS>(L, hd, t0s, q0, tcr, r, lanebase, g32, a32);
+    slab_spmv_rows<B, IW, 3, NR, QS>(L, hd, t0s, q0, tcr, r, lanebase, g32, a32);
 #pragma unroll
     for (int k = 0; k < NR; k++) y[k] = a64[k] + (double)a32[k];
 }
 
 // FP64-accumulated product of NV gathered vectors at once (true residuals and
 // recurrence refreshes): every stored value is widened to FP64.
-template <int B, bool IDX8, int CLS, int NV>
+template <int B, int IW, int CLS, int NV>
 __device__ __forceinline__ void slab_spmv_f64(const DevL &L, const RowCtx &c, int2 h, int r, int lanebase,
                                               const double *const (&g)[NV], double (&acc)[NV])
 {
@@ -294,13 +314,13 @@ __device__ __forceinline__ void slab_spmv_f64(const DevL &L, const RowCtx &c, in
     const uint8_t *idx = vals + (size_t)nq * B * 4 * cls_width(CLS);
     for (int j = 0; j < nq; j++) {
         Quad<CLS> v;
-        IdxQuad<IDX8> id;
+        IdxQuad<IW> id;
         v.load(vals, (size_t)j * B + r);
         id.load(idx, (size_t)j * B + r);
         static_for<4>([&](auto e) {
             if (j * 4 + e < Lc) {
                 const double a = (double)v.template get<e>();
-                const int gi = gather_index<B, IDX8>(id.template get<e>(), c, L, lanebase);
+                const int gi = gather_index<B, IW>(id.template get<e>(), c, L, lanebase);
 #pragma unroll
                 for (int q = 0; q < NV; q++) acc[q] = __builtin_fma(a, g[q][gi], acc[q]);
             }
@@ -308,21 +328,21 @@ __device__ __forceinline__ void slab_spmv_f64(const DevL &L, const RowCtx &c, in
     }
 }
 
-template <int B, bool IDX8, int NV>
+template <int B, int IW, int NV>
 __device__ __forceinline__ void tile_row_spmv_f64_c(const DevL &L, const RowCtx &c, int r, int lanebase,
                                                     const double *const (&g)[NV], double (&acc)[NV])
 {
-    if (c.h01.y) slab_spmv_f64<B, IDX8, 0, NV>(L, c, make_int2(c.h01.x, c.h01.y), r, lanebase, g, acc);
-    if (c.h01.w) slab_spmv_f64<B, IDX8, 1, NV>(L, c, make_int2(c.h01.z, c.h01.w), r, lanebase, g, acc);
-    if (c.h23.y) slab_spmv_f64<B, IDX8, 2, NV>(L, c, make_int2(c.h23.x, c.h23.y), r, lanebase, g, acc);
-    if (c.h23.w) slab_spmv_f64<B, IDX8, 3, NV>(L, c, make_int2(c.h23.z, c.h23.w), r, lanebase, g, acc);
+    if (c.h01.y) slab_spmv_f64<B, IW, 0, NV>(L, c, make_int2(c.h01.x, c.h01.y), r, lanebase, g, acc);
+    if (c.h01.w) slab_spmv_f64<B, IW, 1, NV>(L, c, make_int2(c.h01.z, c.h01.w), r, lanebase, g, acc);
+    if (c.h23.y) slab_spmv_f64<B, IW, 2, NV>(L, c, make_int2(c.h23.x, c.h23.y), r, lanebase, g, acc);
+    if (c.h23.w) slab_spmv_f64<B, IW, 3, NV>(L, c, make_int2(c.h23.z, c.h23.w), r, lanebase, g, acc);
 }
 
-template <int B, bool IDX8, int NV>
+template <int B, int IW, int NV>
 __device__ __forceinline__ void tile_row_spmv_f64(const DevL &L, int tr, int r, int lanebase,
                                                   const double *const (&g)[NV], double (&acc)[NV])
 {
-    tile_row_spmv_f64_c<B, IDX8, NV>(L, row_ctx<B, IDX8>(L, tr, r), r, lanebase, g, acc);
+    tile_row_spmv_f64_c<B, IW, NV>(L, row_ctx<B, IW>(L, tr, r), r, lanebase, g, acc);
 }
 
 // ---------------------------------------------------------- wave64 helpers

@@ -27,7 +27,8 @@ struct Layout {
     int2 *trh = nullptr;
     int *stc = nullptr;
     uint8_t *blob = nullptr;
-    bool idx8 = true;           // one-byte slab indices (else four-byte matrix columns)
+    int iw = 1;                 // slab index bytes: 1 slot form, 2 tile-row relative, 4 matrix column
+    int min_iw = 1;             // narrowest width the build was allowed
     size_t blob_bytes = 0;
     long long tiles_per_code[5] = {0, 0, 0, 0, 0};
     long long nnz_per_code[5] = {0, 0, 0, 0, 0};
@@ -72,12 +73,13 @@ struct mf_matrix_s {
     mf::Layout L64, Lmix;
     mf::Workspace ws;
     double ms_build = 0.0, ms_layout = 0.0;
+    int min_iw = 1;  // narrowest slab index width of this matrix's layouts
 };
 
 namespace mf {
 int build_reference_tiles(mf_matrix m);              // tiles.hip
 int build_layout(mf_matrix m, Layout &L, double tau, int allow_mask,
-                 bool force_wide = false);  // tiles.hip
+                 int min_iw = 1);  // tiles.hip (narrowest slab index width allowed)
 int encode_cols_device(mf_matrix m, uint8_t *d_out);  // tiles.hip
 int classes_device(mf_matrix m, double tau, int allow_mask, uint8_t *d_codes);  // tiles.hip
 hipStream_t stream();

@@ -40,7 +40,7 @@ namespace mf {
 // covers SPMV_NR * 256/B consecutive tile rows; lane group q handles tile
 // rows base + k * 256/B + q (k < SPMV_NR) together.
 constexpr int SPMV_NR = 2;
-template <int B, bool IDX8>
+template <int B, int IW>
 __global__ __launch_bounds__(256) void k_spmv(DevL L, int tilem, const double *__restrict__ g64,
                                               const float *__restrict__ g32, double *__restrict__ y)
 {
@@ -61,7 +61,7 @@ __global__ __launch_bounds__(256) void k_spmv(DevL L, int tilem, const double *_
     const int grp = threadIdx.x / B, r = threadIdx.x % B;
     const int lanebase = (threadIdx.x & 63) & ~(B - 1);
     double v[SPMV_NR];
-    tile_rows_spmv<B, IDX8, SPMV_NR, GPB>(L, s_h, s_t0, grp, r, lanebase, g64, g32, v);
+    tile_rows_spmv<B, IW, SPMV_NR, GPB>(L, s_h, s_t0, grp, tb + grp, r, lanebase, g64, g32, v);
 #pragma unroll
     for (int k = 0; k < SPMV_NR; k++) {
         const int tr = tb + k * GPB + grp;
@@ -76,7 +76,7 @@ __global__ void k_to_f32(const double *__restrict__ x, float *__restrict__ y, lo
 }
 
 // r = b - A x (FP64 accumulation over stored values); per-block partial ||r||^2
-template <int B, bool IDX8>
+template <int B, int IW>
 __global__ __launch_bounds__(256) void k_residual(DevL L, int tilem, const double *__restrict__ b,
                                                   const double *__restrict__ x, double *__restrict__ r,
                                                   double *__restrict__ part)
@@ -88,7 +88,7 @@ __global__ __launch_bounds__(256) void k_residual(DevL L, int tilem, const doubl
     if (tr < tilem) {
         const double *g[1] = {x};
         double acc[1] = {0.0};
-        tile_row_spmv_f64<B, IDX8, 1>(L, tr, lane, lanebase, g, acc);
+        tile_row_spmv_f64<B, IW, 1>(L, tr, lane, lanebase, g, acc);
         const size_t row = (size_t)tr * B + lane;
         const double rv = b[row] - acc[0];
         r[row] = rv;
@@ -273,10 +273,10 @@ struct Reducer {
 };
 
 // tile rows whose loads are interleaved inside the persistent kernels' SpMV
-// (four-byte indices take twice the registers: two rows)
-__host__ __device__ constexpr int spmv_rows(int rpt, bool idx8)
+// (four-byte indices take four times the index registers: two rows)
+__host__ __device__ constexpr int spmv_rows(int rpt, int iw)
 {
-    return rpt < (idx8 ? 4 : 2) ? rpt : (idx8 ? 4 : 2);
+    return rpt < (iw < 4 ? 4 : 2) ? rpt : (iw < 4 ? 4 : 2);
 }
 
 __device__ __forceinline__ bool finite_all(double a, double b) { return isfinite(a) && isfinite(b); }
@@ -295,7 +295,7 @@ __device__ __forceinline__ int opaque(int v)
 // holds r and s = A p.  Per iteration: [r published, (r,r) reduced] ->
 // convergence test -> w = A r with s = w + beta s and (w, r) in the same
 // pass -> reduce (w, r) -> p, x, r updates -> publish r, reduce (r, r).
-template <int B, bool IDX8, int TPB, int RPT>
+template <int B, int IW, int TPB, int RPT>
 __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
 {
     __shared__ double s_r[RPT * TPB];
@@ -304,7 +304,7 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
     __shared__ double s_bc[4];
     __shared__ int s_flag;
     Reducer<TPB, 4> R{s_red, s_bc, &s_flag};
-    constexpr int GPB = TPB / B, NR = spmv_rows(RPT, IDX8);
+    constexpr int GPB = TPB / B, NR = spmv_rows(RPT, IW);
     const int tid = threadIdx.x;
     const int chunk = xcd_chunk(blockIdx.x, gridDim.x);
     const int grp = tid / B, lane = tid % B, lanebase = (tid & 63) & ~(B - 1);
@@ -397,10 +397,10 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
                     double acc[2] = {0.0, 0.0};
                     const int q = i * GPB + grp;
                     if (A.use64)
-                        tile_row_spmv_f64_c<B, IDX8, 2>(A.L64, row_ctx<B, IDX8>(A.L64, tr, lane), lane, lanebase, g, acc);
+                        tile_row_spmv_f64_c<B, IW, 2>(A.L64, row_ctx<B, IW>(A.L64, tr, lane), lane, lanebase, g, acc);
                     else
-                        tile_row_spmv_f64_c<B, IDX8, 2>(
-                            A.L, row_ctx_from<IDX8>(A.L, s_t0[q], s_t0[q + 1] - s_t0[q], s_h[2 * q], s_h[2 * q + 1], lane),
+                        tile_row_spmv_f64_c<B, IW, 2>(
+                            A.L, row_ctx_from<B, IW>(A.L, tr, s_t0[q], s_t0[q + 1] - s_t0[q], s_h[2 * q], s_h[2 * q + 1], lane),
                             lane, lanebase, g, acc);
                     const size_t row = (size_t)tr * B + lane;
                     const double rv = A.b[row] - acc[0];
@@ -439,7 +439,7 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
 #pragma unroll 1
         for (int i0 = 0; i0 < RPT; i0 += NR) {
             double w[NR];
-            tile_rows_spmv<B, IDX8, NR, GPB>(A.L, s_h, s_t0, i0 * GPB + grp, lane, lanebase, A.g64, A.g32, w);
+            tile_rows_spmv<B, IW, NR, GPB>(A.L, s_h, s_t0, i0 * GPB + grp, tile_row(i0), lane, lanebase, A.g64, A.g32, w);
 #pragma unroll
             for (int k = 0; k < NR; k++) {
                 const int i = i0 + k;
@@ -491,7 +491,7 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
 }
 
 // BiCGSTAB: registers hold x, r, r^, p, s; LDS holds v = A p and t = A s.
-template <int B, bool IDX8, int TPB, int RPT>
+template <int B, int IW, int TPB, int RPT>
 __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
 {
     __shared__ double s_v[RPT * TPB];
@@ -500,7 +500,7 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
     __shared__ double s_bc[4];
     __shared__ int s_flag;
     Reducer<TPB, 4> R{s_red, s_bc, &s_flag};
-    constexpr int GPB = TPB / B, NR = spmv_rows(RPT, IDX8);
+    constexpr int GPB = TPB / B, NR = spmv_rows(RPT, IW);
     const int tid = threadIdx.x;
     const int chunk = xcd_chunk(blockIdx.x, gridDim.x);
     const int grp = tid / B, lane = tid % B, lanebase = (tid & 63) & ~(B - 1);
@@ -546,7 +546,7 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
 #pragma unroll 1
         for (int i0 = 0; i0 < RPT; i0 += NR) {
             double w[NR];
-            tile_rows_spmv<B, IDX8, NR, GPB>(A.L, s_h, s_t0, i0 * GPB + grp, lane, lanebase, A.g64, A.g32, w);
+            tile_rows_spmv<B, IW, NR, GPB>(A.L, s_h, s_t0, i0 * GPB + grp, tile_row(i0), lane, lanebase, A.g64, A.g32, w);
 #pragma unroll
             for (int k = 0; k < NR; k++)
                 if (tile_row(i0 + k) < tr1) dst[(i0 + k) * TPB + tid] = w[k];
@@ -629,10 +629,10 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
                     double acc[1] = {0.0};
                     const int q = i * GPB + grp;
                     if (A.use64)
-                        tile_row_spmv_f64_c<B, IDX8, 1>(A.L64, row_ctx<B, IDX8>(A.L64, tr, lane), lane, lanebase, g, acc);
+                        tile_row_spmv_f64_c<B, IW, 1>(A.L64, row_ctx<B, IW>(A.L64, tr, lane), lane, lanebase, g, acc);
                     else
-                        tile_row_spmv_f64_c<B, IDX8, 1>(
-                            A.L, row_ctx_from<IDX8>(A.L, s_t0[q], s_t0[q + 1] - s_t0[q], s_h[2 * q], s_h[2 * q + 1], lane),
+                        tile_row_spmv_f64_c<B, IW, 1>(
+                            A.L, row_ctx_from<B, IW>(A.L, tr, s_t0[q], s_t0[q + 1] - s_t0[q], s_h[2 * q], s_h[2 * q + 1], lane),
                             lane, lanebase, g, acc);
                     s_t[i * TPB + tid] = A.b[(size_t)tr * B + lane] - acc[0];
                 }
@@ -678,29 +678,32 @@ struct KernelChoice {
 
 // 512 threads (8 waves per CU, 256 VGPRs per wave) hold the register-resident
 // vectors of up to 16 rows per thread without spilling.
-template <int B, bool IDX8>
+template <int B, int IW>
 static KernelChoice pick_kernel(int method, long long rows_per_wg)
 {
     KernelChoice k;
     const int q = (int)((rows_per_wg + 511) / 512);
     if (method == MF_CG) {
-        if (q <= 1) k = {k_cg<B, IDX8, 512, 1>, 512, 1};
-        else if (q <= 2) k = {k_cg<B, IDX8, 512, 2>, 512, 2};
-        else if (q <= 4) k = {k_cg<B, IDX8, 512, 4>, 512, 4};
-        else if (q <= 8) k = {k_cg<B, IDX8, 512, 8>, 512, 8};
-        else if (q <= 16) k = {k_cg<B, IDX8, 512, 16>, 512, 16};
+        if (q <= 1) k = {k_cg<B, IW, 512, 1>, 512, 1};
+        else if (q <= 2) k = {k_cg<B, IW, 512, 2>, 512, 2};
+        else if (q <= 4) k = {k_cg<B, IW, 512, 4>, 512, 4};
+        else if (q <= 8) k = {k_cg<B, IW, 512, 8>, 512, 8};
+        else if (q <= 16) k = {k_cg<B, IW, 512, 16>, 512, 16};
     } else {
-        if (q <= 1) k = {k_bicgstab<B, IDX8, 512, 1>, 512, 1};
-        else if (q <= 2) k = {k_bicgstab<B, IDX8, 512, 2>, 512, 2};
-        else if (q <= 4) k = {k_bicgstab<B, IDX8, 512, 4>, 512, 4};
-        else if (q <= 8) k = {k_bicgstab<B, IDX8, 512, 8>, 512, 8};
+        if (q <= 1) k = {k_bicgstab<B, IW, 512, 1>, 512, 1};
+        else if (q <= 2) k = {k_bicgstab<B, IW, 512, 2>, 512, 2};
+        else if (q <= 4) k = {k_bicgstab<B, IW, 512, 4>, 512, 4};
+        else if (q <= 8) k = {k_bicgstab<B, IW, 512, 8>, 512, 8};
     }
     return k;
 }
 
-static KernelChoice pick(int B, bool idx8, int method, long long rows_per_wg)
+static KernelChoice pick(int B, int iw, int method, long long rows_per_wg)
 {
-    if (B == 16) return idx8 ? pick_kernel<16, true>(method, rows_per_wg) : pick_kernel<16, false>(method, rows_per_wg);
+    if (B == 16)
+        return iw == 1   ? pick_kernel<16, 1>(method, rows_per_wg)
+               : iw == 2 ? pick_kernel<16, 2>(method, rows_per_wg)
+                         : pick_kernel<16, 4>(method, rows_per_wg);
     return KernelChoice{};
 }
 
@@ -765,7 +768,7 @@ static int launch_inner(mf_matrix m, const Layout &L, const Layout *L64, int met
     G = std::max(1, std::min(G, m->tilem));
     const int trw = (m->tilem + G - 1) / G;
     G = (m->tilem + trw - 1) / trw;  // no empty workgroups
-    KernelChoice kc = pick(m->B, L.idx8, method, (long long)trw * m->B);
+    KernelChoice kc = pick(m->B, L.iw, method, (long long)trw * m->B);
     if (!kc.fn)
         return fail(MF_E_UNSUPPORTED,
                     "system too large for register-resident vectors at this grid size");
@@ -776,7 +779,7 @@ static int launch_inner(mf_matrix m, const Layout &L, const Layout *L64, int met
     A.L = L.view(m->d_tile_ptr);
     A.use64 = L64 && L64 != &L ? 1 : 0;
     A.L64 = A.use64 ? L64->view(m->d_tile_ptr) : A.L;
-    if (A.use64 && L64->idx8 != L.idx8) return fail(MF_E_FORMAT, "layouts with different index forms");
+    if (A.use64 && L64->iw != L.iw) return fail(MF_E_FORMAT, "layouts with different index forms");
     A.tilem = m->tilem;
     A.G = G;
     A.trw = trw;
@@ -874,10 +877,12 @@ static int residual(mf_matrix m, const Layout &L, const double *d_b, const doubl
     const int B = m->B;
     const int nb = (m->tilem * B + 255) / 256;
     DevL V = L.view(m->d_tile_ptr);
-#define MF_RES(BB, I8) k_residual<BB, I8><<<nb, 256, 0, st>>>(V, m->tilem, d_b, d_x, d_r, m->ws.red)
-    if (B == 16) { if (L.idx8) MF_RES(16, true); else MF_RES(16, false); }
-    else if (B == 8) { if (L.idx8) MF_RES(8, true); else MF_RES(8, false); }
-    else { if (L.idx8) MF_RES(32, true); else MF_RES(32, false); }
+#define MF_RES(BB, W) k_residual<BB, W><<<nb, 256, 0, st>>>(V, m->tilem, d_b, d_x, d_r, m->ws.red)
+#define MF_RES_B(BB) if (L.iw == 1) MF_RES(BB, 1); else if (L.iw == 2) MF_RES(BB, 2); else MF_RES(BB, 4)
+    if (B == 16) { MF_RES_B(16); }
+    else if (B == 8) { MF_RES_B(8); }
+    else { MF_RES_B(32); }
+#undef MF_RES_B
 #undef MF_RES
     MF_HIP(hipGetLastError());
     std::vector<double> part(nb);
@@ -894,15 +899,19 @@ int solve_device(mf_matrix m, const double *d_b, double *d_x, const mf_opts &o, 
     hipStream_t st = stream();
     std::memset(rep, 0, sizeof(*rep));
     if (m->rowA != m->colA) return fail(MF_E_ARG, "solver needs a square matrix");
+    if (o.index_width < 0 || o.index_width == 3 || o.index_width > 4)
+        return fail(MF_E_ARG, "index width must be 0, 1, 2 or 4");
     double t0 = now_ms();
     int rc;
-    if (!m->L64.built) {
-        rc = build_layout(m, m->L64, 0.0, 0);
+    const int miw = std::max(m->min_iw, o.index_width);
+    if (!m->L64.built || m->L64.min_iw != miw) {
+        rc = build_layout(m, m->L64, 0.0, 0, miw);
         if (rc) return rc;
     }
     const bool mixed = o.precision == MF_MIXED;
-    if (mixed && (!m->Lmix.built || m->Lmix.tau != o.tau || m->Lmix.allow_mask != o.allow_mask)) {
-        rc = build_layout(m, m->Lmix, o.tau, o.allow_mask);
+    if (mixed && (!m->Lmix.built || m->Lmix.tau != o.tau || m->Lmix.allow_mask != o.allow_mask ||
+                  m->Lmix.min_iw != miw)) {
+        rc = build_layout(m, m->Lmix, o.tau, o.allow_mask, miw);
         if (rc) return rc;
     }
     rep->ms_setup = now_ms() - t0;
@@ -996,10 +1005,12 @@ int spmv_device(mf_matrix m, const Layout &L, const double *d_x, double *d_y, bo
         k_to_f32<<<(unsigned)((ncol + 255) / 256), 256, 0, st>>>(d_x, m->ws.g32, ncol);
     const int nb = (int)(((long long)m->tilem * B + 256 * SPMV_NR - 1) / (256 * SPMV_NR));
     DevL V = L.view(m->d_tile_ptr);
-#define MF_SPMV(BB, I8) k_spmv<BB, I8><<<nb, 256, 0, st>>>(V, m->tilem, d_x, m->ws.g32, d_y)
-    if (B == 16) { if (L.idx8) MF_SPMV(16, true); else MF_SPMV(16, false); }
-    else if (B == 8) { if (L.idx8) MF_SPMV(8, true); else MF_SPMV(8, false); }
-    else { if (L.idx8) MF_SPMV(32, true); else MF_SPMV(32, false); }
+#define MF_SPMV(BB, W) k_spmv<BB, W><<<nb, 256, 0, st>>>(V, m->tilem, d_x, m->ws.g32, d_y)
+#define MF_SPMV_B(BB) if (L.iw == 1) MF_SPMV(BB, 1); else if (L.iw == 2) MF_SPMV(BB, 2); else MF_SPMV(BB, 4)
+    if (B == 16) { MF_SPMV_B(16); }
+    else if (B == 8) { MF_SPMV_B(8); }
+    else { MF_SPMV_B(32); }
+#undef MF_SPMV_B
 #undef MF_SPMV
     MF_HIP(hipGetLastError());
     return MF_OK;

@@ -446,10 +446,17 @@ __global__ __launch_bounds__(256) void k_classes(const int *__restrict__ tile_st
     codes[t] = code;
 }
 
-__global__ __launch_bounds__(256) void k_max_tiles(const int *__restrict__ tile_ptr, int tilem, int *out)
+// out[0] = most tiles in a tile row, out[1] = largest |tile column - tile row|
+__global__ __launch_bounds__(256) void k_max_tiles(const int *__restrict__ tile_ptr,
+                                                   const int *__restrict__ tile_columnidx, int tilem, int *out)
 {
     const int tr = blockIdx.x * 256 + threadIdx.x;
-    if (tr < tilem) atomicMax(out, tile_ptr[tr + 1] - tile_ptr[tr]);
+    if (tr >= tilem) return;
+    const int t0 = tile_ptr[tr], t1 = tile_ptr[tr + 1];
+    int d = 0;
+    for (int t = t0; t < t1; t++) d = max(d, abs(tile_columnidx[t] - tr));
+    atomicMax(out, t1 - t0);
+    atomicMax(out + 1, d);
 }
 
 // Per tile row: row lengths per class -> slab sizes (8-byte units) and the
@@ -539,8 +546,12 @@ __global__ __launch_bounds__(256) void k_slab_pack(const int *__restrict__ tile_
                 case 2: reinterpret_cast<uint16_t *>(vals)[e] = f64_to_f16(v); break;
                 default: vals[e] = f64_to_e4m3(v); break;
                 }
-                if (iw == 1) idx[e] = (uint8_t)(((unsigned)slot << log2_of<B>()) | bcol[k]);
-                else reinterpret_cast<uint32_t *>(idx)[e] = (uint32_t)tile_columnidx[t] * B + bcol[k];
+                if (iw == 1)
+                    idx[e] = (uint8_t)(((unsigned)slot << log2_of<B>()) | bcol[k]);
+                else if (iw == 2)
+                    reinterpret_cast<int16_t *>(idx)[e] = (int16_t)((tile_columnidx[t] - tr) * B + bcol[k]);
+                else
+                    reinterpret_cast<uint32_t *>(idx)[e] = (uint32_t)tile_columnidx[t] * B + bcol[k];
             }
             slot++;
         }
@@ -721,7 +732,7 @@ void Layout::release()
 }
 
 template <int B>
-static int build_layout_B(mf_matrix m, Layout &L, double tau, int allow_mask, bool force_wide)
+static int build_layout_B(mf_matrix m, Layout &L, double tau, int allow_mask, int min_iw)
 {
     hipStream_t st = stream();
     L.release();
@@ -733,10 +744,10 @@ static int build_layout_B(mf_matrix m, Layout &L, double tau, int allow_mask, bo
     MF_HIP(hipMallocAsync((void **)&units, sizeof(unsigned long long) * (tilem + 1), st));
     MF_HIP(hipMallocAsync((void **)&stats, sizeof(unsigned long long) * 10, st));
     MF_HIP(hipMallocAsync((void **)&err, sizeof(int), st));
-    MF_HIP(hipMallocAsync((void **)&maxnt, sizeof(int), st));
+    MF_HIP(hipMallocAsync((void **)&maxnt, 2 * sizeof(int), st));
     MF_HIP(hipMemsetAsync(stats, 0, sizeof(unsigned long long) * 10, st));
     MF_HIP(hipMemsetAsync(err, 0, sizeof(int), st));
-    MF_HIP(hipMemsetAsync(maxnt, 0, sizeof(int), st));
+    MF_HIP(hipMemsetAsync(maxnt, 0, 2 * sizeof(int), st));
     MF_HIP(hipMemsetAsync(units, 0, sizeof(unsigned long long) * (tilem + 1), st));
     const unsigned nbt = (unsigned)((tilenum + 255) / 256), nbr = (unsigned)((tilem + 255) / 256);
     if (allow_mask) {
@@ -744,15 +755,17 @@ static int build_layout_B(mf_matrix m, Layout &L, double tau, int allow_mask, bo
     } else {
         MF_HIP(hipMemsetAsync(codes, 0, (size_t)tilenum + 1, st));
     }
-    k_max_tiles<<<nbr, 256, 0, st>>>(m->d_tile_ptr, tilem, maxnt);
-    int hmax = 0;
-    MF_HIP(hipMemcpyAsync(&hmax, maxnt, sizeof(int), hipMemcpyDeviceToHost, st));
+    if (tilem) k_max_tiles<<<nbr, 256, 0, st>>>(m->d_tile_ptr, m->d_tile_columnidx, tilem, maxnt);
+    int hmax[2] = {0, 0};
+    MF_HIP(hipMemcpyAsync(hmax, maxnt, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
     MF_HIP(hipStreamSynchronize(st));
-    // one-byte index: slot and column fit 8 bits and every slot has a lane of
-    // the group to hold its tile column (B = 8: 8 slots, 16: 16, 32: 8);
-    // otherwise a four-byte index holds the matrix column itself
-    const bool idx8 = !force_wide && hmax <= std::min(B, 1 << (8 - log2_of<B>()));
-    const int iw = idx8 ? 1 : 4;
+    // index width (at least min_iw): one byte when slot and column fit 8 bits
+    // and every slot has a lane of the group to hold its tile column (B = 8:
+    // 8 slots, 16: 16, 32: 8); two bytes when every column - tr * B fits an
+    // int16 (|tile column - tr| <= 32768 / B - 1); else the four-byte column
+    int iw = 4;
+    if (min_iw <= 1 && hmax[0] <= std::min(B, 1 << (8 - log2_of<B>()))) iw = 1;
+    else if (min_iw <= 2 && (long long)(hmax[1] + 1) * B <= 32768) iw = 2;
     k_slab_sizes<B><<<nbr, 256, 0, st>>>(m->d_tile_ptr, tilem, m->d_tile_start, m->d_rowtmp, codes, iw,
                                          units, stats);
     int rc = scan_exclusive<unsigned long long>(units, units, tilem + 1, st);
@@ -788,7 +801,8 @@ static int build_layout_B(mf_matrix m, Layout &L, double tau, int allow_mask, bo
         L.tiles_per_code[c] = (long long)hs[c];
         L.nnz_per_code[c] = (long long)hs[5 + c];
     }
-    L.idx8 = idx8;
+    L.iw = iw;
+    L.min_iw = min_iw;
     L.has64 = L.tiles_per_code[0] > 0;
     L.haslow = L.tiles_per_code[2] + L.tiles_per_code[3] + L.tiles_per_code[4] > 0;
     // algorithmic bytes: sum nnz (w+1) + tilenum (8+B) + 4 (tilem+1) + n (w_x + 8),
@@ -811,14 +825,15 @@ static int build_layout_B(mf_matrix m, Layout &L, double tau, int allow_mask, bo
     return MF_OK;
 }
 
-int build_layout(mf_matrix m, Layout &L, double tau, int allow_mask, bool force_wide)
+int build_layout(mf_matrix m, Layout &L, double tau, int allow_mask, int min_iw)
 {
     double t0 = now_ms();
     int rc;
+    min_iw = std::max(min_iw, m->min_iw);
     switch (m->B) {
-    case 8: rc = build_layout_B<8>(m, L, tau, allow_mask, force_wide); break;
-    case 16: rc = build_layout_B<16>(m, L, tau, allow_mask, force_wide); break;
-    case 32: rc = build_layout_B<32>(m, L, tau, allow_mask, force_wide); break;
+    case 8: rc = build_layout_B<8>(m, L, tau, allow_mask, min_iw); break;
+    case 16: rc = build_layout_B<16>(m, L, tau, allow_mask, min_iw); break;
+    case 32: rc = build_layout_B<32>(m, L, tau, allow_mask, min_iw); break;
     default: return fail(MF_E_ARG, "tile size");
     }
     m->ms_layout += now_ms() - t0;

@@ -88,6 +88,27 @@ def test_gpu_batched_multi_workgroup_groups(gpu, orc):
     bt.close()
 
 
+def test_gpu_batched_common_index_width(gpu, orc):
+    # a 27-point system needs the two-byte form, the Laplacians fit one
+    # byte: the joined layout takes two bytes for all; forcing the four-byte
+    # column form (rebased per system) gives the same iterates bit for bit
+    cd = gpu.gen_convdiff3d(32, 0.0, 0.0, 0)  # symmetric (no convection, constant diffusion)
+    systems = [gpu.gen_laplace2d(30, 30), cd] + _systems(gpu, [1500], seed=11)
+    b = _rhs(systems, 8, 12)
+    out = {}
+    for w in (0, 4):
+        bt = gpu.Batch(systems, nrhs=8, tol=TOL, index_width=w)
+        info = bt.info()
+        print(f"width {w}: {info['index_width']}", flush=True)
+        assert info["index_width"] == (2 if w == 0 else 4)
+        xs, it, rel, rep = bt.solve(b)
+        _check_batch(orc, systems, b, xs, it, rel, rep, check_iters=(w == 0))
+        out[w] = (xs, it)
+        bt.close()
+    assert np.array_equal(out[0][1], out[4][1])
+    assert all(np.array_equal(a, c) for a, c in zip(out[0][0], out[4][0]))
+
+
 def test_gpu_batched_mixed_lossy_tiles(gpu):
     # FP16-rounded tiles (tau = 2^-11) as the inner operator; the FP64
     # true-residual passes still bring every column to 1e-10

@@ -65,6 +65,62 @@ def test_gpu_spmv_mixed_bit_exact(gpu, orc, tau):
         assert np.array_equal(A.spmv(x, mixed=False), t.spmv(x))
 
 
+def _index_width_cases(gpu):
+    # natural widths at B = 16: Poisson 1 (<= 16 tiles per tile row), the
+    # 27-point stencil on 32^3 2 (18 tiles per tile row), random SPD n = 20000 2
+    # (tiles up to 1249 tile columns off the diagonal), random SPD n = 40000 4
+    return {"poisson3d_24": gpu.gen_poisson3d(24, 24, 24), "convdiff27_32": gpu.gen_convdiff3d(32, 10.0, 1.0, 0),
+            "random_spd_20k": gpu.gen_random_spd(20000, seed=1), "random_spd_40k": gpu.gen_random_spd(40000, seed=2)}
+
+
+@pytest.mark.parametrize("name,natural", [("poisson3d_24", 1), ("convdiff27_32", 2), ("random_spd_20k", 2),
+                                          ("random_spd_40k", 4)])
+def test_gpu_spmv_index_widths_bit_exact(gpu, orc, name, natural):
+    """The one-byte slot, two-byte tile-row-relative and four-byte column
+    index forms address the same entries in the same order: every form the
+    matrix fits gives the oracle's SpMV bit for bit, FP64 and mixed."""
+    rp, ci, va = _index_width_cases(gpu)[name]
+    m = len(rp) - 1
+    t = orc.tiles(16, m, m, rp, ci, va)
+    codes = t.classes(2.0 ** -11)
+    x = np.random.default_rng(4).standard_normal(m)
+    y64, ymix = t.spmv(x), t.spmv_mixed(codes, x)
+    A = gpu.Matrix(rp, ci, va)
+    for w in (1, 2, 4):
+        A.set_index_width(w)
+        A.layout(2.0 ** -11)
+        got = A.info()["index_width"]
+        print(f"case {name} min width {w} -> {got}", flush=True)
+        assert got == max(w, natural)
+        assert np.array_equal(A.spmv(x, mixed=False), y64)
+        assert np.array_equal(A.spmv(x, mixed=True), ymix)
+    A.close()
+
+
+@pytest.mark.parametrize("method", ["cg", "bicgstab"])
+def test_gpu_solvers_index_widths_identical(gpu, orc, method):
+    # same sums in the same order: bit-identical iterates for every index form
+    if method == "cg":
+        rp, ci, va = gpu.gen_poisson3d(24, 24, 24)
+        mth, widths = gpu.MF_CG, (1, 2, 4)
+    else:
+        rp, ci, va = gpu.gen_convdiff3d(32, 10.0, 1.0, 0)
+        mth, widths = gpu.MF_BICGSTAB, (2, 4)
+    b = orc.csr_spmv(rp, ci, va, gpu.gen_uniform(5, len(rp) - 1, -1, 1))
+    A = gpu.Matrix(rp, ci, va)
+    ref = None
+    for w in widths:
+        for prec in (gpu.MF_FP64, gpu.MF_MIXED):
+            x, rep = A.solve(b, method=mth, tol=TOL, precision=prec, index_width=w, rr_factor=1e-2)
+            print(f"{method} width {w} prec {prec} it={rep['iterations']} relres={rep['relres_true']:.2e}", flush=True)
+            assert rep["status"] == 0 and rep["relres_true"] <= TOL
+            if prec == gpu.MF_FP64:
+                if ref is None:
+                    ref = (x, rep["iterations"])
+                assert rep["iterations"] == ref[1] and np.array_equal(x, ref[0])
+    A.close()
+
+
 def _cg_case(gpu, orc, rp, ci, va, b, **opts):
     A = gpu.Matrix(rp, ci, va)
     x, rep = A.solve(b, method=gpu.MF_CG, tol=TOL, **opts)

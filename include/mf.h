@@ -139,11 +139,12 @@ typedef struct {
 } mf_matrix_info;
 int mf_matrix_get_info(mf_matrix m, mf_matrix_info *info);
 
-/* Narrowest slab index width (bytes) of this matrix's compute layouts:
- * 1 (default) = one-byte slot form where every tile row fits it, else the
- * two-byte tile-row-relative form where every tile lies within 32768/B - 1
- * tile columns of the diagonal, else the four-byte column; 2 or 4 skip the
- * narrower forms.  Layouts already built are released.  No reference
+/* Slab index width (bytes) of this matrix's compute layouts.  0 (default)
+ * = automatic: the two-byte tile-row-relative form where every tile lies
+ * within 32768/B - 1 tile columns of the diagonal, else the one-byte slot
+ * form where every tile row has few enough tiles (B = 8: 8, 16: 16, 32: 8),
+ * else the four-byte column.  1, 2 or 4 = that form where it fits, else the
+ * next wider one.  Layouts already built are released.  No reference
  * counterpart (the reference stores four-byte columns only). */
 int mf_matrix_set_index_width(mf_matrix m, int width);
 
@@ -167,8 +168,9 @@ typedef struct {
     int grid;           /* workgroups (0: one per CU) */
     int block;          /* threads per workgroup (0: auto, 512 or 1024) */
     long long spin_limit; /* grid-barrier polls before MF_E_TIMEOUT */
-    int index_width;    /* narrowest slab index width, as mf_matrix_set_index_width
-                           (0 = that of the matrix; batched: 0 = 1) */
+    int index_width;    /* slab index width, as mf_matrix_set_index_width
+                           (0 = the matrix's setting; batched: 0 = automatic,
+                           one width that every system fits) */
 } mf_opts;
 void mf_opts_default(mf_opts *o);
 

@@ -339,7 +339,8 @@ class Matrix:
         _check(lib().mf_matrix_layout(self.h, tau, allow_mask))
 
     def set_index_width(self, width):
-        """Narrowest slab index width (1, 2 or 4 bytes) of the compute layouts."""
+        """Slab index width of the compute layouts: 0 automatic, or 1, 2, 4
+        bytes (that form where it fits, else the next wider one)."""
         _check(lib().mf_matrix_set_index_width(self.h, width))
 
     def spmv(self, x, mixed=False):

@@ -829,29 +829,25 @@ static int batch_build(mf_batch_s *bt)
             if (m) mf_matrix_destroy(m);
     };
     int rc = MF_OK;
-    // every system takes the narrowest index form that all of them fit: build
-    // at the narrowest width each allows, then rebuild at the widest width
-    // found until they agree (the width only grows: at most three passes)
-    int iw = std::max(1, bt->o.index_width), got = 0;
-    for (int pass = 0; pass < 3 && !got; pass++) {
-        int lo = 4, hi = 1;
-        for (int s = 0; s < bt->nsys && !rc; s++) {
-            if (!mats[s]) rc = mf_matrix_create(bt->n[s], bt->n[s], bt->h_rp[s][bt->n[s]], bt->h_rp[s].data(),
-                                                 bt->h_ci[s].data(), bt->h_va[s].data(), nullptr, bt->B, &mats[s]);
-            if (!rc) rc = build_layout(mats[s], mats[s]->L64, 0.0, 0, iw);
-            if (!rc && bt->mixed) rc = build_layout(mats[s], mats[s]->Lmix, bt->tau[s], bt->o.allow_mask, iw);
-            if (!rc) {
-                const int a = mats[s]->L64.iw, b = bt->mixed ? mats[s]->Lmix.iw : a;
-                lo = std::min(lo, std::min(a, b));
-                hi = std::max(hi, std::max(a, b));
-            }
-        }
-        if (rc) { cleanup(); return rc; }
-        if (lo >= hi) got = hi;  // (no systems: lo > hi)
-        iw = hi;
+    // one index width for all systems: the request resolved against the
+    // forms that every system fits
+    int fits = 3;
+    for (int s = 0; s < bt->nsys && !rc; s++) {
+        rc = mf_matrix_create(bt->n[s], bt->n[s], bt->h_rp[s][bt->n[s]], bt->h_rp[s].data(), bt->h_ci[s].data(),
+                              bt->h_va[s].data(), nullptr, bt->B, &mats[s]);
+        int f = 0;
+        if (!rc) rc = index_fits(mats[s], &f);
+        fits &= f;
     }
-    if (!got) { cleanup(); return fail(MF_E_FORMAT, "no common slab index width"); }
-    bt->iw = got;
+    const int iw = resolve_index_width(bt->o.index_width, fits);
+    for (int s = 0; s < bt->nsys && !rc; s++) {
+        rc = build_layout(mats[s], mats[s]->L64, 0.0, 0, iw);
+        if (!rc && bt->mixed) rc = build_layout(mats[s], mats[s]->Lmix, bt->tau[s], bt->o.allow_mask, iw);
+        if (!rc && (mats[s]->L64.iw != iw || (bt->mixed && mats[s]->Lmix.iw != iw)))
+            rc = fail(MF_E_FORMAT, "slab index width differs between systems");
+    }
+    if (rc) { cleanup(); return rc; }
+    bt->iw = iw;
     bt->tilem.resize(bt->nsys);
     bt->trbase.assign(bt->nsys + 1, 0);
     bt->tbase.assign(bt->nsys + 1, 0);

@@ -291,9 +291,9 @@ int mf_matrix_set_index_width(mf_matrix m, int width)
 {
     clear_error();
     if (!m) return fail(MF_E_ARG, "null");
-    if (width != 1 && width != 2 && width != 4) return fail(MF_E_ARG, "index width must be 1, 2 or 4");
-    if (width != m->min_iw) {
-        m->min_iw = width;
+    if (width != 0 && width != 1 && width != 2 && width != 4) return fail(MF_E_ARG, "index width must be 0, 1, 2 or 4");
+    if (width != m->want_iw) {
+        m->want_iw = width;
         m->L64.release();
         m->Lmix.release();
     }

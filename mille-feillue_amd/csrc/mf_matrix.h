@@ -28,7 +28,7 @@ struct Layout {
     int *stc = nullptr;
     uint8_t *blob = nullptr;
     int iw = 1;                 // slab index bytes: 1 slot form, 2 tile-row relative, 4 matrix column
-    int min_iw = 1;             // narrowest width the build was allowed
+    int want_iw = 0;            // width asked for (0 automatic; see resolve_index_width)
     size_t blob_bytes = 0;
     long long tiles_per_code[5] = {0, 0, 0, 0, 0};
     long long nnz_per_code[5] = {0, 0, 0, 0, 0};
@@ -73,13 +73,18 @@ struct mf_matrix_s {
     mf::Layout L64, Lmix;
     mf::Workspace ws;
     double ms_build = 0.0, ms_layout = 0.0;
-    int min_iw = 1;  // narrowest slab index width of this matrix's layouts
+    int want_iw = 0;  // slab index width asked for this matrix's layouts (0 automatic)
 };
 
 namespace mf {
 int build_reference_tiles(mf_matrix m);              // tiles.hip
-int build_layout(mf_matrix m, Layout &L, double tau, int allow_mask,
-                 int min_iw = 1);  // tiles.hip (narrowest slab index width allowed)
+// want_iw: slab index width asked for (0: the matrix's own setting)
+int build_layout(mf_matrix m, Layout &L, double tau, int allow_mask, int want_iw = 0);  // tiles.hip
+// fits bit 0: the one-byte slot form fits, bit 1: the two-byte form fits
+int index_fits(mf_matrix m, int *fits);  // tiles.hip
+// the width used for a request (0 automatic: two bytes, else one, else four;
+// 1, 2 or 4: that form if it fits, else the next wider one that fits)
+int resolve_index_width(int want, int fits);  // tiles.hip
 int encode_cols_device(mf_matrix m, uint8_t *d_out);  // tiles.hip
 int classes_device(mf_matrix m, double tau, int allow_mask, uint8_t *d_codes);  // tiles.hip
 hipStream_t stream();

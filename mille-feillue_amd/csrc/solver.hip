@@ -903,15 +903,15 @@ int solve_device(mf_matrix m, const double *d_b, double *d_x, const mf_opts &o, 
         return fail(MF_E_ARG, "index width must be 0, 1, 2 or 4");
     double t0 = now_ms();
     int rc;
-    const int miw = std::max(m->min_iw, o.index_width);
-    if (!m->L64.built || m->L64.min_iw != miw) {
-        rc = build_layout(m, m->L64, 0.0, 0, miw);
+    const int want = o.index_width ? o.index_width : m->want_iw;
+    if (!m->L64.built || m->L64.want_iw != want) {
+        rc = build_layout(m, m->L64, 0.0, 0, want);
         if (rc) return rc;
     }
     const bool mixed = o.precision == MF_MIXED;
     if (mixed && (!m->Lmix.built || m->Lmix.tau != o.tau || m->Lmix.allow_mask != o.allow_mask ||
-                  m->Lmix.min_iw != miw)) {
-        rc = build_layout(m, m->Lmix, o.tau, o.allow_mask, miw);
+                  m->Lmix.want_iw != want)) {
+        rc = build_layout(m, m->Lmix, o.tau, o.allow_mask, want);
         if (rc) return rc;
     }
     rep->ms_setup = now_ms() - t0;

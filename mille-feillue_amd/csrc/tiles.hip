@@ -732,22 +732,20 @@ void Layout::release()
 }
 
 template <int B>
-static int build_layout_B(mf_matrix m, Layout &L, double tau, int allow_mask, int min_iw)
+static int build_layout_B(mf_matrix m, Layout &L, double tau, int allow_mask, int want)
 {
     hipStream_t st = stream();
     L.release();
     const int tilenum = m->tilenum, tilem = m->tilem;
     uint8_t *codes = nullptr;
     unsigned long long *units = nullptr, *stats = nullptr;
-    int *err = nullptr, *maxnt = nullptr;
+    int *err = nullptr;
     MF_HIP(hipMallocAsync((void **)&codes, (size_t)tilenum + 1, st));
     MF_HIP(hipMallocAsync((void **)&units, sizeof(unsigned long long) * (tilem + 1), st));
     MF_HIP(hipMallocAsync((void **)&stats, sizeof(unsigned long long) * 10, st));
     MF_HIP(hipMallocAsync((void **)&err, sizeof(int), st));
-    MF_HIP(hipMallocAsync((void **)&maxnt, 2 * sizeof(int), st));
     MF_HIP(hipMemsetAsync(stats, 0, sizeof(unsigned long long) * 10, st));
     MF_HIP(hipMemsetAsync(err, 0, sizeof(int), st));
-    MF_HIP(hipMemsetAsync(maxnt, 0, 2 * sizeof(int), st));
     MF_HIP(hipMemsetAsync(units, 0, sizeof(unsigned long long) * (tilem + 1), st));
     const unsigned nbt = (unsigned)((tilenum + 255) / 256), nbr = (unsigned)((tilem + 255) / 256);
     if (allow_mask) {
@@ -755,20 +753,13 @@ static int build_layout_B(mf_matrix m, Layout &L, double tau, int allow_mask, in
     } else {
         MF_HIP(hipMemsetAsync(codes, 0, (size_t)tilenum + 1, st));
     }
-    if (tilem) k_max_tiles<<<nbr, 256, 0, st>>>(m->d_tile_ptr, m->d_tile_columnidx, tilem, maxnt);
-    int hmax[2] = {0, 0};
-    MF_HIP(hipMemcpyAsync(hmax, maxnt, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
-    MF_HIP(hipStreamSynchronize(st));
-    // index width (at least min_iw): one byte when slot and column fit 8 bits
-    // and every slot has a lane of the group to hold its tile column (B = 8:
-    // 8 slots, 16: 16, 32: 8); two bytes when every column - tr * B fits an
-    // int16 (|tile column - tr| <= 32768 / B - 1); else the four-byte column
-    int iw = 4;
-    if (min_iw <= 1 && hmax[0] <= std::min(B, 1 << (8 - log2_of<B>()))) iw = 1;
-    else if (min_iw <= 2 && (long long)(hmax[1] + 1) * B <= 32768) iw = 2;
+    int fits = 0;
+    int rc = index_fits(m, &fits);
+    if (rc) return rc;
+    const int iw = resolve_index_width(want, fits);
     k_slab_sizes<B><<<nbr, 256, 0, st>>>(m->d_tile_ptr, tilem, m->d_tile_start, m->d_rowtmp, codes, iw,
                                          units, stats);
-    int rc = scan_exclusive<unsigned long long>(units, units, tilem + 1, st);
+    rc = scan_exclusive<unsigned long long>(units, units, tilem + 1, st);
     if (rc) return rc;
     unsigned long long total8 = 0, hs[10];
     MF_HIP(hipMemcpyAsync(&total8, units + tilem, sizeof(total8), hipMemcpyDeviceToHost, st));
@@ -792,7 +783,6 @@ static int build_layout_B(mf_matrix m, Layout &L, double tau, int allow_mask, in
     MF_HIP(hipFree(units));
     MF_HIP(hipFree(stats));
     MF_HIP(hipFree(err));
-    MF_HIP(hipFree(maxnt));
     if (herr) {
         L.release();
         return fail(MF_E_FORMAT, "layout larger than 32 GB");
@@ -802,7 +792,7 @@ static int build_layout_B(mf_matrix m, Layout &L, double tau, int allow_mask, in
         L.nnz_per_code[c] = (long long)hs[5 + c];
     }
     L.iw = iw;
-    L.min_iw = min_iw;
+    L.want_iw = want;
     L.has64 = L.tiles_per_code[0] > 0;
     L.haslow = L.tiles_per_code[2] + L.tiles_per_code[3] + L.tiles_per_code[4] > 0;
     // algorithmic bytes: sum nnz (w+1) + tilenum (8+B) + 4 (tilem+1) + n (w_x + 8),
@@ -825,19 +815,51 @@ static int build_layout_B(mf_matrix m, Layout &L, double tau, int allow_mask, in
     return MF_OK;
 }
 
-int build_layout(mf_matrix m, Layout &L, double tau, int allow_mask, int min_iw)
+int build_layout(mf_matrix m, Layout &L, double tau, int allow_mask, int want_iw)
 {
     double t0 = now_ms();
     int rc;
-    min_iw = std::max(min_iw, m->min_iw);
+    if (!want_iw) want_iw = m->want_iw;
     switch (m->B) {
-    case 8: rc = build_layout_B<8>(m, L, tau, allow_mask, min_iw); break;
-    case 16: rc = build_layout_B<16>(m, L, tau, allow_mask, min_iw); break;
-    case 32: rc = build_layout_B<32>(m, L, tau, allow_mask, min_iw); break;
+    case 8: rc = build_layout_B<8>(m, L, tau, allow_mask, want_iw); break;
+    case 16: rc = build_layout_B<16>(m, L, tau, allow_mask, want_iw); break;
+    case 32: rc = build_layout_B<32>(m, L, tau, allow_mask, want_iw); break;
     default: return fail(MF_E_ARG, "tile size");
     }
     m->ms_layout += now_ms() - t0;
     return rc;
+}
+
+int index_fits(mf_matrix m, int *fits)
+{
+    hipStream_t st = stream();
+    const int B = m->B, tilem = m->tilem;
+    int *d = nullptr;
+    MF_HIP(hipMallocAsync((void **)&d, 2 * sizeof(int), st));
+    MF_HIP(hipMemsetAsync(d, 0, 2 * sizeof(int), st));
+    if (tilem) k_max_tiles<<<(unsigned)((tilem + 255) / 256), 256, 0, st>>>(m->d_tile_ptr, m->d_tile_columnidx, tilem, d);
+    MF_HIP(hipGetLastError());
+    int h[2] = {0, 0};
+    MF_HIP(hipMemcpyAsync(h, d, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+    MF_HIP(hipStreamSynchronize(st));
+    MF_HIP(hipFree(d));
+    // one byte: slot and column fit 8 bits and every slot has a lane of the
+    // group to hold its tile column (B = 8: 8 slots, 16: 16, 32: 8); two
+    // bytes: every column - tr * B fits an int16 (|tile column - tr| <= 32768 / B - 1)
+    const int lb = B == 8 ? 3 : (B == 16 ? 4 : 5);
+    *fits = (h[0] <= std::min(B, 1 << (8 - lb)) ? 1 : 0) | ((long long)(h[1] + 1) * B <= 32768 ? 2 : 0);
+    return MF_OK;
+}
+
+int resolve_index_width(int want, int fits)
+{
+    const bool f1 = fits & 1, f2 = fits & 2;
+    switch (want) {
+    case 1: return f1 ? 1 : f2 ? 2 : 4;
+    case 2: return f2 ? 2 : 4;
+    case 4: return 4;
+    default: return f2 ? 2 : f1 ? 1 : 4;
+    }
 }
 
 }  // namespace mf

@@ -89,24 +89,34 @@ def test_gpu_batched_multi_workgroup_groups(gpu, orc):
 
 
 def test_gpu_batched_common_index_width(gpu, orc):
-    # a 27-point system needs the two-byte form, the Laplacians fit one
-    # byte: the joined layout takes two bytes for all; forcing the four-byte
-    # column form (rebased per system) gives the same iterates bit for bit
+    # the 27-point system (18 tiles per tile row) does not fit the one-byte
+    # slot form, so a request for one byte gives the two-byte form for all;
+    # the four-byte column form (rebased per system when the layouts are
+    # joined) gives the same iterates bit for bit; the Laplacians alone run
+    # in the slot form
     cd = gpu.gen_convdiff3d(32, 0.0, 0.0, 0)  # symmetric (no convection, constant diffusion)
-    systems = [gpu.gen_laplace2d(30, 30), cd] + _systems(gpu, [1500], seed=11)
+    lap = [gpu.gen_laplace2d(30, 30)] + _systems(gpu, [1500], seed=11)
+    systems = [lap[0], cd, lap[1]]
     b = _rhs(systems, 8, 12)
     out = {}
-    for w in (0, 4):
+    for w, want in ((0, 2), (1, 2), (4, 4)):
         bt = gpu.Batch(systems, nrhs=8, tol=TOL, index_width=w)
         info = bt.info()
         print(f"width {w}: {info['index_width']}", flush=True)
-        assert info["index_width"] == (2 if w == 0 else 4)
+        assert info["index_width"] == want
         xs, it, rel, rep = bt.solve(b)
         _check_batch(orc, systems, b, xs, it, rel, rep, check_iters=(w == 0))
         out[w] = (xs, it)
         bt.close()
-    assert np.array_equal(out[0][1], out[4][1])
-    assert all(np.array_equal(a, c) for a, c in zip(out[0][0], out[4][0]))
+    for w in (1, 4):
+        assert np.array_equal(out[0][1], out[w][1])
+        assert all(np.array_equal(a, c) for a, c in zip(out[0][0], out[w][0]))
+    bl = [b[0], b[2]]
+    bt = gpu.Batch(lap, nrhs=8, tol=TOL, index_width=1)
+    assert bt.info()["index_width"] == 1
+    xs, it, rel, rep = bt.solve(bl)
+    bt.close()
+    _check_batch(orc, lap, bl, xs, it, rel, rep)
 
 
 def test_gpu_batched_mixed_lossy_tiles(gpu):

@@ -66,16 +66,25 @@ def test_gpu_spmv_mixed_bit_exact(gpu, orc, tau):
 
 
 def _index_width_cases(gpu):
-    # natural widths at B = 16: Poisson 1 (<= 16 tiles per tile row), the
-    # 27-point stencil on 32^3 2 (18 tiles per tile row), random SPD n = 20000 2
-    # (tiles up to 1249 tile columns off the diagonal), random SPD n = 40000 4
+    # B = 16: Poisson 24^3 fits the one-byte slot form (<= 16 tiles per tile
+    # row) and the two-byte form; the 27-point stencil on 32^3 (18 tiles per
+    # tile row) and random SPD n = 20000 (tiles up to 1249 tile columns off
+    # the diagonal) only the two-byte form; random SPD n = 40000 neither
     return {"poisson3d_24": gpu.gen_poisson3d(24, 24, 24), "convdiff27_32": gpu.gen_convdiff3d(32, 10.0, 1.0, 0),
             "random_spd_20k": gpu.gen_random_spd(20000, seed=1), "random_spd_40k": gpu.gen_random_spd(40000, seed=2)}
 
 
-@pytest.mark.parametrize("name,natural", [("poisson3d_24", 1), ("convdiff27_32", 2), ("random_spd_20k", 2),
-                                          ("random_spd_40k", 4)])
-def test_gpu_spmv_index_widths_bit_exact(gpu, orc, name, natural):
+def _expected_width(want, fits1, fits2):
+    if want == 0:
+        return 2 if fits2 else 1 if fits1 else 4
+    if want == 1 and fits1:
+        return 1
+    return 2 if want <= 2 and fits2 else 4
+
+
+@pytest.mark.parametrize("name,fits1,fits2", [("poisson3d_24", True, True), ("convdiff27_32", False, True),
+                                              ("random_spd_20k", False, True), ("random_spd_40k", False, False)])
+def test_gpu_spmv_index_widths_bit_exact(gpu, orc, name, fits1, fits2):
     """The one-byte slot, two-byte tile-row-relative and four-byte column
     index forms address the same entries in the same order: every form the
     matrix fits gives the oracle's SpMV bit for bit, FP64 and mixed."""
@@ -86,12 +95,12 @@ def test_gpu_spmv_index_widths_bit_exact(gpu, orc, name, natural):
     x = np.random.default_rng(4).standard_normal(m)
     y64, ymix = t.spmv(x), t.spmv_mixed(codes, x)
     A = gpu.Matrix(rp, ci, va)
-    for w in (1, 2, 4):
+    for w in (0, 1, 2, 4):
         A.set_index_width(w)
         A.layout(2.0 ** -11)
         got = A.info()["index_width"]
-        print(f"case {name} min width {w} -> {got}", flush=True)
-        assert got == max(w, natural)
+        print(f"case {name} width {w} -> {got}", flush=True)
+        assert got == _expected_width(w, fits1, fits2)
         assert np.array_equal(A.spmv(x, mixed=False), y64)
         assert np.array_equal(A.spmv(x, mixed=True), ymix)
     A.close()

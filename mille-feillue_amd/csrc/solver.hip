@@ -37,14 +37,15 @@ namespace mf {
 
 // ------------------------------------------------------------ SpMV kernel
 // 256 threads = 256/B lane groups, lane = row inside the tile row.  A block
-// covers SPMV_NR * 256/B consecutive tile rows; lane group q handles tile
-// rows base + k * 256/B + q (k < SPMV_NR) together.
-constexpr int SPMV_NR = 2;
-template <int B, int IW>
+// covers NR * 256/B consecutive tile rows; lane group q handles tile rows
+// base + k * 256/B + q (k < NR) together.  NR = 2 for layouts with narrow
+// tiles and one- or two-byte indices, 1 otherwise (measured fastest at 128^3
+// Poisson and the 96^3 27-point stencil).
+template <int B, int IW, int NR>
 __global__ __launch_bounds__(256) void k_spmv(DevL L, int tilem, const double *__restrict__ g64,
                                               const float *__restrict__ g32, double *__restrict__ y)
 {
-    constexpr int GPB = 256 / B, TRB = GPB * SPMV_NR;
+    constexpr int GPB = 256 / B, TRB = GPB * NR;
     __shared__ int4 s_h[2 * TRB];
     __shared__ int s_t0[TRB + 1];
     const int tb = xcd_chunk(blockIdx.x, gridDim.x) * TRB;
@@ -60,10 +61,10 @@ __global__ __launch_bounds__(256) void k_spmv(DevL L, int tilem, const double *_
     __syncthreads();
     const int grp = threadIdx.x / B, r = threadIdx.x % B;
     const int lanebase = (threadIdx.x & 63) & ~(B - 1);
-    double v[SPMV_NR];
-    tile_rows_spmv<B, IW, SPMV_NR, GPB>(L, s_h, s_t0, grp, tb + grp, r, lanebase, g64, g32, v);
+    double v[NR];
+    tile_rows_spmv<B, IW, NR, GPB>(L, s_h, s_t0, grp, tb + grp, r, lanebase, g64, g32, v);
 #pragma unroll
-    for (int k = 0; k < SPMV_NR; k++) {
+    for (int k = 0; k < NR; k++) {
         const int tr = tb + k * GPB + grp;
         if (tr < tilem) y[(size_t)tr * B + r] = v[k];
     }
@@ -864,6 +865,20 @@ static int launch_inner(mf_matrix m, const Layout &L, const Layout *L64, int met
                              "exit spread %.2f us\n", PHASE_EP0 + e, (a1 - a0) / 100.0, ((double)x0 - (double)a1) / 100.0,
                              (x1 - x0) / 100.0);
             }
+            // MF_PHASE_PROFILE=2: per workgroup, the time from leaving one
+            // stamped sync to arriving at the next (its own work between them)
+            if (std::atoi(std::getenv("MF_PHASE_PROFILE")) == 2) {
+                for (int e = 1; e < 4; e++) {
+                    const unsigned long long *p = sv.data() + (size_t)e * 2 * G, *q = p - 2 * G;
+                    if (!p[0] || !q[0]) continue;
+                    for (int g = 0; g < G; g++) {
+                        const int per = G >> 3, rem = G & 7, xcd = g & 7, k = g >> 3;
+                        const int chunk = (xcd < rem ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per) + k;
+                        std::fprintf(stderr, "[mf phase wg] sync %u wg %d xcd %d chunk %d work %.2f us\n",
+                                     PHASE_EP0 + e, g, xcd, chunk, ((double)p[2 * g] - (double)q[2 * g + 1]) / 100.0);
+                    }
+                }
+            }
         }
     }
     return MF_OK;
@@ -1003,14 +1018,17 @@ int spmv_device(mf_matrix m, const Layout &L, const double *d_x, double *d_y, bo
     if (rc) return rc;
     if (L.haslow && convert)
         k_to_f32<<<(unsigned)((ncol + 255) / 256), 256, 0, st>>>(d_x, m->ws.g32, ncol);
-    const int nb = (int)(((long long)m->tilem * B + 256 * SPMV_NR - 1) / (256 * SPMV_NR));
+    const int nr = L.haslow && L.iw < 4 ? 2 : 1;
+    const int nb = (int)(((long long)m->tilem * B + 256 * nr - 1) / (256 * nr));
     DevL V = L.view(m->d_tile_ptr);
-#define MF_SPMV(BB, W) k_spmv<BB, W><<<nb, 256, 0, st>>>(V, m->tilem, d_x, m->ws.g32, d_y)
-#define MF_SPMV_B(BB) if (L.iw == 1) MF_SPMV(BB, 1); else if (L.iw == 2) MF_SPMV(BB, 2); else MF_SPMV(BB, 4)
+#define MF_SPMV(BB, W, NR) k_spmv<BB, W, NR><<<nb, 256, 0, st>>>(V, m->tilem, d_x, m->ws.g32, d_y)
+#define MF_SPMV_W(BB, NR) if (L.iw == 1) MF_SPMV(BB, 1, NR); else if (L.iw == 2) MF_SPMV(BB, 2, NR); else MF_SPMV(BB, 4, NR)
+#define MF_SPMV_B(BB) if (nr == 2) { MF_SPMV_W(BB, 2); } else { MF_SPMV_W(BB, 1); }
     if (B == 16) { MF_SPMV_B(16); }
     else if (B == 8) { MF_SPMV_B(8); }
     else { MF_SPMV_B(32); }
 #undef MF_SPMV_B
+#undef MF_SPMV_W
 #undef MF_SPMV
     MF_HIP(hipGetLastError());
     return MF_OK;

@@ -189,6 +189,20 @@ def main():
     spmvmx = A.spmv_bench(50, mixed=True)
     b64, bmx = info["bytes_alg_fp64"], info["bytes_alg_mixed"]
 
+    # SpMV phase inside the persistent solver: one more (untimed) solve of
+    # each kind with workgroup 0's phase clock on
+    def phase(precision, balg):
+        opts = dict(method=mf.MF_CG, tol=TOL, precision=precision, profile=1)
+        if precision == mf.MF_MIXED:
+            opts.update(tau=0.0, rr_factor=args.rr)
+        r = A.solve_device(d_b.data_ptr(), d_x.data_ptr(), **opts)
+        us = r["phase_us"]
+        return {"us_per_iteration": {"spmv": round(us[0], 2), "sync_after_spmv": round(us[1], 2),
+                                     "update": round(us[2], 2), "sync_after_update": round(us[3], 2)},
+                "spmv_GBps": round(balg / us[0] / 1e3, 1), "spmv_frac": round(balg / (us[0] * 1e-6) / HBM_PEAK, 4)}
+    phases = {"fp64": phase(mf.MF_FP64, b64), "mixed": phase(mf.MF_MIXED, bmx)}
+    log(f"spmv phase in the solver: fp64 {phases['fp64']['spmv_frac']:.3f}, mixed {phases['mixed']['spmv_frac']:.3f} of peak")
+
     # dominant kernel: the persistent CG launch.  Unit = one CG iteration,
     # algorithmic bytes B_alg + 56 n (the SpMV plus x, r, d read and written
     # and q re-read by a kernel that streams its vectors); true-residual
@@ -256,6 +270,7 @@ def main():
             "mixed": {"ms": round(spmvmx, 5), "GB/s": round(bmx / spmvmx / 1e6, 1),
                       "frac": round(bmx / (spmvmx * 1e-3) / HBM_PEAK, 4), "bytes_alg": bmx},
         },
+        "spmv_phase_in_solver": phases,
         "solver_GBps": {"fp64": round(it64 * (b64 + 56.0 * n) / ms64 / 1e6, 1), "mixed": round(ach / 1e9, 1)},
         "roofline": roof,
         "batched": batched,

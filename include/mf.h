@@ -171,6 +171,8 @@ typedef struct {
     int index_width;    /* slab index width, as mf_matrix_set_index_width
                            (0 = the matrix's setting; batched: 0 = automatic,
                            one width that every system fits) */
+    int profile;        /* CG: 1 = time the phases of the persistent launch
+                           (workgroup 0's clock) into mf_report.phase_us */
 } mf_opts;
 void mf_opts_default(mf_opts *o);
 
@@ -183,6 +185,9 @@ typedef struct {
     double ms_solve;     /* time to tolerance (device events, inputs resident) */
     double ms_setup;     /* layout build */
     int grid, block, rows_per_thread;
+    double phase_us[5];  /* opts.profile (CG): per iteration SpMV, sync after it,
+                            vector update, sync after it; [4] = true-residual
+                            passes, total; of the last persistent launch */
 } mf_report;
 
 /* Host buffers in/out (copies included; the timed ms_solve excludes them). */

@@ -53,17 +53,19 @@ class Opts(C.Structure):
     _fields_ = [("method", C.c_int), ("precision", C.c_int), ("tol", C.c_double), ("maxit", C.c_int),
                 ("tau", C.c_double), ("allow_mask", C.c_int), ("rr_factor", C.c_double),
                 ("ir_max_outer", C.c_int), ("ir_inner_tol", C.c_double), ("grid", C.c_int),
-                ("block", C.c_int), ("spin_limit", C.c_longlong), ("index_width", C.c_int)]
+                ("block", C.c_int), ("spin_limit", C.c_longlong), ("index_width", C.c_int), ("profile", C.c_int)]
 
 
 class Report(C.Structure):
     _fields_ = [("status", C.c_int), ("iterations", C.c_int), ("outer", C.c_int),
                 ("replacements", C.c_int), ("relres_true", C.c_double), ("ms_solve", C.c_double),
                 ("ms_setup", C.c_double), ("grid", C.c_int), ("block", C.c_int),
-                ("rows_per_thread", C.c_int)]
+                ("rows_per_thread", C.c_int), ("phase_us", C.c_double * 5)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["phase_us"] = list(self.phase_us)
+        return d
 
 
 class Compact(C.Structure):

@@ -746,6 +746,7 @@ static int ensure_ws(mf_matrix m, int G)
 struct InnerResult {
     int status = MF_OK, iterations = 0, replacements = 0;
     double bnorm = 0, rt2 = 0, g2 = 0;
+    double phase_us[5] = {0, 0, 0, 0, 0};  // CG, block 0: per iteration x4, passes total
 };
 
 // Zeroes the barrier counters (global + 8 groups, one 128-byte line apart)
@@ -802,8 +803,9 @@ static int launch_inner(mf_matrix m, const Layout &L, const Layout *L64, int met
     static const bool phase_profile = std::getenv("MF_PHASE_PROFILE") != nullptr;
     static unsigned long long *d_phase = nullptr;
     const size_t phase_words = 16 + 8 * 1024;
-    if (phase_profile && !d_phase) MF_HIP(hipMalloc((void **)&d_phase, phase_words * sizeof(unsigned long long)));
-    A.phase = phase_profile ? d_phase : nullptr;
+    const bool prof = phase_profile || (o.profile && method == MF_CG);
+    if (prof && !d_phase) MF_HIP(hipMalloc((void **)&d_phase, phase_words * sizeof(unsigned long long)));
+    A.phase = prof ? d_phase : nullptr;
     if (A.phase) MF_HIP(hipMemsetAsync(d_phase, 0, phase_words * sizeof(unsigned long long), st));
     k_reset_ctl<<<1, 1024, 0, st>>>(w.slots, 2 * SYNC_K * 2 * G, w.ctl);
     MF_HIP(hipGetLastError());
@@ -841,6 +843,13 @@ static int launch_inner(mf_matrix m, const Layout &L, const Layout *L64, int met
         res->rt2 = out[1];
         res->g2 = out[2];
         if (A.phase) {
+            unsigned long long ph[8];
+            MF_HIP(hipMemcpy(ph, A.phase, sizeof(ph), hipMemcpyDeviceToHost));
+            const double it = std::max(1, res->iterations);
+            for (int k = 0; k < 4; k++) res->phase_us[k] = ph[k] / it / 100.0;
+            res->phase_us[4] = ph[4] / 100.0;
+        }
+        if (A.phase && phase_profile) {
             unsigned long long ph[8];
             MF_HIP(hipMemcpy(ph, A.phase, sizeof(ph), hipMemcpyDeviceToHost));
             const double it = std::max(1, res->iterations);
@@ -1005,6 +1014,7 @@ int solve_device(mf_matrix m, const double *d_b, double *d_x, const mf_opts &o, 
     rep->grid = G;
     rep->block = TPB;
     rep->rows_per_thread = RPT;
+    for (int k = 0; k < 5; k++) rep->phase_us[k] = ir.phase_us[k];
     return status == MF_E_TIMEOUT ? fail(MF_E_TIMEOUT, "grid barrier timeout") : MF_OK;
 }
 

@@ -279,3 +279,21 @@ def test_gpu_dropin_cg_solve_inc_csv(gpu, tmp_path):
     # path,iterations,time,time,nnzR,l2_norm,time_format,index,each_block_nnz,norm
     assert len(row) == 10 and row[0] == mtx and int(row[1]) > 0 and int(row[4]) == 4054
     assert float(row[5]) <= TOL
+
+
+def test_gpu_cg_phase_profile(gpu, orc):
+    # the phase clock of workgroup 0 accounts for the solve: the four phases
+    # per iteration times the iterations plus the passes come to the launch
+    # time within its set-up and tail
+    rp, ci, va = gpu.gen_poisson3d(48, 48, 48)
+    b = orc.csr_spmv(rp, ci, va, gpu.gen_uniform(7, len(rp) - 1, -1, 1))
+    A = gpu.Matrix(rp, ci, va)
+    x, rep = A.solve(b, method=gpu.MF_CG, tol=TOL, profile=1)
+    ph = rep["phase_us"]
+    total_ms = (sum(ph[:4]) * rep["iterations"] + ph[4]) * 1e-3
+    print(f"phases {ph} it={rep['iterations']} sum={total_ms:.3f} ms solve={rep['ms_solve']:.3f} ms", flush=True)
+    assert rep["status"] == 0 and all(p > 0 for p in ph[:4])
+    assert 0.7 * rep["ms_solve"] < total_ms <= rep["ms_solve"]
+    _, rep0 = A.solve(b, method=gpu.MF_CG, tol=TOL)
+    assert rep0["phase_us"] == [0.0] * 5
+    A.close()

@@ -70,14 +70,16 @@ struct GroupSync {
     int gi, gn, gw0;  // index in group, group size, first workgroup of the group
     unsigned ep = 0;
 
-    template <int N>
-    __device__ bool allreduce(const BatchArgs &A, double (&v)[N], bool publish)
+    // value(k), k < N: this thread's partial of sum k; the sums are left in
+    // s_bc (read them there before the next allreduce)
+    template <int N, typename F>
+    __device__ bool allreduce_f(const BatchArgs &A, F value, bool publish)
     {
         static_assert(N <= BK_MAXV, "group reducer width");
         const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
 #pragma unroll
         for (int k = 0; k < N; k++) {
-            const double s = wave_sum(v[k]);
+            const double s = wave_sum(value(k));
             if (lane == 0) s_red[wv * BK_MAXV + k] = s;
         }
         if (publish && gn > 1) __builtin_amdgcn_s_waitcnt(0);
@@ -127,52 +129,53 @@ struct GroupSync {
             if (lane == 0) *s_flag = ab;
         }
         __syncthreads();
-        if (*s_flag) return false;
-#pragma unroll
-        for (int k = 0; k < N; k++) v[k] = s_bc[k];
-        return true;
+        return !*s_flag;
+    }
+
+    template <int N>
+    __device__ bool allreduce(const BatchArgs &A, const double (&v)[N], bool publish)
+    {
+        return allreduce_f<N>(A, [&](int k) { return v[k]; }, publish);
     }
 };
 
-// K consecutive entries of a [row][K] vector
-template <int K>
-__device__ __forceinline__ void ldK(const double *__restrict__ g, uint32_t col, double (&o)[K])
+// Entries k0 .. k0 + KL - 1 of row col of a [row][K] vector (KL = 1, 2 or 8;
+// k0 is a multiple of KL, so the vector loads are aligned).
+template <int K, int KL>
+__device__ __forceinline__ void ldKL(const double *__restrict__ g, uint32_t col, int k0, double (&o)[KL])
 {
-    if constexpr (K == 1) {
-        o[0] = g[col];
+    const double *p = g + (size_t)col * K + k0;
+    if constexpr (KL == 1) {
+        o[0] = *p;
     } else {
-        const double2 *q = reinterpret_cast<const double2 *>(g + (size_t)col * K);
 #pragma unroll
-        for (int j = 0; j < K / 2; j++) {
-            const double2 d = q[j];
+        for (int j = 0; j < KL / 2; j++) {
+            const double2 d = reinterpret_cast<const double2 *>(p)[j];
             o[2 * j] = d.x;
             o[2 * j + 1] = d.y;
         }
     }
 }
-template <int K>
-__device__ __forceinline__ void ldK(const float *__restrict__ g, uint32_t col, float (&o)[K])
+template <int K, int KL>
+__device__ __forceinline__ void ldKL(const float *__restrict__ g, uint32_t col, int k0, float (&o)[KL])
 {
-    if constexpr (K % 4 == 0) {
-        const float4 *q = reinterpret_cast<const float4 *>(g + (size_t)col * K);
-#pragma unroll
-        for (int j = 0; j < K / 4; j++) {
-            const float4 d = q[j];
-            o[4 * j] = d.x;
-            o[4 * j + 1] = d.y;
-            o[4 * j + 2] = d.z;
-            o[4 * j + 3] = d.w;
-        }
+    const float *p = g + (size_t)col * K + k0;
+    if constexpr (KL == 1) {
+        o[0] = *p;
     } else {
 #pragma unroll
-        for (int j = 0; j < K; j++) o[j] = g[(size_t)col * K + j];
+        for (int j = 0; j < KL / 2; j++) {
+            const float2 d = reinterpret_cast<const float2 *>(p)[j];
+            o[2 * j] = d.x;
+            o[2 * j + 1] = d.y;
+        }
     }
 }
 
-// One class slab of this lane's row applied to K columns.
-template <int B, int IW, int CLS, int K, typename AccT, typename GT>
-__device__ __forceinline__ void slabK(const DevL &L, int2 h, int tcr, int t0, int r, int lanebase,
-                                      const GT *__restrict__ g, AccT (&acc)[K])
+// One class slab of this lane's row applied to its KL columns k0 .. k0 + KL - 1.
+template <int B, int IW, int CLS, int K, int KL, typename AccT, typename GT>
+__device__ __forceinline__ void slabK(const DevL &L, int2 h, int tcr, int t0, int r, int lanebase, int k0,
+                                      const GT *__restrict__ g, AccT (&acc)[KL])
 {
     const int nq = (h.y + 3) >> 2;
     const uint8_t *vals = L.blob + (size_t)(unsigned)h.x * 8;
@@ -182,12 +185,14 @@ __device__ __forceinline__ void slabK(const DevL &L, int2 h, int tcr, int t0, in
         IdxQuad<IW> id;
         v.load(vals, (size_t)j * B + r);
         id.load(idx, (size_t)j * B + r);
-        GT xg[4][K];
-        static_for<4>([&](auto e) { ldK<K>(g, gather_col<B, IW>(id.template get<e>(), tcr, t0, L, lanebase), xg[e]); });
+        GT xg[4][KL];
+        static_for<4>([&](auto e) {
+            ldKL<K, KL>(g, gather_col<B, IW>(id.template get<e>(), tcr, t0, L, lanebase), k0, xg[e]);
+        });
         static_for<4>([&](auto e) {
             const AccT a = (AccT)v.template get<e>();
 #pragma unroll
-            for (int k = 0; k < K; k++) acc[k] = fma_t(a, (AccT)xg[e][k], acc[k]);
+            for (int k = 0; k < KL; k++) acc[k] = fma_t(a, (AccT)xg[e][k], acc[k]);
         });
     }
 }
@@ -199,41 +204,43 @@ __device__ __forceinline__ int row_tcr(const DevL &L, int tr, int t0, int nt, in
     return IW == 1 ? (r < nt ? L.stc[t0 + r] * B : 0) : IW == 2 ? tr * B : 0;
 }
 
-// y = A_row g for K columns: FP64 tiles in FP64 over g64, the other classes
-// in FP32 over g32, as the single-vector SpMV.
-template <int B, int IW, int K>
-__device__ __forceinline__ void row_spmvK(const DevL &L, int tr, int r, int lanebase, const double *__restrict__ g64,
-                                          const float *__restrict__ g32, double (&y)[K])
+// y = A_row g for the lane's KL columns: FP64 tiles in FP64 over g64, the
+// other classes in FP32 over g32, as the single-vector SpMV.
+template <int B, int IW, int K, int KL>
+__device__ __forceinline__ void row_spmvK(const DevL &L, int tr, int r, int lanebase, int k0,
+                                          const double *__restrict__ g64, const float *__restrict__ g32,
+                                          double (&y)[KL])
 {
     const int4 *h = reinterpret_cast<const int4 *>(L.trh + (size_t)tr * 4);
     const int4 h01 = h[0], h23 = h[1];
     const int t0 = L.tile_ptr[tr], nt = L.tile_ptr[tr + 1] - t0;
     const int tcr = row_tcr<IW, B>(L, tr, t0, nt, r);
-    double a64[K];
-    float a32[K];
+    double a64[KL];
+    float a32[KL];
 #pragma unroll
-    for (int k = 0; k < K; k++) {
+    for (int k = 0; k < KL; k++) {
         a64[k] = 0.0;
         a32[k] = 0.0f;
     }
-    if (h01.y) slabK<B, IW, 0, K>(L, make_int2(h01.x, h01.y), tcr, t0, r, lanebase, g64, a64);
-    if (h01.w) slabK<B, IW, 1, K>(L, make_int2(h01.z, h01.w), tcr, t0, r, lanebase, g32, a32);
-    if (h23.y) slabK<B, IW, 2, K>(L, make_int2(h23.x, h23.y), tcr, t0, r, lanebase, g32, a32);
-    if (h23.w) slabK<B, IW, 3, K>(L, make_int2(h23.z, h23.w), tcr, t0, r, lanebase, g32, a32);
+    if (h01.y) slabK<B, IW, 0, K, KL>(L, make_int2(h01.x, h01.y), tcr, t0, r, lanebase, k0, g64, a64);
+    if (h01.w) slabK<B, IW, 1, K, KL>(L, make_int2(h01.z, h01.w), tcr, t0, r, lanebase, k0, g32, a32);
+    if (h23.y) slabK<B, IW, 2, K, KL>(L, make_int2(h23.x, h23.y), tcr, t0, r, lanebase, k0, g32, a32);
+    if (h23.w) slabK<B, IW, 3, K, KL>(L, make_int2(h23.z, h23.w), tcr, t0, r, lanebase, k0, g32, a32);
 #pragma unroll
-    for (int k = 0; k < K; k++) y[k] = a64[k] + (double)a32[k];
+    for (int k = 0; k < KL; k++) y[k] = a64[k] + (double)a32[k];
 }
 
-// A x and A p on the FP64 layout (class 0 only) for K columns each.
-template <int B, int IW, int K>
-__device__ __forceinline__ void row_spmv64_xp(const DevL &L, int tr, int r, int lanebase, const double *__restrict__ gx,
-                                              const double *__restrict__ gp, double (&ax)[K], double (&ap)[K])
+// A x and A p on the FP64 layout (class 0 only) for the lane's KL columns.
+template <int B, int IW, int K, int KL>
+__device__ __forceinline__ void row_spmv64_xp(const DevL &L, int tr, int r, int lanebase, int k0,
+                                              const double *__restrict__ gx, const double *__restrict__ gp,
+                                              double (&ax)[KL], double (&ap)[KL])
 {
     const int2 h = reinterpret_cast<const int2 *>(L.trh)[(size_t)tr * 4];
     const int t0 = L.tile_ptr[tr], nt = L.tile_ptr[tr + 1] - t0;
     const int tcr = row_tcr<IW, B>(L, tr, t0, nt, r);
 #pragma unroll
-    for (int k = 0; k < K; k++) ax[k] = ap[k] = 0.0;
+    for (int k = 0; k < KL; k++) ax[k] = ap[k] = 0.0;
     const int nq = (h.y + 3) >> 2;
     const uint8_t *vals = L.blob + (size_t)(unsigned)h.x * 8;
     const uint8_t *idx = vals + (size_t)nq * B * 4 * 8;
@@ -244,12 +251,12 @@ __device__ __forceinline__ void row_spmv64_xp(const DevL &L, int tr, int r, int 
         id.load(idx, (size_t)j * B + r);
         static_for<4>([&](auto e) {
             const uint32_t c = gather_col<B, IW>(id.template get<e>(), tcr, t0, L, lanebase);
-            double vx[K], vp[K];
-            ldK<K>(gx, c, vx);
-            ldK<K>(gp, c, vp);
+            double vx[KL], vp[KL];
+            ldKL<K, KL>(gx, c, k0, vx);
+            ldKL<K, KL>(gp, c, k0, vp);
             const double a = v.template get<e>();
 #pragma unroll
-            for (int k = 0; k < K; k++) {
+            for (int k = 0; k < KL; k++) {
                 ax[k] = __builtin_fma(a, vx[k], ax[k]);
                 ap[k] = __builtin_fma(a, vp[k], ap[k]);
             }
@@ -277,13 +284,23 @@ struct BatchShared {
 template <int B, int IW, int K, int TPB>
 __device__ bool cg_system(const BatchArgs &A, BatchShared<K, TPB> &sh, int sys, int tr0, int tr1, int gw0, int gn)
 {
-    constexpr int GPB = TPB / B;
+    // K = 8: four lanes per row, two columns each (the lanes of a row read the
+    // same slab quads, which one wave fetches once); K = 1: one lane per row
+    // SpMV phase: one lane per row, all K columns (fewer memory instructions
+    // per FMA than splitting the columns, which measured slower).  The FP64
+    // true-residual pass, which needs A x and A p, splits the K = 8 columns
+    // over four lanes of a row (two each) to keep its registers below the
+    // SpMV's.
+    constexpr int GPB = TPB / B;                      // tile rows per sweep
+    constexpr int CGP = K == 8 ? 4 : 1, KLP = K / CGP;  // pass: lanes per row, columns per lane
+    constexpr int GPBP = TPB / (B * CGP);
     double *sh_g = sh.g, *sh_gp = sh.gp, *sh_ap = sh.ap, *sh_ref = sh.ref, *sh_b2 = sh.b2, *sh_rs = sh.rs,
            *sh_ps = sh.ps, *sh_al = sh.al, *sh_be = sh.be;
     int *sh_mode = sh.mode;
     int &sh_bd = sh.bd;
     const int g = blockIdx.x, tid = threadIdx.x;
     const int grp = tid / B, lane = tid % B, lanebase = (tid & 63) & ~(B - 1);
+    const int grpp = tid / (B * CGP), k0p = ((tid / B) % CGP) * KLP;  // pass mapping
     GroupSync<TPB> S{sh.red, sh.bc, &sh.flag, g - gw0, gn, gw0};
     const int KA = A.nrhs;
     const double tol2 = A.tol * A.tol, rr2 = A.rr_factor * A.rr_factor;
@@ -314,7 +331,7 @@ __device__ bool cg_system(const BatchArgs &A, BatchShared<K, TPB> &sh, int sys, 
     }
     if (!S.allreduce(A, v, true)) return false;
     if (tid < K) {
-        sh_b2[tid] = sh_g[tid] = sh_ref[tid] = v[tid];
+        sh_b2[tid] = sh_g[tid] = sh_ref[tid] = sh.bc[tid];
         sh_gp[tid] = sh_ap[tid] = 1.0;
         sh_mode[tid] = 1;
     }
@@ -342,42 +359,51 @@ __device__ bool cg_system(const BatchArgs &A, BatchShared<K, TPB> &sh, int sys, 
         if (conv || due || force) {
             // r = b - A x and s = A p on the FP64 layout; (r,r), (r,s), (p,s)
             if (!S.allreduce(A, v, true)) return false;  // x and p of every row visible
-            double t[3 * K];
+            double t0[KLP], t1[KLP], t2[KLP];
 #pragma unroll
-            for (int k = 0; k < 3 * K; k++) t[k] = 0.0;
-            for (int tr = tr0 + grp; tr < tr1; tr += GPB) {
+            for (int j = 0; j < KLP; j++) t0[j] = t1[j] = t2[j] = 0.0;
+            for (int tr = tr0 + grpp; tr < tr1; tr += GPBP) {
                 const size_t row = (size_t)tr * B + lane;
-                double ax[K], ap[K];
-                row_spmv64_xp<B, IW, K>(A.L64, tr, lane, lanebase, A.x, A.p, ax, ap);
+                double ax[KLP], ap[KLP];
+                row_spmv64_xp<B, IW, K, KLP>(A.L64, tr, lane, lanebase, k0p, A.x, A.p, ax, ap);
 #pragma unroll
-                for (int k = 0; k < K; k++) {
-                    const size_t e = row * K + k;
-                    const double rv = A.b[e] - ax[k], pv = A.p[e];
+                for (int j = 0; j < KLP; j++) {
+                    const size_t e = row * K + k0p + j;
+                    const double rv = A.b[e] - ax[j], pv = A.p[e];
                     st_ag(A.r + e, rv);
                     if (A.haslow) st_ag(A.r32 + e, (float)rv);
-                    A.s[e] = ap[k];
-                    t[k] += rv * rv;
-                    t[K + k] += rv * ap[k];
-                    t[2 * K + k] += pv * ap[k];
+                    A.s[e] = ap[j];
+                    t0[j] += rv * rv;
+                    t1[j] += rv * ap[j];
+                    t2[j] += pv * ap[j];
                 }
             }
-            if (!S.allreduce(A, t, true)) return false;
+            // sum k: column k % K of (r, r), (r, s), (p, s) for k / K = 0, 1, 2
+            auto tv = [&](int k) {
+                const int c = k % K, j = c - k0p, q = k / K;
+                double v = 0.0;
+#pragma unroll
+                for (int i = 0; i < KLP; i++)
+                    if (j == i) v = q == 0 ? t0[i] : q == 1 ? t1[i] : t2[i];
+                return v;
+            };
+            if (!S.template allreduce_f<3 * K>(A, tv, true)) return false;
             lap(3);
             ++repl;
             bool done = true;
             double rel = 0.0;
             for (int k = 0; k < KA; k++) {
-                done = done && t[k] <= tol2 * sh_b2[k];
-                if (sh_b2[k] > 0.0) rel = fmax(rel, sqrt(t[k] / sh_b2[k]));
+                done = done && sh.bc[k] <= tol2 * sh_b2[k];
+                if (sh_b2[k] > 0.0) rel = fmax(rel, sqrt(sh.bc[k] / sh_b2[k]));
             }
             if (!isfinite(rel)) { finish(MF_E_NONFINITE, iter, repl, rel); return true; }
             if (done) { finish(MF_OK, iter, repl, rel); return true; }
             if (tid < K) {
-                const bool restart = force || t[tid] > 4.0 * sh_g[tid];
+                const bool restart = force || sh.bc[tid] > 4.0 * sh_g[tid];
                 sh_mode[tid] = (restart ? 1 : 0) | 2;
-                sh_rs[tid] = t[K + tid];
-                sh_ps[tid] = t[2 * K + tid];
-                sh_g[tid] = sh_ref[tid] = t[tid];
+                sh_rs[tid] = sh.bc[K + tid];
+                sh_ps[tid] = sh.bc[2 * K + tid];
+                sh_g[tid] = sh_ref[tid] = sh.bc[tid];
             }
             force = false;
             __syncthreads();
@@ -406,7 +432,7 @@ __device__ bool cg_system(const BatchArgs &A, BatchShared<K, TPB> &sh, int sys, 
         for (int tr = tr0 + grp; tr < tr1; tr += GPB) {
             const size_t row = (size_t)tr * B + lane;
             double w[K];
-            row_spmvK<B, IW, K>(A.L, tr, lane, lanebase, A.r, A.r32, w);
+            row_spmvK<B, IW, K, K>(A.L, tr, lane, lanebase, 0, A.r, A.r32, w);
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 const size_t e = row * K + k;
@@ -420,9 +446,10 @@ __device__ bool cg_system(const BatchArgs &A, BatchShared<K, TPB> &sh, int sys, 
         if (tid < K) {
             const int m = sh_mode[tid];
             const double be = sh_be[tid], gk = sh_g[tid];
-            const double denom = (m & 4)   ? d[tid]
-                                 : (m & 8) ? d[tid] + be * (2.0 * sh_rs[tid] + be * sh_ps[tid])
-                                           : d[tid] - be * gk / sh_ap[tid];
+            const double dk = sh.bc[tid];
+            const double denom = (m & 4)   ? dk
+                                 : (m & 8) ? dk + be * (2.0 * sh_rs[tid] + be * sh_ps[tid])
+                                           : dk - be * gk / sh_ap[tid];
             if (m & 16) {
                 sh_al[tid] = 0.0;
             } else if (!(denom > 0.0) || !isfinite(denom)) {
@@ -484,7 +511,7 @@ __device__ bool cg_system(const BatchArgs &A, BatchShared<K, TPB> &sh, int sys, 
                 sh_gp[tid] = sh_g[tid];
                 sh_ap[tid] = sh_al[tid];
             }
-            sh_g[tid] = gg[tid];
+            sh_g[tid] = sh.bc[tid];
             sh_mode[tid] = 0;
         }
         __syncthreads();

@@ -42,7 +42,7 @@ struct BatchArgs {
     float *r32;                           // FP32 copy of r for low-precision tiles
     unsigned long long *slots;            // sync words [2][G][2 * BK_MAXV]
     int *ctl;                             // [0] abort word
-    int *sys_out;                         // [4 s + 0] status, +1 iterations, +2 replacements
+    int *sys_out;                         // [4 s + 0] status, +1 iterations, +2 replacements, +3 time (us)
     double *sys_relres;                   // max over columns of the true relative residual
     double tol, rr_factor;
     int maxit, haslow;
@@ -304,11 +304,14 @@ __device__ bool cg_system(const BatchArgs &A, BatchShared<K, TPB> &sh, int sys, 
     GroupSync<TPB> S{sh.red, sh.bc, &sh.flag, g - gw0, gn, gw0};
     const int KA = A.nrhs;
     const double tol2 = A.tol * A.tol, rr2 = A.rr_factor * A.rr_factor;
+    const unsigned long long t_start = wall_clock64();
     auto finish = [&](int status, int iter, int repl, double relres) {
         if (g == gw0 && tid == 0) {
             A.sys_out[4 * sys] = status;
             A.sys_out[4 * sys + 1] = iter;
             A.sys_out[4 * sys + 2] = repl;
+            // time of the system on its first workgroup (us; the clock runs at 100 MHz)
+            A.sys_out[4 * sys + 3] = (int)min((wall_clock64() - t_start) / 100ull, 0x7fffffffull);
             A.sys_relres[sys] = relres;
         }
     };
@@ -597,6 +600,7 @@ struct mf_batch_s {
     std::vector<std::vector<int>> h_rp, h_ci;  // host copies of the systems (rebuilds)
     std::vector<std::vector<double>> h_va;
     std::vector<int> weights;                  // iteration weights of the assignment
+    std::vector<double> wtime;                 // measured work per system (us on one workgroup), or empty
     std::string tuned;                         // what the tuner chose
     std::vector<long long> tbase;
     std::vector<double> tau;
@@ -702,56 +706,98 @@ static int concat_layouts(mf_batch_s *bt, const std::vector<mf_matrix> &mats, bo
     return MF_OK;
 }
 
-// Work assignment: a system whose work exceeds 1.5 workgroup shares gets a
-// group of its own (one task per workgroup of the group, tile rows split
-// evenly); the other systems are packed onto the remaining workgroups,
-// largest first onto the least loaded one, and run one after another.
-// weights: per-system iteration counts of an earlier solve (work per
-// iteration x iterations), or empty (work per iteration only).
+// Work assignment.  Work of system s = iterations x (bytes per iteration +
+// a fixed per-iteration cost); a system split over w > 1 workgroups also pays
+// its group syncs every iteration.  For a bound T on the work of one
+// workgroup, every system above T gets the fewest workgroups that bring its
+// share (plus the sync cost) under T, and the others are packed largest
+// first onto the least loaded workgroup (after the split system on a split
+// system's workgroup); bisection finds the smallest T that fits g_target
+// workgroups, and that assignment is used.
+// iters: per-system iteration counts of an earlier solve (or estimates).
 static int batch_assign(mf_batch_s *bt, int g_target, const std::vector<int> &iters)
 {
-    std::vector<double> work(bt->nsys);
+    // work in microseconds of one workgroup: measured (wtime) when a solve
+    // has timed every system, else the byte model at about 2.5 KB per us plus
+    // ~3 us per iteration of scalar work and block reductions; a split
+    // system's three group syncs cost ~9 us per iteration
+    constexpr double RATE = 2.5e3, C_ITER = 3.0, C_SYNC = 9.0;
+    const int n_sys = bt->nsys;
+    const bool measured = (int)bt->wtime.size() == n_sys;
+    std::vector<double> work(n_sys), its(n_sys);
     double total = 0.0;
-    for (int s = 0; s < bt->nsys; s++) {
-        work[s] = 12.0 * bt->nnz[s] + 48.0 * bt->K * bt->tilem[s] * bt->B;
-        if (!iters.empty()) work[s] *= std::max(1, iters[s]);
+    for (int s = 0; s < n_sys; s++) {
+        its[s] = iters.empty() ? 1.0 : std::max(1, iters[s]);
+        work[s] = measured ? std::max(bt->wtime[s], 1.0)
+                           : its[s] * ((12.0 * bt->nnz[s] + 48.0 * bt->K * bt->tilem[s] * bt->B) / RATE + C_ITER);
         total += work[s];
     }
     g_target = std::max(1, g_target);
     const int gpb = bt->TPB / bt->B;
-    const double share = total / g_target;
-    bt->sys_nwg.assign(bt->nsys, 1);
-    std::vector<int> big, small;
-    int gbig = 0;
-    for (int s = 0; s < bt->nsys; s++) {
-        int w = work[s] > 1.5 * share ? (int)std::ceil(work[s] / share) : 1;
-        w = std::min(w, std::max(1, bt->tilem[s] / gpb));
-        bt->sys_nwg[s] = w;
-        if (w > 1) {
-            big.push_back(s);
-            gbig += w;
-        } else {
-            small.push_back(s);
+    std::vector<int> order(n_sys);
+    for (int s = 0; s < n_sys; s++) order[s] = s;
+    std::sort(order.begin(), order.end(), [&](int a, int c) { return work[a] > work[c] || (work[a] == work[c] && a < c); });
+    // workgroups of system s under bound T (1 when it fits one workgroup)
+    auto nwg = [&](int s, double T) {
+        if (work[s] <= T) return 1;
+        const double room = T - its[s] * C_SYNC;
+        const int cap = std::max(1, bt->tilem[s] / gpb);
+        if (room <= 0.0) return cap;
+        return std::min(cap, std::max(2, (int)std::ceil(work[s] / room)));
+    };
+    // assignment under T: per-system workgroups, then the other systems
+    // packed largest first onto the least loaded workgroup, the workgroups of
+    // split systems included (there a packed system runs after the split one,
+    // so the group is in step during the split system's syncs); returns the
+    // largest workgroup load and the number of workgroups used
+    std::vector<int> w_of(n_sys), pack_of(n_sys);
+    auto plan = [&](double T) {
+        int gbig = 0, nsmall = 0;
+        for (int s = 0; s < n_sys; s++) {
+            w_of[s] = nwg(s, T);
+            if (w_of[s] > 1) gbig += w_of[s];
+            else nsmall++;
         }
+        if (gbig > g_target) return std::make_pair(1e300, gbig);
+        const int G = std::min(g_target, gbig + nsmall);
+        std::vector<double> load(G, 0.0);
+        int g = 0;
+        for (int s : order)
+            if (w_of[s] > 1)
+                for (int q = 0; q < w_of[s]; q++, g++) load[g] = work[s] / w_of[s] + its[s] * C_SYNC;
+        double worst = 0.0;
+        for (int q = 0; q < G; q++) worst = std::max(worst, load[q]);
+        for (int s : order) {
+            if (w_of[s] > 1) continue;
+            int best = 0;
+            for (int q = 1; q < G; q++)
+                if (load[q] < load[best]) best = q;
+            load[best] += work[s];
+            pack_of[s] = best;
+            worst = std::max(worst, load[best]);
+        }
+        return std::make_pair(G ? worst : 1e300, G);
+    };
+    double lo = total / g_target, hi = std::max(total, lo) * 2.0;
+    for (int it = 0; it < 40; it++) {
+        const double mid = 0.5 * (lo + hi);
+        if (plan(mid).first <= mid) hi = mid;
+        else lo = mid;
     }
-    const int gsmall = small.empty() ? 0 : std::max(1, std::min<int>((int)small.size(), g_target - gbig));
-    const int G = gbig + gsmall;
+    const int G = plan(hi).second;
+    bt->sys_nwg.assign(w_of.begin(), w_of.end());
     std::vector<std::vector<int>> tasks(G);  // per workgroup: task records (5 ints)
     int g = 0;
-    for (int s : big) {
-        const int w = bt->sys_nwg[s];
+    for (int s : order) {
+        const int w = w_of[s];
+        if (w == 1) continue;
         for (int q = 0; q < w; q++, g++)
             tasks[g] = {s, bt->trbase[s] + (int)((long long)bt->tilem[s] * q / w),
                         bt->trbase[s] + (int)((long long)bt->tilem[s] * (q + 1) / w), g - q, w};
     }
-    std::sort(small.begin(), small.end(), [&](int a, int c) { return work[a] > work[c] || (work[a] == work[c] && a < c); });
-    std::vector<double> load(gsmall, 0.0);
-    for (int s : small) {
-        int best = 0;
-        for (int q = 1; q < gsmall; q++)
-            if (load[q] < load[best]) best = q;
-        load[best] += work[s];
-        const int wg = gbig + best;
+    for (int s : order) {
+        if (w_of[s] > 1) continue;
+        const int wg = pack_of[s];
         tasks[wg].insert(tasks[wg].end(), {s, bt->trbase[s], bt->trbase[s] + bt->tilem[s], wg, 1});
     }
     std::vector<int> h(G + 1, 0);
@@ -1044,13 +1090,28 @@ struct BatchOutcome {
     double worst = 0.0;
     std::vector<int> iters;
     std::vector<double> relres;
+    std::vector<double> us;  // per system: time on its first workgroup
 };
+
+// Work of every system on one workgroup, from a solve's per-system times: a
+// system split over w workgroups took (work / w + iterations x sync) there.
+static void batch_measure(mf_batch_s *bt, const BatchOutcome &oc)
+{
+    constexpr double C_SYNC = 9.0;
+    bt->wtime.assign(bt->nsys, 0.0);
+    for (int s = 0; s < bt->nsys; s++) {
+        const int w = s < (int)bt->sys_nwg.size() ? bt->sys_nwg[s] : 1;
+        const double t = oc.us[s];
+        bt->wtime[s] = w > 1 ? std::max(t - oc.iters[s] * C_SYNC, 0.5 * t) * w : t;
+    }
+}
 
 static int batch_outcome(mf_batch_s *bt, BatchOutcome &oc)
 {
     std::vector<int> so(4 * (size_t)bt->nsys);
     oc.relres.assign(bt->nsys, 0.0);
     oc.iters.assign(bt->nsys, 0);
+    oc.us.assign(bt->nsys, 0.0);
     int ab = 0;
     MF_HIP(hipMemcpy(so.data(), bt->d_sys_out, sizeof(int) * so.size(), hipMemcpyDeviceToHost));
     MF_HIP(hipMemcpy(oc.relres.data(), bt->d_relres, sizeof(double) * bt->nsys, hipMemcpyDeviceToHost));
@@ -1060,6 +1121,7 @@ static int batch_outcome(mf_batch_s *bt, BatchOutcome &oc)
     oc.worst = 0.0;
     for (int s = 0; s < bt->nsys; s++) {
         oc.iters[s] = so[4 * s + 1];
+        oc.us[s] = so[4 * s + 3];
         oc.itmax = std::max(oc.itmax, so[4 * s + 1]);
         oc.repl += so[4 * s + 2];
         oc.worst = std::max(oc.worst, oc.relres[s]);
@@ -1124,21 +1186,44 @@ int mf_batch_solve(mf_batch bt, const double *const *b, double *const *x, int *i
                      "busiest workgroup %d: spmv %.0f update %.0f sync %.0f pass %.0f\n",
                      sum[0] / bt->G, sum[1] / bt->G, sum[2] / bt->G, sum[3] / bt->G, gmax, pf[4 * gmax] / 100.0,
                      pf[4 * gmax + 1] / 100.0, pf[4 * gmax + 2] / 100.0, pf[4 * gmax + 3] / 100.0);
+        // MF_PHASE_PROFILE=2: every workgroup's time and tasks (system, rows,
+        // iterations, group size, measured time of the system)
+        if (std::atoi(std::getenv("MF_PHASE_PROFILE")) == 2) {
+            std::vector<int> h(bt->G + 1);
+            MF_HIP(hipMemcpy(h.data(), bt->d_wg, sizeof(int) * (bt->G + 1), hipMemcpyDeviceToHost));
+            std::vector<int> tk(5 * (size_t)h[bt->G]);
+            MF_HIP(hipMemcpy(tk.data(), bt->d_wg + bt->G + 1, sizeof(int) * tk.size(), hipMemcpyDeviceToHost));
+            for (int g = 0; g < bt->G; g++) {
+                double t = 0;
+                for (int k = 0; k < 4; k++) t += pf[4 * g + k] / 100.0;
+                std::fprintf(stderr, "[mf batch wg] %d total %.0f spmv %.0f update %.0f sync %.0f |", g, t,
+                             pf[4 * g] / 100.0, pf[4 * g + 1] / 100.0, pf[4 * g + 2] / 100.0);
+                for (int q = h[g]; q < h[g + 1]; q++) {
+                    const int *r = tk.data() + 5 * (size_t)q;
+                    std::fprintf(stderr, " s%d n=%d it=%d w=%d us=%.0f;", r[0], bt->n[r[0]], oc.iters[r[0]], r[4],
+                                 oc.us[r[0]]);
+                }
+                std::fprintf(stderr, "\n");
+            }
+        }
         MF_HIP(hipMemset(bt->args.prof, 0, sizeof(unsigned long long) * pf.size()));
     }
     return oc.status == MF_E_TIMEOUT ? fail(MF_E_TIMEOUT, "group sync timeout") : MF_OK;
 }
 
 // Autotuner.  Trial solves of the caller's right-hand sides, each timed with
-// device events:
-//  1. the current configuration, to measure every system's iterations;
-//  2. workgroups re-assigned by (work per iteration) x (iterations), so that
-//     the systems needing many iterations get more workgroups;
-//  3. tile sizes 8, 16 and 32 (layouts rebuilt on the GPU);
-//  4. when the caller asked for mixed precision: tile thresholds tau = 2^-24
+// device events.  Every configuration is solved twice: on the current
+// assignment, and again after the workgroups were re-assigned from the time
+// every system took in the first solve (the kernel times each system on its
+// first workgroup), so systems that need many iterations or run slowly get
+// more workgroups and the packed workgroups finish together.  Configurations:
+//  1. the current one;
+//  2. tile sizes 8, 16 and 32 (layouts rebuilt on the GPU);
+//  3. when the caller asked for mixed precision: tile thresholds tau = 2^-24
 //     (FP32-rounded tiles) and 2^-11 (FP16-rounded), kept only if every
 //     column still reaches the tolerance.
-// The fastest converged configuration stays captured in the batch's graph.
+// The fastest converged configuration stays captured in the batch's graph,
+// assigned from its own measured times.
 int mf_batch_tune(mf_batch bt, const double *const *b, mf_report *rep)
 {
     clear_error();
@@ -1163,53 +1248,65 @@ int mf_batch_tune(mf_batch bt, const double *const *b, mf_report *rep)
         if (!rc) rc = batch_outcome(bt, oc);
         return rc;
     };
+    // a configuration is tried twice: once on the current assignment, then
+    // re-assigned from the per-system times that solve measured
+    auto measured_trial = [&](float *ms, BatchOutcome &oc) -> int {
+        int rc = trial(ms, oc);
+        if (rc || oc.status != MF_OK) return rc;
+        bt->weights = oc.iters;
+        batch_measure(bt, oc);
+        rc = batch_assign(bt, batch_target(bt), bt->weights);
+        if (!rc) rc = batch_capture(bt);
+        if (!rc) rc = trial(ms, oc);
+        return rc;
+    };
     const bool want_mixed = bt->o.precision == MF_MIXED;
     Cfg cur{bt->B, bt->mixed, bt->tau.empty() ? 0.0 : bt->tau[0]};
     float ms = 0.0f;
     BatchOutcome oc;
-    int rc = trial(&ms, oc);
+    int rc = measured_trial(&ms, oc);
     if (rc) return rc;
     if (oc.status != MF_OK) return fail(oc.status, "tuning trial did not converge");
-    bt->weights = oc.iters;
-    rc = apply(cur, false);
-    if (!rc) rc = trial(&ms, oc);
-    if (rc) return rc;
     Cfg best = cur;
     float best_ms = ms;
-    bt->weights = oc.iters;
+    std::vector<double> best_wtime = bt->wtime;
+    std::vector<int> best_weights = bt->weights;
     char buf[256];
     std::string log;
     std::snprintf(buf, sizeof buf, "B=%d %s: %.2f ms", cur.B, cur.mixed ? "mixed" : "fp64", ms);
     log += buf;
-    std::vector<Cfg> cands;
-    for (int B : {8, 16, 32})
-        if (B != cur.B) cands.push_back({B, cur.mixed, cur.tau});
-    for (const Cfg &c : cands) {
-        rc = apply(c, true);
-        if (!rc) rc = trial(&ms, oc);
+    auto consider = [&](const Cfg &c, const char *label) -> int {
+        int rc = apply(c, true);
+        if (!rc) rc = measured_trial(&ms, oc);
         if (rc) return rc;
-        std::snprintf(buf, sizeof buf, "; B=%d: %.2f ms%s", c.B, ms, oc.status ? " (not converged)" : "");
+        std::snprintf(buf, sizeof buf, "; %s: %.2f ms%s", label, ms, oc.status ? " (not converged)" : "");
         log += buf;
         if (oc.status == MF_OK && oc.worst <= bt->o.tol && ms < best_ms) {
             best = c;
             best_ms = ms;
+            best_wtime = bt->wtime;
+            best_weights = bt->weights;
         }
+        return MF_OK;
+    };
+    for (int B : {8, 16, 32}) {
+        if (B == cur.B) continue;
+        char label[32];
+        std::snprintf(label, sizeof label, "B=%d", B);
+        rc = consider({B, cur.mixed, cur.tau}, label);
+        if (rc) return rc;
     }
     if (want_mixed) {
         for (double tau : {std::ldexp(1.0, -24), std::ldexp(1.0, -11)}) {
-            const Cfg c{best.B, true, tau};
-            rc = apply(c, true);
-            if (!rc) rc = trial(&ms, oc);
+            char label[32];
+            std::snprintf(label, sizeof label, "tau=2^%d", (int)std::lround(std::log2(tau)));
+            rc = consider({best.B, true, tau}, label);
             if (rc) return rc;
-            std::snprintf(buf, sizeof buf, "; tau=2^%d: %.2f ms%s", (int)std::lround(std::log2(tau)), ms,
-                          oc.status ? " (not converged)" : "");
-            log += buf;
-            if (oc.status == MF_OK && oc.worst <= bt->o.tol && ms < best_ms) {
-                best = c;
-                best_ms = ms;
-            }
         }
     }
+    // the best configuration again, assigned from its own measured times
+    bt->wtime = best_wtime;
+    bt->weights = best_weights;
     rc = apply(best, true);
     if (!rc) rc = trial(&ms, oc);
     if (rc) return rc;

@@ -151,6 +151,7 @@ int mf_matrix_set_index_width(mf_matrix m, int width);
 /* Precision / solver options. */
 enum { MF_CG = 0, MF_BICGSTAB = 1 };
 enum { MF_FP64 = 0, MF_MIXED = 1 };
+enum { MF_REORDER_AUTO = 0, MF_REORDER_OFF = 1, MF_REORDER_ON = 2 };
 typedef struct {
     int method;         /* MF_CG or MF_BICGSTAB */
     int precision;      /* MF_FP64 or MF_MIXED */
@@ -173,6 +174,10 @@ typedef struct {
                            one width that every system fits) */
     int profile;        /* CG: 1 = time the phases of the persistent launch
                            (workgroup 0's clock) into mf_report.phase_us */
+    int reorder;        /* MF_REORDER_AUTO (default): the solvers run on P A P^T
+                           with rows sorted by length and tile rows dealt to the
+                           workgroups when the natural slabs hold more than
+                           twice the nonzeros; _OFF: never; _ON: always */
 } mf_opts;
 void mf_opts_default(mf_opts *o);
 
@@ -188,6 +193,8 @@ typedef struct {
     double phase_us[5];  /* opts.profile (CG): per iteration SpMV, sync after it,
                             vector update, sync after it; [4] = true-residual
                             passes, total; of the last persistent launch */
+    int reordered;       /* 1: solved on the row-reordered copy */
+    double pad_factor;   /* stored slab entries / nonzeros of the natural FP64 layout */
 } mf_report;
 
 /* Host buffers in/out (copies included; the timed ms_solve excludes them). */

@@ -75,6 +75,11 @@ static void free_dev(mf_matrix m)
                   m->d_Tile_csr_Col, m->d_rowtmp, m->d_Blockcsr_Val, m->d_Blockcsr_Val_Low};
     for (void *p : ps)
         if (p) (void)hipFree(p);
+    void *qs[] = {m->d_perm, m->d_pb, m->d_px};
+    for (void *p : qs)
+        if (p) (void)hipFree(p);
+    if (m->child) mf_matrix_destroy(m->child);
+    m->child = nullptr;
     m->L64.release();
     m->Lmix.release();
     m->ws.release();
@@ -104,6 +109,7 @@ void mf_opts_default(mf_opts *o)
     o->block = 0;
     o->spin_limit = 1LL << 22;  // a few seconds of polling
     o->index_width = 0;
+    o->reorder = MF_REORDER_AUTO;
 }
 
 int mf_matrix_create(int rowA, int colA, int nnz, const int *rowptr, const int *colidx,

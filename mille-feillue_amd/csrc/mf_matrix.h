@@ -74,6 +74,12 @@ struct mf_matrix_s {
     mf::Workspace ws;
     double ms_build = 0.0, ms_layout = 0.0;
     int want_iw = 0;  // slab index width asked for this matrix's layouts (0 automatic)
+    // solvers on a row-reordered copy (rows sorted by length, tile rows dealt
+    // to the persistent workgroups' chunks): child = P A P^T, perm[new] = old
+    mf_matrix_s *child = nullptr;
+    int *d_perm = nullptr;
+    double *d_pb = nullptr, *d_px = nullptr;
+    int child_grid = 0;
 };
 
 namespace mf {

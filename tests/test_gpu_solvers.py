@@ -297,3 +297,51 @@ def test_gpu_cg_phase_profile(gpu, orc):
     _, rep0 = A.solve(b, method=gpu.MF_CG, tol=TOL)
     assert rep0["phase_us"] == [0.0] * 5
     A.close()
+
+
+def test_gpu_cg_reordered_rows(gpu, orc):
+    # Zipf row lengths pad the natural slabs far beyond the nonzeros: the
+    # solver switches to P A P^T (rows sorted by length, tile rows dealt to
+    # the workgroups) and returns x in the original order; n = 30001 leaves a
+    # partial last tile row
+    rp, ci, va = gpu.gen_random_spd(30001, seed=3)
+    n = len(rp) - 1
+    b = orc.csr_spmv(rp, ci, va, gpu.gen_uniform(9, n, -1, 1))
+    A = gpu.Matrix(rp, ci, va)
+    x, rep = A.solve(b, method=gpu.MF_CG, tol=TOL)
+    x0, rep0 = A.solve(b, method=gpu.MF_CG, tol=TOL, reorder=gpu.MF_REORDER_OFF)
+    xo, orep = orc.cg(rp, ci, va, b, TOL, 20000)
+    print(f"reordered={rep['reordered']} pad={rep['pad_factor']:.2f} it={rep['iterations']} "
+          f"({rep['ms_solve']:.2f} ms) natural it={rep0['iterations']} ({rep0['ms_solve']:.2f} ms) "
+          f"oracle it={orep.iterations}", flush=True)
+    assert rep["reordered"] == 1 and rep0["reordered"] == 0 and rep["pad_factor"] > 2.0
+    for xx, rr in ((x, rep), (x0, rep0)):
+        assert rr["status"] == 0 and rr["relres_true"] <= TOL and _true_relres(rp, ci, va, xx, b) <= 1.5 * TOL
+        assert _iters_close(rr["iterations"], orep.iterations)
+        assert np.linalg.norm(xx - xo) / np.linalg.norm(xo) < 1e-6
+    # mixed tiles on the reordered copy
+    xm, repm = A.solve(b, method=gpu.MF_CG, tol=TOL, precision=gpu.MF_MIXED, tau=2.0 ** -11, rr_factor=1e-2)
+    assert repm["reordered"] == 1 and repm["status"] == 0 and _true_relres(rp, ci, va, xm, b) <= 1.5 * TOL
+    A.close()
+
+
+@pytest.mark.parametrize("method", ["cg", "bicgstab"])
+def test_gpu_forced_reorder_keeps_the_answer(gpu, orc, method):
+    if method == "cg":
+        rp, ci, va = gpu.gen_poisson3d(20, 20, 21)
+        mth = gpu.MF_CG
+    else:
+        rp, ci, va = gpu.gen_convdiff3d(20, 10.0, 1.0, 0)
+        mth = gpu.MF_BICGSTAB
+    n = len(rp) - 1
+    b = orc.csr_spmv(rp, ci, va, gpu.gen_uniform(10, n, -1, 1))
+    A = gpu.Matrix(rp, ci, va)
+    x1, r1 = A.solve(b, method=mth, tol=TOL, reorder=gpu.MF_REORDER_ON)
+    x0, r0 = A.solve(b, method=mth, tol=TOL)
+    print(f"{method}: forced it={r1['iterations']} natural it={r0['iterations']} reordered={r0['reordered']}",
+          flush=True)
+    assert r1["reordered"] == 1 and r0["reordered"] == 0
+    assert r1["status"] == 0 and _true_relres(rp, ci, va, x1, b) <= 1.5 * TOL
+    assert _iters_close(r1["iterations"], r0["iterations"])
+    assert np.linalg.norm(x1 - x0) / np.linalg.norm(x0) < 1e-8
+    A.close()

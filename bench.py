@@ -208,6 +208,20 @@ def main():
                                      "update": round(us[2], 2), "sync_after_update": round(us[3], 2)},
                 "spmv_GBps": round(balg / us[0] / 1e3, 1), "spmv_frac": round(balg / (us[0] * 1e-6) / HBM_PEAK, 4)}
     phases = {"fp64": phase(mf.MF_FP64, b64), "mixed": phase(mf.MF_MIXED, bmx)}
+
+    # vendor-library baseline on the same system: rocSPARSE SpMV + rocBLAS
+    # dots in a host loop (as the reference's hipSPARSE driver), FP64 CG to
+    # the same true relative residual
+    vreps = [A.vendor_solve_device(d_b.data_ptr(), d_x.data_ptr(), method=mf.MF_CG, tol=TOL) for _ in range(3)]
+    for r in vreps:
+        if r["status"] != 0 or r["relres_true"] > TOL:
+            raise RuntimeError(f"vendor baseline did not reach tolerance: {r}")
+    vms = statistics.median(r["ms_solve"] for r in vreps)
+    vendor = {"ms": round(vms, 4), "iterations": vreps[0]["iterations"], "relres_true": vreps[0]["relres_true"],
+              "kind": "rocSPARSE CSR SpMV (adaptive) + rocBLAS ddot, host loop with a residual read per iteration",
+              "speedup_mixed": round(vms / msmx, 3), "speedup_fp64": round(vms / ms64, 3)}
+    log(f"vendor CG: {vms:.3f} ms, {vreps[0]['iterations']} it (mixed {vendor['speedup_mixed']}x, "
+        f"fp64 {vendor['speedup_fp64']}x faster)")
     log(f"spmv phase in the solver: fp64 {phases['fp64']['spmv_frac']:.3f}, mixed {phases['mixed']['spmv_frac']:.3f} of peak")
 
     # dominant kernel: the persistent CG launch.  Unit = one CG iteration,
@@ -278,6 +292,7 @@ def main():
                       "frac": round(bmx / (spmvmx * 1e-3) / HBM_PEAK, 4), "bytes_alg": bmx},
         },
         "spmv_phase_in_solver": phases,
+        "vendor_baseline": vendor,
         "solver_GBps": {"fp64": round(it64 * (b64 + 56.0 * n) / ms64 / 1e6, 1), "mixed": round(ach / 1e9, 1)},
         "roofline": roof,
         "batched": batched,

@@ -204,6 +204,16 @@ int mf_solve(mf_matrix m, const double *b, double *x, const mf_opts *o, mf_repor
 int mf_solve_device(mf_matrix m, const double *d_b, double *d_x, const mf_opts *o,
                     mf_report *rep);
 
+/* Vendor-library baseline on the same matrix: CG or BiCGSTAB (opts->method)
+ * in FP64 as the reference's hipSPARSE drivers run it
+ * (Performance-Comparison/hipSPARSE_CG.cu:182, hipSPARSE_BiCGSTAB.cu:282):
+ * host loop, rocSPARSE CSR SpMV (adaptive), rocBLAS dots, ||r|| read by the
+ * host every iteration; to opts->tol in true relative residual (opts->maxit
+ * iterations at most).  rep->ms_solve covers the loop and the final check.
+ * Device buffers of rowA entries (mf_vendor_solve_device) or host buffers. */
+int mf_vendor_solve_device(mf_matrix m, const double *d_b, double *d_x, const mf_opts *o, mf_report *rep);
+int mf_vendor_solve(mf_matrix m, const double *b, double *x, const mf_opts *o, mf_report *rep);
+
 /* Tiled SpMV y = A x.  mixed: 0 = FP64 layout, 1 = mixed layout of the last
  * mf_solve/mf_matrix_layout call.  Host arrays: x has tilen*B entries (zero
  * padded by the call from colA), y receives tilem*B. */

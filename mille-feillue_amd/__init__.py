@@ -89,7 +89,7 @@ EXPORTS = (
     "Tile_create", "Tile_destroy", "cg_solve_inc", "bicgstab", "mf_status_str", "mf_last_error",
     "mf_init", "mf_matrix_create", "mf_matrix_destroy", "mf_matrix_export", "mf_ref_tiles_free",
     "mf_matrix_encode_cols", "mf_matrix_classes", "mf_matrix_get_info", "mf_opts_default",
-    "mf_solve", "mf_solve_device", "mf_matrix_layout", "mf_matrix_set_index_width", "mf_spmv", "mf_spmv_bench",
+    "mf_solve", "mf_solve_device", "mf_vendor_solve", "mf_vendor_solve_device", "mf_matrix_layout", "mf_matrix_set_index_width", "mf_spmv", "mf_spmv_bench",
     "mf_solve_batched", "mf_mc_classify", "mf_compact_rows", "mf_partition_build", "mf_free",
     "mf_gen_laplace2d", "mf_gen_poisson3d", "mf_gen_convdiff3d", "mf_gen_random_spd",
     "mf_gen_stencil_lognormal", "mf_gen_uniform", "mf_gen_normal",
@@ -368,6 +368,21 @@ class Matrix:
         x = np.zeros(self.rowA, np.float64)
         _check(lib().mf_solve(self.h, _p(bb, C.c_double), _p(x, C.c_double), C.byref(o), C.byref(rep)))
         return x, rep.as_dict()
+
+    def vendor_solve(self, b, **opts):
+        """The vendor-library baseline (rocSPARSE SpMV + rocBLAS, host loop)."""
+        o = default_opts(**opts)
+        rep = Report()
+        bb = np.ascontiguousarray(b, np.float64)
+        x = np.zeros(self.rowA, np.float64)
+        _check(lib().mf_vendor_solve(self.h, _p(bb, C.c_double), _p(x, C.c_double), C.byref(o), C.byref(rep)))
+        return x, rep.as_dict()
+
+    def vendor_solve_device(self, d_b, d_x, **opts):
+        o = default_opts(**opts)
+        rep = Report()
+        _check(lib().mf_vendor_solve_device(self.h, C.c_void_p(d_b), C.c_void_p(d_x), C.byref(o), C.byref(rep)))
+        return rep.as_dict()
 
     def solve_device(self, d_b, d_x, **opts):
         """Solve with device pointers (tilem*B doubles each, b zero padded)."""

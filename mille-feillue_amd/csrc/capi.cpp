@@ -80,6 +80,7 @@ static void free_dev(mf_matrix m)
         if (p) (void)hipFree(p);
     if (m->child) mf_matrix_destroy(m->child);
     m->child = nullptr;
+    mf::vendor_release(m);
     m->L64.release();
     m->Lmix.release();
     m->ws.release();
@@ -397,6 +398,31 @@ int mf_solve_device(mf_matrix m, const double *d_b, double *d_x, const mf_opts *
     clear_error();
     if (!m || !d_b || !d_x || !o || !rep) return fail(MF_E_ARG, "null");
     return solve_device(m, d_b, d_x, *o, rep);
+}
+
+int mf_vendor_solve_device(mf_matrix m, const double *d_b, double *d_x, const mf_opts *o, mf_report *rep)
+{
+    clear_error();
+    if (!m || !d_b || !d_x || !o || !rep) return fail(MF_E_ARG, "null");
+    return mf::vendor_solve_device(m, d_b, d_x, *o, rep);
+}
+
+int mf_vendor_solve(mf_matrix m, const double *b, double *x, const mf_opts *o, mf_report *rep)
+{
+    clear_error();
+    if (!m || !b || !x || !o || !rep) return fail(MF_E_ARG, "null");
+    if (m->rowA != m->colA) return fail(MF_E_ARG, "solver needs a square matrix");
+    const size_t n = (size_t)m->rowA;
+    double *db = nullptr, *dx = nullptr;
+    MF_HIP(hipMalloc((void **)&db, sizeof(double) * n));
+    MF_HIP(hipMalloc((void **)&dx, sizeof(double) * n));
+    MF_HIP(hipMemcpy(db, b, sizeof(double) * n, hipMemcpyHostToDevice));
+    int rc = mf::vendor_solve_device(m, db, dx, *o, rep);
+    if (hipMemcpy(x, dx, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess && !rc)
+        rc = fail(MF_E_HIP, "copy x");
+    (void)hipFree(db);
+    (void)hipFree(dx);
+    return rc;
 }
 
 int mf_solve(mf_matrix m, const double *b, double *x, const mf_opts *o, mf_report *rep)

@@ -55,6 +55,10 @@ struct Workspace {
 
 }  // namespace mf
 
+namespace mf {
+struct VendorCtx;
+}
+
 struct mf_matrix_s {
     int B = 16, rowA = 0, colA = 0, nnz = 0;
     int tilem = 0, tilen = 0, tilenum = 0, csrsize = 0, csrptrlen = 0;
@@ -77,6 +81,7 @@ struct mf_matrix_s {
     // solvers on a row-reordered copy (rows sorted by length, tile rows dealt
     // to the persistent workgroups' chunks): child = P A P^T, perm[new] = old
     mf_matrix_s *child = nullptr;
+    mf::VendorCtx *vendor = nullptr;  // vendor-library baseline state (vendor.hip)
     int *d_perm = nullptr;
     double *d_pb = nullptr, *d_px = nullptr;
     int child_grid = 0;
@@ -91,6 +96,8 @@ int index_fits(mf_matrix m, int *fits);  // tiles.hip
 // the width used for a request (0 automatic: two bytes, else one, else four;
 // 1, 2 or 4: that form if it fits, else the next wider one that fits)
 int resolve_index_width(int want, int fits);  // tiles.hip
+void vendor_release(mf_matrix m);             // vendor.hip
+int vendor_solve_device(mf_matrix m, const double *d_b, double *d_x, const mf_opts &o, mf_report *rep);
 int encode_cols_device(mf_matrix m, uint8_t *d_out);  // tiles.hip
 int classes_device(mf_matrix m, double tau, int allow_mask, uint8_t *d_codes);  // tiles.hip
 hipStream_t stream();

@@ -1137,9 +1137,12 @@ static int build_reordered(mf_matrix m, int grid)
     if (!m->d_perm) MF_HIP(hipMalloc((void **)&m->d_perm, sizeof(int) * (size_t)n));
     if (!m->d_pb) MF_HIP(hipMalloc((void **)&m->d_pb, sizeof(double) * npad));
     if (!m->d_px) MF_HIP(hipMalloc((void **)&m->d_px, sizeof(double) * npad));
-    MF_HIP(hipMemcpy(m->d_perm, perm.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice));
-    MF_HIP(hipMemset(m->d_pb, 0, sizeof(double) * npad));
-    MF_HIP(hipMemset(m->d_px, 0, sizeof(double) * npad));
+    // on the library stream, ordered before the solves that read them
+    hipStream_t st = stream();
+    MF_HIP(hipMemcpyAsync(m->d_perm, perm.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice, st));
+    MF_HIP(hipMemsetAsync(m->d_pb, 0, sizeof(double) * npad, st));
+    MF_HIP(hipMemsetAsync(m->d_px, 0, sizeof(double) * npad, st));
+    MF_HIP(hipStreamSynchronize(st));
     return MF_OK;
 }
 

@@ -345,3 +345,31 @@ def test_gpu_forced_reorder_keeps_the_answer(gpu, orc, method):
     assert _iters_close(r1["iterations"], r0["iterations"])
     assert np.linalg.norm(x1 - x0) / np.linalg.norm(x0) < 1e-8
     A.close()
+
+
+@pytest.mark.parametrize("method", ["cg", "bicgstab"])
+def test_gpu_vendor_baseline(gpu, orc, method):
+    # the rocSPARSE/rocBLAS host-loop baseline reaches the same tolerance in
+    # the oracle's iteration count (same recurrences, FP64)
+    if method == "cg":
+        rp, ci, va = gpu.gen_poisson3d(32, 32, 32)
+        mth = gpu.MF_CG
+    else:
+        rp, ci, va = gpu.gen_convdiff3d(20, 10.0, 1.0, 0)
+        mth = gpu.MF_BICGSTAB
+    n = len(rp) - 1
+    b = orc.csr_spmv(rp, ci, va, gpu.gen_uniform(11, n, -1, 1))
+    A = gpu.Matrix(rp, ci, va)
+    x, rep = A.vendor_solve(b, method=mth, tol=TOL)
+    if method == "cg":
+        xo, orep = orc.cg(rp, ci, va, b, TOL, 20000)
+    else:
+        xo, orep = orc.bicgstab(rp, ci, va, b, TOL, 20000)
+    print(f"vendor {method}: it={rep['iterations']} oracle it={orep.iterations} relres={rep['relres_true']:.2e} "
+          f"ms={rep['ms_solve']:.2f}", flush=True)
+    cpu_rel = _true_relres(rp, ci, va, x, b)
+    assert rep["status"] == 0 and rep["relres_true"] <= TOL and cpu_rel <= 1.5 * TOL
+    assert abs(rep["relres_true"] - cpu_rel) <= 0.1 * cpu_rel  # the reported residual is the solution's
+    assert _iters_close(rep["iterations"], orep.iterations)
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-6
+    A.close()

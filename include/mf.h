@@ -16,6 +16,7 @@
 #ifndef MF_H
 #define MF_H
 #include <stddef.h>
+#include <stdint.h>
 #include "mf_tile_matrix.h"
 
 #ifdef __cplusplus
@@ -51,6 +52,27 @@ int mmio_info(int *m, int *n, int *nnz, int *isSymmetric, char *filename);
 /* PC/mmio_highlevel.h:275 -- CSR in file order (no column sort, duplicates
  * kept), symmetric/hermitian off-diagonals mirrored. */
 int mmio_data(int *csrRowPtr, int *csrColIdx, double *csrVal, char *filename);
+/* Binary CSR cache, PC/biio2.0/src/biio.h (format: int rows, int cols, int64
+ * nnz, int64 row_ptr[rows+1], int col_idx[nnz], double val[nnz]).
+ * binary_read_csr (:22) returns 0 or -1 (missing or short file) and mallocs
+ * the arrays; binary_write_csr (:75) writes that layout; read_Dmatrix (:126)
+ * reads a .cbd or .mtx by extension; read_Dmatrix_32 (:153) likewise with
+ * 32-bit row pointers (left untouched beyond INT_MAX nonzeros);
+ * read_Dmatrix_convert (:97) reads <name>.cbd when present, else the .mtx,
+ * writing the .cbd cache.  Arrays are freed with free() or mf_free(). */
+int binary_read_csr(int *row, int *col, int64_t *nnz, int64_t **row_ptr, int **col_idx, double **val,
+                    char *filename);
+void binary_write_csr(int row, int col, int64_t nnz, int64_t *row_ptr, int *col_idx, double *val, char *filename);
+void read_Dmatrix(int *row, int *col, int64_t *nnz, int64_t **row_ptr, int **col_idx, double **val,
+                  int *isSymmeticeR, char *filename);
+void read_Dmatrix_32(int *row, int *col, int *nnz, int **row_ptr, int **col_idx, double **val, int *isSymmeticeR,
+                     char *filename);
+void read_Dmatrix_convert(int *row, int *col, int64_t *nnz, int64_t **row_ptr, int **col_idx, double **val,
+                          int *isSymmeticeR, char *filename);
+/* Matrix Market writer ("coordinate real general", 17 significant digits):
+ * mmio_data reads the file back to the same CSR bit for bit.  No reference
+ * counterpart (the reference only reads); for the generated configurations. */
+int mf_write_mtx(const char *path, int m, int n, const int *rowptr, const int *colidx, const double *val);
 /* PC/encode.h:29 / :52 -- 4-bit column packing, two per byte. */
 void encode(unsigned char *str, unsigned char *res, int str_len, int offset);
 void decode(unsigned char *str, unsigned char *res, int res_len);

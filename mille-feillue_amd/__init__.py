@@ -86,6 +86,7 @@ class Partition(C.Structure):
 # symbols declared in include/mf.h (checked by the CPU tests)
 EXPORTS = (
     "mmio_info", "mmio_data", "encode", "decode", "exclusive_scan", "exclusive_scan_char",
+    "binary_read_csr", "binary_write_csr", "read_Dmatrix", "read_Dmatrix_32", "read_Dmatrix_convert", "mf_write_mtx",
     "Tile_create", "Tile_destroy", "cg_solve_inc", "bicgstab", "mf_status_str", "mf_last_error",
     "mf_init", "mf_matrix_create", "mf_matrix_destroy", "mf_matrix_export", "mf_ref_tiles_free",
     "mf_matrix_encode_cols", "mf_matrix_classes", "mf_matrix_get_info", "mf_opts_default",
@@ -463,6 +464,87 @@ def mmio(path):
     rc = L.mmio_data(_p(rp, C.c_int), _p(ci, C.c_int), _p(va, C.c_double), path.encode())
     return rc, dict(m=m.value, n=n.value, nnz=nnz.value, sym=sym.value, rowptr=rp, col=ci[: nnz.value],
                     val=va[: nnz.value])
+
+
+_i64p = C.POINTER(C.c_int64)
+
+
+def _take64(r, z, rp, ci, va):
+    out = dict(m=r, rowptr=np.ctypeslib.as_array(rp, (r + 1,)).astype(np.int64),
+               col=np.ctypeslib.as_array(ci, (max(z, 1),))[:z].copy(),
+               val=np.ctypeslib.as_array(va, (max(z, 1),))[:z].copy())
+    for q in (rp, ci, va):
+        lib().mf_free(C.cast(q, C.c_void_p))
+    return out
+
+
+def read_cbd(path):
+    """binary_read_csr: (rc, dict(m, n, nnz, rowptr int64, col, val))."""
+    L = lib()
+    r, c, z = C.c_int(), C.c_int(), C.c_int64()
+    rp, ci, va = _i64p(), _i(), _d()
+    L.binary_read_csr.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64), C.POINTER(_i64p),
+                                  C.POINTER(_i), C.POINTER(_d), C.c_char_p]
+    rc = L.binary_read_csr(C.byref(r), C.byref(c), C.byref(z), C.byref(rp), C.byref(ci), C.byref(va), path.encode())
+    if rc:
+        return rc, None
+    d = _take64(r.value, z.value, rp, ci, va)
+    d.update(n=c.value, nnz=z.value)
+    return 0, d
+
+
+def write_cbd(path, m, n, rowptr, col, val):
+    L = lib()
+    rp = np.ascontiguousarray(rowptr, np.int64)
+    ci = np.ascontiguousarray(col, np.int32)
+    va = np.ascontiguousarray(val, np.float64)
+    L.binary_write_csr.argtypes = [C.c_int, C.c_int, C.c_int64, _i64p, _i, _d, C.c_char_p]
+    L.binary_write_csr(m, n, int(rp[m]), rp.ctypes.data_as(_i64p), _p(ci, C.c_int), _p(va, C.c_double),
+                       path.encode())
+
+
+def read_matrix_32(path):
+    """read_Dmatrix_32: dict(m, n, nnz, sym, rowptr, col, val) or None."""
+    L = lib()
+    r, c, z, sym = C.c_int(-1), C.c_int(), C.c_int(), C.c_int()
+    rp, ci, va = _i(), _i(), _d()
+    L.read_Dmatrix_32.argtypes = [C.POINTER(C.c_int)] * 3 + [C.POINTER(_i), C.POINTER(_i), C.POINTER(_d),
+                                                              C.POINTER(C.c_int), C.c_char_p]
+    L.read_Dmatrix_32(C.byref(r), C.byref(c), C.byref(z), C.byref(rp), C.byref(ci), C.byref(va), C.byref(sym),
+                      path.encode())
+    if r.value < 0:
+        return None
+    d = _take64(r.value, z.value, rp, ci, va)
+    d["rowptr"] = d["rowptr"].astype(np.int32)
+    d.update(n=c.value, nnz=z.value, sym=sym.value)
+    return d
+
+
+def read_matrix_convert(path):
+    """read_Dmatrix_convert: reads <path minus 3 chars>cbd when present, else
+    the .mtx, writing that cache."""
+    L = lib()
+    r, c, z, sym = C.c_int(-1), C.c_int(), C.c_int64(), C.c_int()
+    rp, ci, va = _i64p(), _i(), _d()
+    L.read_Dmatrix_convert.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64),
+                                       C.POINTER(_i64p), C.POINTER(_i), C.POINTER(_d), C.POINTER(C.c_int),
+                                       C.c_char_p]
+    L.read_Dmatrix_convert(C.byref(r), C.byref(c), C.byref(z), C.byref(rp), C.byref(ci), C.byref(va),
+                           C.byref(sym), path.encode())
+    if r.value < 0:
+        return None
+    d = _take64(r.value, z.value, rp, ci, va)
+    d.update(n=c.value, nnz=z.value)
+    return d
+
+
+def write_mtx(path, m, n, rowptr, col, val):
+    L = lib()
+    rp = np.ascontiguousarray(rowptr, np.int32)
+    ci = np.ascontiguousarray(col, np.int32)
+    va = np.ascontiguousarray(val, np.float64)
+    L.mf_write_mtx.argtypes = [C.c_char_p, C.c_int, C.c_int, _i, _i, _d]
+    _check(L.mf_write_mtx(path.encode(), m, n, _p(rp, C.c_int), _p(ci, C.c_int), _p(va, C.c_double)))
 
 
 def compact_rows(ref_struct):

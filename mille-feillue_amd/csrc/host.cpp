@@ -5,6 +5,7 @@
 #include <cctype>
 #include <chrono>
 #include <cmath>
+#include <climits>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -283,6 +284,179 @@ int mmio_data(int *csrRowPtr, int *csrColIdx, double *csrVal, char *filename)
     mm_to_csr(d, csrRowPtr, csrColIdx, csrVal);
     mm_free(&d);
     return 0;
+}
+
+// ------------------------------------------------------ binary CSR (.cbd)
+// Format of PC/biio2.0/src/biio.h:22-94: int rows, int cols, int64 nnz,
+// int64 row_ptr[rows + 1], int col_idx[nnz], double val[nnz] (native byte
+// order, no padding).
+
+// PC/biio2.0/src/biio.h:22.  Returns 0, or -1 when the file is missing or
+// short; the arrays are malloc'ed (free with free / mf_free).
+int binary_read_csr(int *row, int *col, int64_t *nnz, int64_t **row_ptr, int **col_idx, double **val,
+                    char *filename)
+{
+    FILE *fp = std::fopen(filename, "rb");
+    if (!fp) return -1;
+    int r = 0, c = 0;
+    int64_t z = 0;
+    bool ok = std::fread(&r, sizeof(int), 1, fp) == 1 && std::fread(&c, sizeof(int), 1, fp) == 1 &&
+              std::fread(&z, sizeof(int64_t), 1, fp) == 1 && r >= 0 && z >= 0;
+    int64_t *rp = nullptr;
+    int *ci = nullptr;
+    double *va = nullptr;
+    if (ok) {
+        rp = (int64_t *)std::malloc(sizeof(int64_t) * ((size_t)r + 1));
+        ci = (int *)std::malloc(sizeof(int) * (size_t)std::max<int64_t>(z, 1));
+        va = (double *)std::malloc(sizeof(double) * (size_t)std::max<int64_t>(z, 1));
+        ok = rp && ci && va && std::fread(rp, sizeof(int64_t), (size_t)r + 1, fp) == (size_t)r + 1 &&
+             std::fread(ci, sizeof(int), (size_t)z, fp) == (size_t)z &&
+             std::fread(va, sizeof(double), (size_t)z, fp) == (size_t)z;
+    }
+    std::fclose(fp);
+    if (!ok) {
+        std::free(rp);
+        std::free(ci);
+        std::free(va);
+        return -1;
+    }
+    *row = r;
+    *col = c;
+    *nnz = z;
+    *row_ptr = rp;
+    *col_idx = ci;
+    *val = va;
+    return 0;
+}
+
+// PC/biio2.0/src/biio.h:75.  Nothing is written when the file cannot be opened.
+void binary_write_csr(int row, int col, int64_t nnz, int64_t *row_ptr, int *col_idx, double *val, char *filename)
+{
+    FILE *fp = std::fopen(filename, "wb");
+    if (!fp) return;
+    std::fwrite(&row, sizeof(int), 1, fp);
+    std::fwrite(&col, sizeof(int), 1, fp);
+    std::fwrite(&nnz, sizeof(int64_t), 1, fp);
+    std::fwrite(row_ptr, sizeof(int64_t), (size_t)row + 1, fp);
+    std::fwrite(col_idx, sizeof(int), (size_t)nnz, fp);
+    std::fwrite(val, sizeof(double), (size_t)nnz, fp);
+    std::fclose(fp);
+}
+
+// Matrix Market file -> malloc'ed CSR with 64-bit row pointers (the biio
+// readers' use of mmio_info / mmio_data).
+static int mtx_to_csr64(const char *path, int *row, int *col, int64_t *nnz, int64_t **row_ptr, int **col_idx,
+                        double **val, int *isSym)
+{
+    MMData d;
+    int rc = mm_parse(path, &d);
+    if (rc) return rc;
+    const long long z = mm_nnz_expanded(d);
+    std::vector<int> rp32((size_t)d.m + 1);
+    int *ci = (int *)std::malloc(sizeof(int) * (size_t)std::max<long long>(z, 1));
+    double *va = (double *)std::malloc(sizeof(double) * (size_t)std::max<long long>(z, 1));
+    int64_t *rp = (int64_t *)std::malloc(sizeof(int64_t) * ((size_t)d.m + 1));
+    if (!ci || !va || !rp) {
+        std::free(ci);
+        std::free(va);
+        std::free(rp);
+        mm_free(&d);
+        return fail(MF_E_NOMEM, "matrix arrays");
+    }
+    mm_to_csr(d, rp32.data(), ci, va);
+    for (size_t i = 0; i < rp32.size(); i++) rp[i] = rp32[i];
+    *row = d.m;
+    *col = d.n;
+    *nnz = z;
+    *row_ptr = rp;
+    *col_idx = ci;
+    *val = va;
+    if (isSym) *isSym = d.is_sym ? 1 : 0;
+    mm_free(&d);
+    return 0;
+}
+
+static bool ends_with(const char *s, const char *tail)
+{
+    const size_t n = std::strlen(s), k = std::strlen(tail);
+    return n >= k && std::strcmp(s + n - k, tail) == 0;
+}
+
+// PC/biio2.0/src/biio.h:97.  The cache name replaces the last three
+// characters of filename by "cbd"; a readable cache is used, otherwise the
+// Matrix Market file is read and the cache written.
+void read_Dmatrix_convert(int *row, int *col, int64_t *nnz, int64_t **row_ptr, int **col_idx, double **val,
+                          int *isSymmeticeR, char *filename)
+{
+    const size_t len = std::strlen(filename);
+    if (len < 3) return;
+    std::string cache(filename);
+    cache.replace(len - 3, 3, "cbd");
+    if (binary_read_csr(row, col, nnz, row_ptr, col_idx, val, &cache[0]) == 0) return;
+    if (mtx_to_csr64(filename, row, col, nnz, row_ptr, col_idx, val, isSymmeticeR) != 0) return;
+    binary_write_csr(*row, *col, *nnz, *row_ptr, *col_idx, *val, &cache[0]);
+}
+
+// PC/biio2.0/src/biio.h:126: by extension, .cbd (binary) or .mtx (Matrix
+// Market); other names leave the outputs untouched.
+void read_Dmatrix(int *row, int *col, int64_t *nnz, int64_t **row_ptr, int **col_idx, double **val,
+                  int *isSymmeticeR, char *filename)
+{
+    if (ends_with(filename, "cbd"))
+        (void)binary_read_csr(row, col, nnz, row_ptr, col_idx, val, filename);
+    else if (ends_with(filename, "mtx"))
+        (void)mtx_to_csr64(filename, row, col, nnz, row_ptr, col_idx, val, isSymmeticeR);
+}
+
+// PC/biio2.0/src/biio.h:153: as read_Dmatrix with 32-bit row pointers; a
+// matrix with more than INT_MAX nonzeros leaves the outputs untouched.
+void read_Dmatrix_32(int *row, int *col, int *nnz, int **row_ptr, int **col_idx, double **val, int *isSymmeticeR,
+                     char *filename)
+{
+    int r = 0, c = 0;
+    int64_t z = 0, *rp = nullptr;
+    int *ci = nullptr;
+    double *va = nullptr;
+    int rc = -1;
+    if (ends_with(filename, "cbd")) rc = binary_read_csr(&r, &c, &z, &rp, &ci, &va, filename);
+    else if (ends_with(filename, "mtx")) rc = mtx_to_csr64(filename, &r, &c, &z, &rp, &ci, &va, isSymmeticeR);
+    if (rc) return;
+    if (z > INT_MAX) {
+        std::free(rp);
+        std::free(ci);
+        std::free(va);
+        return;
+    }
+    int *rp32 = (int *)std::malloc(sizeof(int) * ((size_t)r + 1));
+    for (int i = 0; i <= r; i++) rp32[i] = (int)rp[i];
+    std::free(rp);
+    *row = r;
+    *col = c;
+    *nnz = (int)z;
+    *row_ptr = rp32;
+    *col_idx = ci;
+    *val = va;
+}
+
+// Matrix Market writer: "coordinate real general", 1-based, rows in order
+// and each row's entries in CSR order, values with 17 significant digits, so
+// mmio_data reads back the same CSR bit for bit.
+int mf_write_mtx(const char *path, int m, int n, const int *rowptr, const int *colidx, const double *val)
+{
+    clear_error();
+    if (!path || m < 0 || n < 0 || !rowptr) return fail(MF_E_ARG, "null or negative argument");
+    FILE *fp = std::fopen(path, "w");
+    if (!fp) return fail(MF_E_IO, std::string("cannot write ") + path);
+    std::fprintf(fp, "%%%%MatrixMarket matrix coordinate real general\n%d %d %d\n", m, n, rowptr[m]);
+    char line[96];
+    for (int i = 0; i < m; i++)
+        for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
+            const int len = std::snprintf(line, sizeof line, "%d %d %.17g\n", i + 1, colidx[k] + 1, val[k]);
+            std::fwrite(line, 1, (size_t)len, fp);
+        }
+    const bool ok = std::ferror(fp) == 0;
+    std::fclose(fp);
+    return ok ? MF_OK : fail(MF_E_IO, std::string("write error on ") + path);
 }
 
 // PC/encode.h:29 -- pairs (c[2k], c[2k+1]) into one byte, high nibble first.

@@ -348,3 +348,57 @@ class Ref:
 REF_ARRAYS = ("tile_ptr", "tile_columnidx", "tile_nnz", "Format", "blknnz", "blknnznnz",
               "csr_offset", "csrptr_offset", "Blockcsr_Val", "Blockcsr_Val_Low", "Blockcsr_Ptr",
               "Tile_csr_Col", "csr_compressedIdx")
+
+
+class RefBiio:
+    """The reference's binary CSR reader/writer (oracle/_ref/libref_biio.so)."""
+
+    def __init__(self):
+        path = os.path.join(REF_DIR, "libref_biio.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        self.L = L = C.CDLL(path)
+        i64p = C.POINTER(C.c_int64)
+        L.ref_binary_read_csr.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64),
+                                          C.POINTER(i64p), C.POINTER(_i), C.POINTER(_d), C.c_char_p]
+        L.ref_binary_write_csr.argtypes = [C.c_int, C.c_int, C.c_int64, i64p, _i, _d, C.c_char_p]
+        L.ref_read_Dmatrix_32.argtypes = [C.POINTER(C.c_int)] * 3 + [C.POINTER(_i), C.POINTER(_i), C.POINTER(_d),
+                                                                     C.POINTER(C.c_int), C.c_char_p]
+        L.ref_biio_free.argtypes = [C.c_void_p]
+
+    def _take(self, r, z, rp, ci, va, rp_dtype):
+        out = dict(m=r, rowptr=np.ctypeslib.as_array(rp, (r + 1,)).astype(rp_dtype),
+                   col=np.ctypeslib.as_array(ci, (max(z, 1),))[:z].copy(),
+                   val=np.ctypeslib.as_array(va, (max(z, 1),))[:z].copy())
+        for q in (rp, ci, va):
+            self.L.ref_biio_free(C.cast(q, C.c_void_p))
+        return out
+
+    def read_cbd(self, path):
+        r, c, z = C.c_int(), C.c_int(), C.c_int64()
+        rp, ci, va = C.POINTER(C.c_int64)(), _i(), _d()
+        rc = self.L.ref_binary_read_csr(C.byref(r), C.byref(c), C.byref(z), C.byref(rp), C.byref(ci),
+                                        C.byref(va), path.encode())
+        if rc:
+            return rc, None
+        d = self._take(r.value, z.value, rp, ci, va, np.int64)
+        d.update(n=c.value, nnz=z.value)
+        return 0, d
+
+    def write_cbd(self, path, m, n, rowptr, col, val):
+        rp = np.ascontiguousarray(rowptr, np.int64)
+        ci = np.ascontiguousarray(col, np.int32)
+        va = np.ascontiguousarray(val, np.float64)
+        self.L.ref_binary_write_csr(m, n, int(rp[m]), rp.ctypes.data_as(C.POINTER(C.c_int64)), _p(ci, C.c_int),
+                                    _p(va, C.c_double), path.encode())
+
+    def read_matrix_32(self, path):
+        r, c, z, sym = C.c_int(-1), C.c_int(), C.c_int(), C.c_int()
+        rp, ci, va = _i(), _i(), _d()
+        self.L.ref_read_Dmatrix_32(C.byref(r), C.byref(c), C.byref(z), C.byref(rp), C.byref(ci), C.byref(va),
+                                   C.byref(sym), path.encode())
+        if r.value < 0:
+            return None
+        d = self._take(r.value, z.value, rp, ci, va, np.int32)
+        d.update(n=c.value, nnz=z.value, sym=sym.value)
+        return d

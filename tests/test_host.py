@@ -152,3 +152,76 @@ def test_generators_shapes_and_properties(mf):
 def test_cli_binaries_built():
     for exe in ("mf_cg", "mf_bicgstab"):
         assert os.access(os.path.join(ROOT, "mille-feillue_amd", exe), os.X_OK)
+
+
+def _ref_biio():
+    from orc import RefBiio
+    try:
+        return RefBiio()
+    except FileNotFoundError as e:
+        pytest.skip(f"reference biio build missing: {e}")
+
+
+def test_mtx_writer_round_trip(mf, tmp_path):
+    # what mf_write_mtx writes, mmio_data reads back bit for bit
+    for rp, ci, va in (mf.gen_poisson3d(7, 6, 5), mf.gen_random_spd(800, seed=4),
+                       mf.gen_stencil_lognormal(2, 500, 1.5, 3)):
+        n = len(rp) - 1
+        p = str(tmp_path / "w.mtx")
+        mf.write_mtx(p, n, n, rp, ci, va)
+        rc, d = mf.mmio(p)
+        assert rc == 0 and d["m"] == n and d["n"] == n and d["nnz"] == len(ci)
+        assert np.array_equal(d["rowptr"], rp) and np.array_equal(d["col"], ci)
+        assert np.array_equal(d["val"].view(np.uint64), np.asarray(va).view(np.uint64))
+
+
+@pytest.mark.parametrize("name", ["1138_bus", "ani1_nonsymm", "LFAT5"])
+def test_binary_csr_matches_reference(mf, tmp_path, name):
+    """.cbd files written by the library and by the reference's biio are the
+    same bytes and each reader reads the other's file; read_Dmatrix_32 equals
+    the reference's on the shipped matrices."""
+    ref = _ref_biio()
+    src = os.path.join(GOLDEN, f"{name}.mtx")
+    mine, theirs = mf.read_matrix_32(src), ref.read_matrix_32(src)
+    assert mine is not None and theirs is not None
+    for k in ("m", "n", "nnz", "sym"):
+        assert mine[k] == theirs[k], k
+    for k in ("rowptr", "col", "val"):
+        assert np.array_equal(mine[k], theirs[k]), k
+    a, b = str(tmp_path / "a.cbd"), str(tmp_path / "b.cbd")
+    m, n = mine["m"], mine["n"]
+    rp64 = mine["rowptr"].astype(np.int64)
+    mf.write_cbd(a, m, n, rp64, mine["col"], mine["val"])
+    ref.write_cbd(b, m, n, rp64, mine["col"], mine["val"])
+    assert open(a, "rb").read() == open(b, "rb").read()
+    for reader in (mf.read_cbd, ref.read_cbd):
+        for path in (a, b):
+            rc, d = reader(path)
+            assert rc == 0 and d["m"] == m and d["n"] == n and d["nnz"] == mine["nnz"]
+            for k in ("rowptr", "col", "val"):
+                assert np.array_equal(d[k], rp64 if k == "rowptr" else mine[k]), (reader, path, k)
+    # read_Dmatrix_32 takes the .cbd by its extension
+    d = mf.read_matrix_32(a)
+    assert np.array_equal(d["rowptr"], mine["rowptr"]) and np.array_equal(d["val"], mine["val"])
+
+
+def test_binary_csr_cache_and_errors(mf, tmp_path):
+    import shutil
+    src = tmp_path / "g.mtx"
+    shutil.copy(os.path.join(GOLDEN, "1138_bus.mtx"), src)
+    cache = tmp_path / "g.cbd"
+    assert not cache.exists()
+    d1 = mf.read_matrix_convert(str(src))
+    assert d1 is not None and cache.exists()
+    rc, dm = mf.mmio(str(src))
+    assert np.array_equal(d1["rowptr"], dm["rowptr"]) and np.array_equal(d1["val"], dm["val"])
+    # with the cache present the Matrix Market file is not needed
+    src.write_text("not a matrix\n")
+    d2 = mf.read_matrix_convert(str(src))
+    assert all(np.array_equal(d1[k], d2[k]) for k in ("rowptr", "col", "val"))
+    # a missing or truncated binary file is reported as -1
+    assert mf.read_cbd(str(tmp_path / "none.cbd"))[0] == -1
+    cache.write_bytes(cache.read_bytes()[:100])
+    assert mf.read_cbd(str(cache))[0] == -1
+    # other extensions leave the outputs untouched
+    assert mf.read_matrix_32(str(tmp_path / "m.txt")) is None

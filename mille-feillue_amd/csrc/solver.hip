@@ -41,15 +41,16 @@ namespace mf {
 // base + k * 256/B + q (k < NR) together.  NR = 2 for layouts with narrow
 // tiles and one- or two-byte indices, 1 otherwise (measured fastest at 128^3
 // Poisson and the 96^3 27-point stencil).
+constexpr int SPMV_TPB = 256;  // 128, 512 and 1024 measured within 1 % (or slower)
 template <int B, int IW, int NR>
-__global__ __launch_bounds__(256) void k_spmv(DevL L, int tilem, const double *__restrict__ g64,
-                                              const float *__restrict__ g32, double *__restrict__ y)
+__global__ __launch_bounds__(SPMV_TPB) void k_spmv(DevL L, int tilem, const double *__restrict__ g64,
+                                                   const float *__restrict__ g32, double *__restrict__ y)
 {
-    constexpr int GPB = 256 / B, TRB = GPB * NR;
+    constexpr int GPB = SPMV_TPB / B, TRB = GPB * NR;
     __shared__ int4 s_h[2 * TRB];
     __shared__ int s_t0[TRB + 1];
     const int tb = xcd_chunk(blockIdx.x, gridDim.x) * TRB;
-    for (int q = threadIdx.x; q <= TRB; q += 256) {
+    for (int q = threadIdx.x; q <= TRB; q += SPMV_TPB) {
         const int tr = tb + q;
         if (q < TRB) {
             const int4 *h = reinterpret_cast<const int4 *>(L.trh + (size_t)min(tr, tilem) * 4);
@@ -1029,9 +1030,9 @@ int spmv_device(mf_matrix m, const Layout &L, const double *d_x, double *d_y, bo
     if (L.haslow && convert)
         k_to_f32<<<(unsigned)((ncol + 255) / 256), 256, 0, st>>>(d_x, m->ws.g32, ncol);
     const int nr = L.haslow && L.iw < 4 ? 2 : 1;
-    const int nb = (int)(((long long)m->tilem * B + 256 * nr - 1) / (256 * nr));
+    const int nb = (int)(((long long)m->tilem * B + SPMV_TPB * nr - 1) / (SPMV_TPB * nr));
     DevL V = L.view(m->d_tile_ptr);
-#define MF_SPMV(BB, W, NR) k_spmv<BB, W, NR><<<nb, 256, 0, st>>>(V, m->tilem, d_x, m->ws.g32, d_y)
+#define MF_SPMV(BB, W, NR) k_spmv<BB, W, NR><<<nb, SPMV_TPB, 0, st>>>(V, m->tilem, d_x, m->ws.g32, d_y)
 #define MF_SPMV_W(BB, NR) if (L.iw == 1) MF_SPMV(BB, 1, NR); else if (L.iw == 2) MF_SPMV(BB, 2, NR); else MF_SPMV(BB, 4, NR)
 #define MF_SPMV_B(BB) if (nr == 2) { MF_SPMV_W(BB, 2); } else { MF_SPMV_W(BB, 1); }
     if (B == 16) { MF_SPMV_B(16); }

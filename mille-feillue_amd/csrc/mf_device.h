@@ -1,5 +1,6 @@
-// Device building blocks for gfx950 (wave64): value decoders, the per-row tile
-// SpMV used by every kernel, the grid barrier and deterministic reductions.
+// Device building blocks for gfx950 (wave64): the slab layout's value and
+// index decoders, the per-row tile SpMV used by every kernel, wave64
+// reductions and the XCD-aware chunk order.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>

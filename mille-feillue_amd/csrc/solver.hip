@@ -762,15 +762,27 @@ __global__ void k_reset_ctl(unsigned long long *slots, int nslots, int *ctl)
 }
 
 // One persistent launch: solves L x = b (x overwritten) on the stream.
+// Grid of the persistent solvers: workgroups and tile rows per workgroup.
+// grid > 0 is taken as given; by default one workgroup per CU, but no more
+// than leave every lane group (B lanes of a 512-thread workgroup) two tile
+// rows, since every grid sync polls all workgroups (a 256^2 Laplacian runs
+// fastest on 64 workgroups).
+static void solver_grid(int tilem, int B, int grid, int *G_out, int *trw_out)
+{
+    int G = grid > 0 ? grid : std::min(device_cu_count(), std::max(1, tilem / (2 * (512 / B))));
+    G = std::max(1, std::min(G, tilem));
+    const int trw = (tilem + G - 1) / G;
+    *G_out = (tilem + trw - 1) / trw;  // no empty workgroups
+    *trw_out = trw;
+}
+
 static int launch_inner(mf_matrix m, const Layout &L, const Layout *L64, int method, const double *d_b,
                         double *d_x, double tol, const mf_opts &o, bool sync, InnerResult *res, int *grid_out,
                         int *block_out, int *rpt_out)
 {
     hipStream_t st = stream();
-    int G = o.grid > 0 ? o.grid : device_cu_count();
-    G = std::max(1, std::min(G, m->tilem));
-    const int trw = (m->tilem + G - 1) / G;
-    G = (m->tilem + trw - 1) / trw;  // no empty workgroups
+    int G = 1, trw = 1;
+    solver_grid(m->tilem, m->B, o.grid, &G, &trw);
     KernelChoice kc = pick(m->B, L.iw, method, (long long)trw * m->B);
     if (!kc.fn)
         return fail(MF_E_UNSUPPORTED,
@@ -1069,16 +1081,6 @@ __global__ void k_permute_out(const int *__restrict__ perm, int n, const double 
     if (i < n) dst[perm[i]] = src[i];
 }
 
-// grid of the persistent solvers (as launch_inner): workgroups and tile rows
-// per workgroup
-static void solver_grid(int tilem, int grid, int *G_out, int *trw_out)
-{
-    int G = grid > 0 ? grid : device_cu_count();
-    G = std::max(1, std::min(G, tilem));
-    const int trw = (tilem + G - 1) / G;
-    *G_out = (tilem + trw - 1) / trw;
-    *trw_out = trw;
-}
 
 static int build_reordered(mf_matrix m, int grid)
 {
@@ -1095,7 +1097,7 @@ static int build_reordered(mf_matrix m, int grid)
     std::stable_sort(order.begin(), order.end(),
                      [&](int a, int b) { return rp[a + 1] - rp[a] > rp[b + 1] - rp[b]; });
     int G = 1, trw = 1;
-    solver_grid(tilem, grid, &G, &trw);
+    solver_grid(tilem, B, grid, &G, &trw);
     std::vector<int> start(G), cap(G), fill(G, 0), pos(tilem);
     for (int g = 0; g < G; g++) {
         start[g] = g * trw;

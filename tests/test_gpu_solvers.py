@@ -373,3 +373,24 @@ def test_gpu_vendor_baseline(gpu, orc, method):
     assert _iters_close(rep["iterations"], orep.iterations)
     assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-6
     A.close()
+
+
+def test_gpu_dropin_bicgstab_csv(gpu, tmp_path):
+    # the reference's BiCGSTAB driver: rhs = A * 1, one row appended to
+    # bicg_syncfree_amd_mi200.csv (PC/Mille-feuille_BiCGSTAB_AMD.cu:1606-1615)
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "mille-feillue_amd",
+                       "mf_bicgstab")
+    mtx = os.path.join(GOLDEN, "ani1_nonsymm.mtx")
+    for run in range(2):
+        r = subprocess.run([exe, mtx, "16"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+        print(r.stdout, r.stderr, flush=True)
+        assert r.returncode == 0
+    rows = (tmp_path / "bicg_syncfree_amd_mi200.csv").read_text().strip().split("\n")
+    assert len(rows) == 2  # appended, not overwritten
+    g = load_golden("ani1_nonsymm")
+    for line in rows:
+        row = line.split(",")
+        # path,itr,time,time,nnzR,l2_norm,residual,each_block_nnz,1
+        assert len(row) == 9 and row[0] == mtx and int(row[1]) > 0 and int(row[4]) == int(g["nnz"])
+        assert float(row[5]) <= TOL and int(row[8]) == 1

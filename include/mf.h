@@ -292,6 +292,21 @@ int mf_solve_batched(int nsys, const int *n, const int *const *rowptr, const int
  * the values. */
 double mf_mc_classify(mf_matrix m, long long counts[4]);
 
+/* Memory footprint: the reference's Memory-Cost figure for its tile format
+ * (MC/memory_Mille_feillue.cu:659-922: tile metadata, compact row lists,
+ * values at the width of their magnitude bin; memory_CP = 8 B per row
+ * segment) beside the CSR FP64 bytes and this library's FP64 and per-tile
+ * mixed layouts at (tau, allow_mask) (slabs with their padding + headers).
+ * The mixed layout is built (and kept) by the call. */
+typedef struct {
+    double ref_memory_MB, ref_memory_CP_MB;
+    double csr_fp64_MB;
+    double layout_fp64_MB, layout_mixed_MB;
+    int index_width;            /* slab index bytes of the mixed layout */
+    long long nnz_per_code[5];  /* nonzeros per code in the mixed layout */
+} mf_memory;
+int mf_memory_report(mf_matrix m, double tau, int allow_mask, mf_memory *out);
+
 /* Host preprocessing of the reference schedule, exported for the CSV rows:
  * compact row lists (PC/Mille-feuille_CG_AMD.cu:1421-1480) and the two-pointer
  * warp partition (:1483-1735; mode 1 = AMD autotune, mode 0 = fixed

@@ -91,7 +91,7 @@ EXPORTS = (
     "mf_init", "mf_matrix_create", "mf_matrix_destroy", "mf_matrix_export", "mf_ref_tiles_free",
     "mf_matrix_encode_cols", "mf_matrix_classes", "mf_matrix_get_info", "mf_opts_default",
     "mf_solve", "mf_solve_device", "mf_vendor_solve", "mf_vendor_solve_device", "mf_matrix_layout", "mf_matrix_set_index_width", "mf_spmv", "mf_spmv_bench",
-    "mf_solve_batched", "mf_mc_classify", "mf_compact_rows", "mf_partition_build", "mf_free",
+    "mf_solve_batched", "mf_mc_classify", "mf_memory_report", "mf_compact_rows", "mf_partition_build", "mf_free",
     "mf_gen_laplace2d", "mf_gen_poisson3d", "mf_gen_convdiff3d", "mf_gen_random_spd",
     "mf_gen_stencil_lognormal", "mf_gen_uniform", "mf_gen_normal",
     "mf_batch_create", "mf_batch_solve", "mf_batch_info", "mf_batch_destroy", "mf_batch_tune",
@@ -391,6 +391,19 @@ class Matrix:
         rep = Report()
         _check(lib().mf_solve_device(self.h, C.c_void_p(d_b), C.c_void_p(d_x), C.byref(o), C.byref(rep)))
         return rep.as_dict()
+
+    def memory_report(self, tau=0.0, allow_mask=ALLOW_ALL_LOW):
+        class Mem(C.Structure):
+            _fields_ = [("ref_memory_MB", C.c_double), ("ref_memory_CP_MB", C.c_double),
+                        ("csr_fp64_MB", C.c_double), ("layout_fp64_MB", C.c_double),
+                        ("layout_mixed_MB", C.c_double), ("index_width", C.c_int),
+                        ("nnz_per_code", C.c_longlong * 5)]
+        r = Mem()
+        lib().mf_memory_report.argtypes = [C.c_void_p, C.c_double, C.c_int, C.POINTER(Mem)]
+        _check(lib().mf_memory_report(self.h, tau, allow_mask, C.byref(r)))
+        d = {k: getattr(r, k) for k, _ in Mem._fields_}
+        d["nnz_per_code"] = list(r.nnz_per_code)
+        return d
 
     def mc_classify(self):
         cnt = (C.c_longlong * 4)()

@@ -470,6 +470,62 @@ double mf_mc_classify(mf_matrix m, long long counts[4])
     return (double)(8 * counts[3] + 2 * counts[1] + 4 * counts[2] + counts[0]) / 1048576.0;
 }
 
+// Device bytes of a compute layout: slabs + headers (trh 32 B per tile row,
+// stc 4 B per tile, tile_ptr).
+static double layout_bytes(const mf_matrix m, const mf::Layout &L)
+{
+    return (double)L.blob_bytes + 32.0 * m->tilem + 4.0 * m->tilenum + 4.0 * (m->tilem + 1);
+}
+
+int mf_memory_report(mf_matrix m, double tau, int allow_mask, mf_memory *out)
+{
+    clear_error();
+    if (!m || !out) return fail(MF_E_ARG, "null");
+    std::memset(out, 0, sizeof(*out));
+    mf_ref_tiles t;
+    int rc = mf_matrix_export(m, &t);
+    if (rc) return rc;
+    mf_compact c;
+    rc = mf_compact_rows(&t, &c);
+    if (rc) { mf_ref_tiles_free(&t); return rc; }
+    // MC/memory_Mille_feillue.cu:659-671 (tile metadata), :859-864 (compact
+    // lists, Tile_csr_Col, row index), :866-918 (values by magnitude bin),
+    // :920-922 (memory_CP: 8 B per row segment); sums in 64 bits
+    const double MB = 1048576.0;
+    const long long tm = t.tilem, tn = t.tilenum;
+    const double a = 4.0 * (tm + 1) + 4.0 * tn + (tn + 1) + 4.0 * (tn + 1) + 4.0 * (tn + 1) + 4.0 * (tn + 3);
+    const double b = 2.0 * ((double)c.cnt_non_new + 1) + tn + (double)t.csrsize + 4.0 * tn;
+    long long cnt[4] = {0, 0, 0, 0};
+    for (int tid = 0; tid < t.tilenum; tid++) {
+        const int off = t.csr_offset[tid];
+        for (int ri = c.nonzero_row_new[tid]; ri < c.nonzero_row_new[tid + 1]; ri++)
+            for (int rj = c.blockcsr_ptr_new[ri]; rj < c.blockcsr_ptr_new[ri + 1]; rj++) {
+                const double v = t.Blockcsr_Val[off + rj];
+                const int bin = (v >= -3.9375 && v <= 3.9375) ? 0 : (v >= -60.0 && v <= 60.0) ? 1
+                                : (v >= -6000.0 && v <= 6000.0) ? 2 : 3;
+                cnt[bin]++;
+            }
+    }
+    const double cv = 8.0 * cnt[3] + 2.0 * cnt[1] + 4.0 * cnt[2] + 1.0 * cnt[0];
+    out->ref_memory_MB = (a + b + cv) / MB;
+    out->ref_memory_CP_MB = 8.0 * (double)c.nonzero_row_new[t.tilenum] / MB;
+    std::free(c.nonzero_row_new); std::free(c.blockrowid_new); std::free(c.blockcsr_ptr_new);
+    std::free(c.block_signal); std::free(c.tile_rowidx);
+    mf_ref_tiles_free(&t);
+    out->csr_fp64_MB = (4.0 * (m->rowA + 1) + 12.0 * m->nnz) / MB;
+    if (!m->L64.built) {
+        rc = mf::build_layout(m, m->L64, 0.0, 0);
+        if (rc) return rc;
+    }
+    rc = mf::build_layout(m, m->Lmix, tau, allow_mask);
+    if (rc) return rc;
+    out->layout_fp64_MB = layout_bytes(m, m->L64) / MB;
+    out->layout_mixed_MB = layout_bytes(m, m->Lmix) / MB;
+    out->index_width = m->Lmix.iw;
+    for (int k = 0; k < 5; k++) out->nnz_per_code[k] = m->Lmix.nnz_per_code[k];
+    return MF_OK;
+}
+
 int mf_solve_batched(int nsys, const int *n, const int *const *rowptr, const int *const *colidx,
                      const double *const *val, int nrhs, const double *const *b, double *const *x,
                      const mf_opts *o, int *iters, double *relres, mf_report *rep);

@@ -90,3 +90,29 @@ def test_gpu_layout_info(gpu):
     assert info["tiles_per_code"][2] == info["tilenum"]
     assert info["nnz_per_code"][2] == len(ci)
     assert info["bytes_alg_mixed"] < info["bytes_alg_fp64"]
+
+
+def test_gpu_memory_report(gpu, orc):
+    """Memory-Cost figure of the reference format from the oracle's compact
+    rows and magnitude bins, beside the slab layouts' bytes: the FP64 slabs
+    hold at least 8 + index bytes per nonzero, the E4M3 slabs of the Poisson
+    matrix (values 6 and -1 exact) at least 1 + index bytes and less than the
+    FP64 ones."""
+    rp, ci, va = gpu.gen_poisson3d(20, 20, 20)
+    m = len(rp) - 1
+    A = gpu.Matrix(rp, ci, va)
+    r = A.memory_report(0.0)
+    t = orc.tiles(16, m, m, rp, ci, va)
+    c = t.compact()
+    tm, tn, nnz = t.d["tilem"], t.d["tilenum"], len(ci)
+    meta = 4 * (tm + 1) + 4 * tn + (tn + 1) + 3 * 4 * (tn + 1) + 8 + 2 * (c["cnt_non_new"] + 1) + tn + nnz + 4 * tn
+    MB = 1048576.0
+    assert abs(r["ref_memory_MB"] - (meta / MB + c["mc_memory_MB"])) < 1e-9
+    assert abs(r["ref_memory_CP_MB"] - 8.0 * c["nonzero_row_new"][-1] / MB) < 1e-12
+    assert abs(r["csr_fp64_MB"] - (4 * (m + 1) + 12 * nnz) / MB) < 1e-12
+    iw = r["index_width"]
+    print(r, flush=True)
+    assert r["nnz_per_code"][2] == nnz  # every value exact in E4M3
+    assert r["layout_fp64_MB"] * MB >= nnz * (8 + iw) and r["layout_mixed_MB"] * MB >= nnz * (1 + iw)
+    assert r["layout_mixed_MB"] < 0.5 * r["layout_fp64_MB"]
+    A.close()

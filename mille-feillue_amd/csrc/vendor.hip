@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -124,8 +125,9 @@ static int vendor_setup(mf_matrix m)
         std::copy(c2.begin(), c2.end(), ci.begin() + a);
         std::copy(v2.begin(), v2.end(), va.begin() + a);
     }
-    VendorCtx *V = new VendorCtx();
-    m->vendor = V;
+    // attached to the matrix only once every step has succeeded
+    std::unique_ptr<VendorCtx> owner(new VendorCtx());
+    VendorCtx *V = owner.get();
     V->n = n;
     const size_t vb = sizeof(double) * (size_t)std::max(n, 1);
     MF_HIP(hipMalloc((void **)&V->rp, sizeof(int) * (n + 1)));
@@ -178,6 +180,7 @@ static int vendor_setup(mf_matrix m)
     rocsparse_destroy_dnvec_descr(dx);
     rocsparse_destroy_dnvec_descr(dy);
     MF_HIP(hipStreamSynchronize(stream()));
+    m->vendor = owner.release();
     return MF_OK;
 }
 

@@ -32,6 +32,7 @@
 #include <vector>
 
 #include "mf_matrix.h"
+#include "persistent.h"
 
 namespace mf {
 
@@ -108,605 +109,14 @@ __global__ void k_axpy(double *__restrict__ x, const double *__restrict__ d, lon
     if (i < n) x[i] += d[i];
 }
 
-// --------------------------------------------------------- solver kernels
-constexpr int SYNC_K = 4;  // values per grid allreduce, at most
-
-struct SolveArgs {
-    DevL L;
-    DevL L64;          // FP64 layout for the true-residual passes of a lossy layout
-    int use64;         // 1: passes use L64 (headers read from L64), else L
-    int tilem, G, trw;
-    const double *b;
-    double *x;
-    double *g64;       // published Krylov vector (FP64)
-    float *g32;        // its FP32 copy for low-precision tiles
-    double *h64, *q64; // publishing slots of the true-residual pass
-    unsigned long long *slots;  // sync words [2][G][SYNC_K][2]
-    int *ctl;          // 0 abort, 1 status, 2 iterations, 3 replacements
-    double *out;       // 0 ||b||, 1 final ||r_true||^2, 2 final updated ||r||^2
-    double tol, rr_factor;
-    int maxit, has64, haslow;
-    long long spin_limit;
-    unsigned long long *phase;  // optional: [0] spmv, [1] reduce after spmv,
-                                // [2] vector update, [3] reduce after update,
-                                // [4] true-residual passes (100 MHz ticks, block 0)
-};
-
-// Phase clock of block 0 (diagnostic; enabled when A.phase is set).
-struct PhaseClock {
-    unsigned long long *acc;
-    unsigned long long t;
-    __device__ PhaseClock(unsigned long long *p) : acc(blockIdx.x == 0 && threadIdx.x == 0 ? p : nullptr), t(0)
-    {
-        if (acc) t = wall_clock64();
-    }
-    __device__ void lap(int k)
-    {
-        if (acc) {
-            const unsigned long long now = wall_clock64();
-            acc[k] += now - t;
-            t = now;
-        }
-    }
-};
-
-constexpr unsigned PHASE_EP0 = 201;
-
-// Stores of vector entries that other workgroups read after the next grid
-// sync: agent-scope stores, complete (visible to every XCD) once the storing
-// wave's memory counter has drained.
-__device__ __forceinline__ void st_agent(double *p, double v)
-{
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_agent(float *p, float v)
-{
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Grid synchronisation and deterministic allreduce without read-modify-write
-// atomics or cache write-back.  Every workgroup writes each FP64 partial as
-// two 64-bit words, each holding the sync epoch in its high half and one half
-// of the partial in its low half; a word is stored and loaded whole, so a
-// word showing the current epoch holds the current half whatever order the
-// words arrive in.  Wave 0 of every workgroup polls all G x N x 2 words until
-// every epoch matches, then sums the partials in workgroup order (lane-strided
-// sums, then a fixed butterfly), so all workgroups obtain identical bits and
-// take identical branches.  Two word buffers alternate by epoch parity: a
-// workgroup writes buffer p at epoch e only after passing epoch e - 1, which
-// every workgroup reaches only after it has read buffer p at epoch e - 2.
-// publish: vector entries written before the sync (with st_agent) are read
-// by other workgroups after it -- each wave drains its stores before the
-// workgroup barrier that precedes the epoch words, and the readers invalidate
-// their caches (agent-scope acquire) after the poll.  The wait is bounded:
-// after spin_limit polls the workgroup raises the abort word and every
-// workgroup leaves (returns false).
-template <int TPB, int K>
-struct Reducer {
-    static_assert(K <= SYNC_K, "reducer width");
-    double *s_red;  // [TPB/64][K]
-    double *s_bc;   // [K]
-    int *s_flag;
-    unsigned ep = 0;
-
-    template <int N>
-    __device__ bool allreduce(const SolveArgs &A, double (&v)[N], bool publish)
-    {
-        static_assert(N >= 1 && N <= K, "reducer width");
-        const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-#pragma unroll
-        for (int k = 0; k < N; k++) {
-            double s = wave_sum(v[k]);
-            if (lane == 0) s_red[wv * K + k] = s;
-        }
-        if (publish) __builtin_amdgcn_s_waitcnt(0);  // this wave's stores are complete
-        __syncthreads();
-        ++ep;
-        // diagnostic stamps of syncs PHASE_EP0 .. PHASE_EP0 + 3: arrival and
-        // exit time of every workgroup
-        const bool stamp = A.phase && tid == 0 && ep - PHASE_EP0 < 4u;
-        unsigned long long *st = stamp ? A.phase + 16 + (size_t)(ep - PHASE_EP0) * 2 * A.G + 2 * blockIdx.x : nullptr;
-        if (stamp) st[0] = wall_clock64();
-        if (tid < 64) {
-            const unsigned G = (unsigned)A.G;
-            // each workgroup's words fill one 64-byte line of their own (stores
-            // of different workgroups never share a line)
-            unsigned long long *W = A.slots + (size_t)(ep & 1u) * G * SYNC_K * 2;
-            const unsigned long long tag = (unsigned long long)ep << 32;
-            if (lane < 2 * N) {
-                const int k = lane >> 1;
-                double s = 0.0;
-                for (int w = 0; w < TPB / 64; w++) s += s_red[w * K + k];
-                const unsigned long long bits = (unsigned long long)__double_as_longlong(s);
-                const unsigned long long half = (lane & 1) ? (bits >> 32) : (bits & 0xffffffffull);
-                __hip_atomic_store(W + ((size_t)blockIdx.x * SYNC_K + k) * 2 + (lane & 1), tag | half,
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-            double acc[N];
-            long long spins = 0;
-            int ab = 0;
-            for (;;) {
-                bool ok = true;
-#pragma unroll
-                for (int k = 0; k < N; k++) acc[k] = 0.0;
-                for (unsigned g = lane; g < G; g += 64) {
-#pragma unroll
-                    for (int k = 0; k < N; k++) {
-                        const unsigned long long *w = W + ((size_t)g * SYNC_K + k) * 2;
-                        const unsigned long long lo = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        const unsigned long long hi = __hip_atomic_load(w + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        ok = ok && (lo >> 32) == ep && (hi >> 32) == ep;
-                        acc[k] += __longlong_as_double((long long)((hi << 32) | (lo & 0xffffffffull)));
-                    }
-                }
-                if (__all(ok)) break;
-                if (__builtin_amdgcn_readfirstlane(ld_agent(A.ctl))) { ab = 1; break; }
-                if (++spins > A.spin_limit) {
-                    if (lane == 0) __hip_atomic_store(A.ctl, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    ab = 1;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-            }
-            if (!ab) {
-#pragma unroll
-                for (int k = 0; k < N; k++) {
-                    const double s = wave_sum(acc[k]);
-                    if (lane == 0) s_bc[k] = s;
-                }
-                if (publish) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            }
-            if (lane == 0) *s_flag = ab;
-        }
-        if (stamp) st[1] = wall_clock64();
-        __syncthreads();
-        if (*s_flag) return false;
-#pragma unroll
-        for (int k = 0; k < N; k++) v[k] = s_bc[k];
-        return true;
-    }
-
-    // Grid barrier after publishing vector entries.
-    __device__ bool barrier(const SolveArgs &A)
-    {
-        double z[1] = {0.0};
-        return allreduce(A, z, true);
-    }
-};
-
-// tile rows whose loads are interleaved inside the persistent kernels' SpMV
-// (four-byte indices take four times the index registers: two rows)
-__host__ __device__ constexpr int spmv_rows(int rpt, int iw)
-{
-    return rpt < (iw < 4 ? 4 : 2) ? rpt : (iw < 4 ? 4 : 2);
-}
-
-__device__ __forceinline__ bool finite_all(double a, double b) { return isfinite(a) && isfinite(b); }
-
-// Hides the value of v from the optimiser.  Applied to the thread's row base
-// at the top of every solver iteration so that the 64-bit addresses of the
-// RPT rows are recomputed where they are used instead of being hoisted out of
-// the loop (which would keep RPT x arrays addresses live and spill).
-__device__ __forceinline__ int opaque(int v)
-{
-    asm volatile("" : "+v"(v));
-    return v;
-}
-
-// CG (Chronopoulos-Gear): registers hold x and p of the thread's rows, LDS
-// holds r and s = A p.  Per iteration: [r published, (r,r) reduced] ->
-// convergence test -> w = A r with s = w + beta s and (w, r) in the same
-// pass -> reduce (w, r) -> p, x, r updates -> publish r, reduce (r, r).
-template <int B, int IW, int TPB, int RPT>
-__global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
-{
-    __shared__ double s_r[RPT * TPB];
-    __shared__ double s_s[RPT * TPB];
-    __shared__ double s_red[(TPB / 64) * 4];
-    __shared__ double s_bc[4];
-    __shared__ int s_flag;
-    Reducer<TPB, 4> R{s_red, s_bc, &s_flag};
-    constexpr int GPB = TPB / B, NR = spmv_rows(RPT, IW);
-    const int tid = threadIdx.x;
-    const int chunk = xcd_chunk(blockIdx.x, gridDim.x);
-    const int grp = tid / B, lane = tid % B, lanebase = (tid & 63) & ~(B - 1);
-    const int tr0 = chunk * A.trw, tr1 = min(A.tilem, tr0 + A.trw);
-    double xr[RPT], pr[RPT];
-    // headers of the block's tile rows, loaded once: the per-iteration SpMV
-    // then issues the tile-column word and the slab loads together
-    constexpr int TRW = RPT * GPB;
-    __shared__ int4 s_h[2 * TRW];
-    __shared__ int s_t0[TRW + 1];
-    for (int q = tid; q <= TRW; q += TPB) {
-        const int tr = tr0 + q;
-        if (q < TRW && tr < tr1) {
-            const int4 *h = reinterpret_cast<const int4 *>(A.L.trh + (size_t)tr * 4);
-            s_h[2 * q] = h[0];
-            s_h[2 * q + 1] = h[1];
-        } else if (q < TRW) {  // beyond the chunk: no slabs
-            s_h[2 * q] = s_h[2 * q + 1] = make_int4(0, 0, 0, 0);
-        }
-        s_t0[q] = A.L.tile_ptr[min(tr, A.tilem)];
-    }
-    __syncthreads();
-    int rbase = tr0 * B + tid;  // row of step i is rbase + i * TPB
-    auto tile_row = [&](int i) { return tr0 + i * GPB + grp; };
-    auto row_of = [&](int i) { return (size_t)(rbase + i * TPB); };
-    auto publish = [&](size_t row, double v) {
-        if (A.has64) st_agent(A.g64 + row, v);
-        if (A.haslow) st_agent(A.g32 + row, (float)v);
-    };
-    auto finish = [&](int status, int iter, int repl, double rt2, double g2) {
-#pragma clang loop unroll(full)
-        for (int i = 0; i < RPT; i++)
-            if (tile_row(i) < tr1) A.x[row_of(i)] = xr[i];
-        if (blockIdx.x == 0 && tid == 0) {
-            A.ctl[1] = status;
-            A.ctl[2] = iter;
-            A.ctl[3] = repl;
-            A.out[1] = rt2;
-            A.out[2] = g2;
-        }
-    };
-
-    double v1[1] = {0.0};
-#pragma clang loop unroll(full)
-    for (int i = 0; i < RPT; i++) {
-        xr[i] = pr[i] = 0.0;
-        s_r[i * TPB + tid] = 0.0;
-        s_s[i * TPB + tid] = 0.0;
-        if (tile_row(i) < tr1) {
-            const size_t row = row_of(i);
-            const double bv = A.b[row];
-            s_r[i * TPB + tid] = bv;
-            publish(row, bv);
-            v1[0] += bv * bv;
-        }
-    }
-    if (!R.allreduce(A, v1, true)) return;
-    const double bnorm2 = v1[0];
-    if (blockIdx.x == 0 && tid == 0) A.out[0] = sqrt(bnorm2);
-    const double tol2 = A.tol * A.tol * bnorm2;
-    if (bnorm2 == 0.0) { finish(MF_OK, 0, 0, 0.0, 0.0); return; }
-    double gamma = bnorm2, ref2 = bnorm2, gprev = 1.0, aprev = 1.0;
-    bool restart = false;   // next iteration starts a new direction (beta = 0)
-    bool replaced = false;  // r was replaced before this iteration
-    bool force = false;     // a breakdown asks for a true-residual pass + restart
-    double rs_repl = 0.0, ps_repl = 0.0;
-    int iter = 0, repl = 0;
-    PhaseClock clk(A.phase);
-    for (;;) {
-        rbase = opaque(rbase);
-        if (!isfinite(gamma)) { finish(MF_E_NONFINITE, iter, repl, -1.0, gamma); return; }
-        const bool conv = gamma <= tol2;
-        const bool due = force || (A.rr_factor > 0.0 && gamma <= A.rr_factor * A.rr_factor * ref2);
-        if (conv || due) {
-            // true residual b - A x and s = A p in one FP64-accumulated pass
-#pragma clang loop unroll(full)
-            for (int i = 0; i < RPT; i++)
-                if (tile_row(i) < tr1) {
-                    st_agent(A.h64 + row_of(i), xr[i]);
-                    st_agent(A.q64 + row_of(i), pr[i]);
-                }
-            if (!R.barrier(A)) return;
-            // (r, r), (r, s), (p, s) of the replaced r and refreshed s = A p
-            double t1[3] = {0.0, 0.0, 0.0};
-#pragma unroll 1
-            for (int i = 0; i < RPT; i++) {
-                const int tr = tile_row(i);
-                if (tr < tr1) {
-                    const double *g[2] = {A.h64, A.q64};
-                    double acc[2] = {0.0, 0.0};
-                    const int q = i * GPB + grp;
-                    if (A.use64)
-                        tile_row_spmv_f64_c<B, IW, 2>(A.L64, row_ctx<B, IW>(A.L64, tr, lane), lane, lanebase, g, acc);
-                    else
-                        tile_row_spmv_f64_c<B, IW, 2>(
-                            A.L, row_ctx_from<B, IW>(A.L, tr, s_t0[q], s_t0[q + 1] - s_t0[q], s_h[2 * q], s_h[2 * q + 1], lane),
-                            lane, lanebase, g, acc);
-                    const size_t row = (size_t)tr * B + lane;
-                    const double rv = A.b[row] - acc[0];
-                    s_r[i * TPB + tid] = rv;
-                    s_s[i * TPB + tid] = acc[1];
-                    publish(row, rv);
-                    t1[0] += rv * rv;
-                    t1[1] += rv * acc[1];
-                    t1[2] += pr[i] * acc[1];
-                }
-            }
-            if (!R.allreduce(A, t1, true)) return;
-            clk.lap(4);
-            ++repl;
-            if (!isfinite(t1[0])) { finish(MF_E_NONFINITE, iter, repl, t1[0], gamma); return; }
-            if (t1[0] <= tol2) { finish(MF_OK, iter, repl, t1[0], gamma); return; }
-            // reliable update: keep the direction p.  The next step's
-            // (p', A p') with p' = r + beta p, A p' = A r + beta s is then
-            // (A r, r) + 2 beta (r, s) + beta^2 (p, s) (A symmetric), not the
-            // recurrence's form.  When the recurrence had drifted by more
-            // than a factor 2 in norm, restart from the true residual.
-            restart = force || t1[0] > 4.0 * gamma;
-            force = false;
-            replaced = true;
-            rs_repl = t1[1];
-            ps_repl = t1[2];
-            gamma = ref2 = t1[0];
-            continue;
-        }
-        if (iter >= A.maxit) { finish(MF_E_MAXIT, iter, repl, -1.0, gamma); return; }
-        const bool fresh = iter == 0 || restart, after_repl = replaced && !fresh;
-        restart = replaced = false;
-        const double beta = fresh ? 0.0 : gamma / gprev;
-        // w = A r; s = w + beta s; (w, r)
-        double red[1] = {0.0};
-#pragma unroll 1
-        for (int i0 = 0; i0 < RPT; i0 += NR) {
-            double w[NR];
-            tile_rows_spmv<B, IW, NR, GPB>(A.L, s_h, s_t0, i0 * GPB + grp, tile_row(i0), lane, lanebase, A.g64, A.g32, w);
-#pragma unroll
-            for (int k = 0; k < NR; k++) {
-                const int i = i0 + k;
-                if (tile_row(i) < tr1) {
-                    red[0] += w[k] * s_r[i * TPB + tid];
-                    s_s[i * TPB + tid] = w[k] + beta * s_s[i * TPB + tid];
-                }
-            }
-        }
-        clk.lap(0);
-        if (!R.allreduce(A, red, false)) return;
-        clk.lap(1);
-        const double delta = red[0];
-        const double denom = fresh        ? delta
-                             : after_repl ? delta + beta * (2.0 * rs_repl + beta * ps_repl)
-                                          : delta - beta * gamma / aprev;
-        if (!isfinite(denom)) { finish(MF_E_NONFINITE, iter, repl, -1.0, gamma); return; }
-        if (!(denom > 0.0)) {
-            // (p, A p) <= 0 from rounding: recompute the residual and restart;
-            // a breakdown right after a restart is final
-            if (fresh) { finish(MF_E_BREAKDOWN, iter, repl, -1.0, gamma); return; }
-            force = true;
-            continue;
-        }
-        const double alpha = gamma / denom;
-#pragma clang loop unroll(full)
-        for (int i = 0; i < RPT; i++)
-            if (tile_row(i) < tr1) {
-                pr[i] = s_r[i * TPB + tid] + beta * pr[i];
-                xr[i] = xr[i] + alpha * pr[i];
-            }
-        double g1[1] = {0.0};
-#pragma unroll 1
-        for (int i = 0; i < RPT; i++)
-            if (tile_row(i) < tr1) {
-                const double rv = s_r[i * TPB + tid] - alpha * s_s[i * TPB + tid];
-                s_r[i * TPB + tid] = rv;
-                publish(row_of(i), rv);
-                g1[0] += rv * rv;
-            }
-        gprev = gamma;
-        aprev = alpha;
-        ++iter;
-        clk.lap(2);
-        if (!R.allreduce(A, g1, true)) return;
-        clk.lap(3);
-        gamma = g1[0];
-    }
-}
-
-// BiCGSTAB: registers hold x, r, r^, p, s; LDS holds v = A p and t = A s.
-template <int B, int IW, int TPB, int RPT>
-__global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
-{
-    __shared__ double s_v[RPT * TPB];
-    __shared__ double s_t[RPT * TPB];
-    __shared__ double s_red[(TPB / 64) * 4];
-    __shared__ double s_bc[4];
-    __shared__ int s_flag;
-    Reducer<TPB, 4> R{s_red, s_bc, &s_flag};
-    constexpr int GPB = TPB / B, NR = spmv_rows(RPT, IW);
-    const int tid = threadIdx.x;
-    const int chunk = xcd_chunk(blockIdx.x, gridDim.x);
-    const int grp = tid / B, lane = tid % B, lanebase = (tid & 63) & ~(B - 1);
-    const int tr0 = chunk * A.trw, tr1 = min(A.tilem, tr0 + A.trw);
-    double xr[RPT], rr[RPT], rh[RPT], pr[RPT], sr[RPT];
-    // headers of the block's tile rows, loaded once: the per-iteration SpMV
-    // then issues the tile-column word and the slab loads together
-    constexpr int TRW = RPT * GPB;
-    __shared__ int4 s_h[2 * TRW];
-    __shared__ int s_t0[TRW + 1];
-    for (int q = tid; q <= TRW; q += TPB) {
-        const int tr = tr0 + q;
-        if (q < TRW && tr < tr1) {
-            const int4 *h = reinterpret_cast<const int4 *>(A.L.trh + (size_t)tr * 4);
-            s_h[2 * q] = h[0];
-            s_h[2 * q + 1] = h[1];
-        } else if (q < TRW) {  // beyond the chunk: no slabs
-            s_h[2 * q] = s_h[2 * q + 1] = make_int4(0, 0, 0, 0);
-        }
-        s_t0[q] = A.L.tile_ptr[min(tr, A.tilem)];
-    }
-    __syncthreads();
-    int rbase = tr0 * B + tid;  // row of step i is rbase + i * TPB
-    auto tile_row = [&](int i) { return tr0 + i * GPB + grp; };
-    auto row_of = [&](int i) { return (size_t)(rbase + i * TPB); };
-    auto publish = [&](size_t row, double v) {
-        if (A.has64) st_agent(A.g64 + row, v);
-        if (A.haslow) st_agent(A.g32 + row, (float)v);
-    };
-    auto finish = [&](int status, int iter, int repl, double rt2, double g2) {
-#pragma clang loop unroll(full)
-        for (int i = 0; i < RPT; i++)
-            if (tile_row(i) < tr1) A.x[row_of(i)] = xr[i];
-        if (blockIdx.x == 0 && tid == 0) {
-            A.ctl[1] = status;
-            A.ctl[2] = iter;
-            A.ctl[3] = repl;
-            A.out[1] = rt2;
-            A.out[2] = g2;
-        }
-    };
-    auto spmv_into = [&](double *dst) {
-#pragma unroll 1
-        for (int i0 = 0; i0 < RPT; i0 += NR) {
-            double w[NR];
-            tile_rows_spmv<B, IW, NR, GPB>(A.L, s_h, s_t0, i0 * GPB + grp, tile_row(i0), lane, lanebase, A.g64, A.g32, w);
-#pragma unroll
-            for (int k = 0; k < NR; k++)
-                if (tile_row(i0 + k) < tr1) dst[(i0 + k) * TPB + tid] = w[k];
-        }
-    };
-
-    double v1[1] = {0.0};
-#pragma clang loop unroll(full)
-    for (int i = 0; i < RPT; i++) {
-        xr[i] = rr[i] = rh[i] = pr[i] = sr[i] = 0.0;
-        s_v[i * TPB + tid] = 0.0;
-        if (tile_row(i) < tr1) {
-            const size_t row = row_of(i);
-            rr[i] = rh[i] = pr[i] = A.b[row];
-            publish(row, pr[i]);
-            v1[0] += rr[i] * rr[i];
-        }
-    }
-    if (!R.allreduce(A, v1, true)) return;
-    const double bnorm2 = v1[0];
-    if (blockIdx.x == 0 && tid == 0) A.out[0] = sqrt(bnorm2);
-    const double tol2 = A.tol * A.tol * bnorm2;
-    if (bnorm2 == 0.0) { finish(MF_OK, 0, 0, 0.0, 0.0); return; }
-    double rho = bnorm2, ref2 = bnorm2;
-    bool restart = false;
-    int iter = 0, repl = 0;
-    for (;;) {
-        rbase = opaque(rbase);
-        spmv_into(s_v);  // v = A p
-        double a1[1] = {0.0};
-#pragma clang loop unroll(full)
-        for (int i = 0; i < RPT; i++)
-            if (tile_row(i) < tr1) a1[0] += rh[i] * s_v[i * TPB + tid];
-        if (!R.allreduce(A, a1, false)) return;
-        if (!(a1[0] != 0.0) || !isfinite(a1[0])) { finish(MF_E_BREAKDOWN, iter, repl, -1.0, -1.0); return; }
-        const double alpha = rho / a1[0];
-#pragma clang loop unroll(full)
-        for (int i = 0; i < RPT; i++)
-            if (tile_row(i) < tr1) {
-                sr[i] = rr[i] - alpha * s_v[i * TPB + tid];
-                publish(row_of(i), sr[i]);
-            }
-        if (!R.barrier(A)) return;
-        spmv_into(s_t);  // t = A s
-        double a2[2] = {0.0, 0.0};
-#pragma clang loop unroll(full)
-        for (int i = 0; i < RPT; i++)
-            if (tile_row(i) < tr1) {
-                const double t = s_t[i * TPB + tid];
-                a2[0] += t * sr[i];
-                a2[1] += t * t;
-            }
-        if (!R.allreduce(A, a2, false)) return;
-        const double omega = a2[1] > 0.0 ? a2[0] / a2[1] : 0.0;
-        double a3[2] = {0.0, 0.0};
-#pragma clang loop unroll(full)
-        for (int i = 0; i < RPT; i++)
-            if (tile_row(i) < tr1) {
-                xr[i] = xr[i] + alpha * pr[i] + omega * sr[i];
-                rr[i] = sr[i] - omega * s_t[i * TPB + tid];
-                a3[0] += rr[i] * rh[i];
-                a3[1] += rr[i] * rr[i];
-            }
-        if (!R.allreduce(A, a3, false)) return;
-        double rho_new = a3[0], g2 = a3[1];
-        ++iter;
-        if (!finite_all(rho_new, g2)) { finish(MF_E_NONFINITE, iter, repl, -1.0, g2); return; }
-        const bool conv = g2 <= tol2;
-        const bool due = A.rr_factor > 0.0 && g2 <= A.rr_factor * A.rr_factor * ref2;
-        if (conv || due) {
-#pragma clang loop unroll(full)
-            for (int i = 0; i < RPT; i++)
-                if (tile_row(i) < tr1) st_agent(A.h64 + row_of(i), xr[i]);
-            if (!R.barrier(A)) return;
-#pragma unroll 1
-            for (int i = 0; i < RPT; i++) {
-                const int tr = tile_row(i);
-                if (tr < tr1) {
-                    const double *g[1] = {A.h64};
-                    double acc[1] = {0.0};
-                    const int q = i * GPB + grp;
-                    if (A.use64)
-                        tile_row_spmv_f64_c<B, IW, 1>(A.L64, row_ctx<B, IW>(A.L64, tr, lane), lane, lanebase, g, acc);
-                    else
-                        tile_row_spmv_f64_c<B, IW, 1>(
-                            A.L, row_ctx_from<B, IW>(A.L, tr, s_t0[q], s_t0[q + 1] - s_t0[q], s_h[2 * q], s_h[2 * q + 1], lane),
-                            lane, lanebase, g, acc);
-                    s_t[i * TPB + tid] = A.b[(size_t)tr * B + lane] - acc[0];
-                }
-            }
-            double t2[2] = {0.0, 0.0};
-#pragma clang loop unroll(full)
-            for (int i = 0; i < RPT; i++)
-                if (tile_row(i) < tr1) {
-                    rr[i] = s_t[i * TPB + tid];
-                    t2[0] += rr[i] * rr[i];
-                    t2[1] += rr[i] * rh[i];
-                }
-            if (!R.allreduce(A, t2, false)) return;
-            ++repl;
-            if (!isfinite(t2[0])) { finish(MF_E_NONFINITE, iter, repl, t2[0], g2); return; }
-            if (t2[0] <= tol2) { finish(MF_OK, iter, repl, t2[0], g2); return; }
-            restart = t2[0] > 4.0 * g2;  // recurrence drifted: new direction p = r
-            ref2 = t2[0];
-            rho_new = t2[1];
-        }
-        if (iter >= A.maxit) { finish(MF_E_MAXIT, iter, repl, -1.0, g2); return; }
-        if (omega == 0.0 || rho == 0.0) { finish(MF_E_BREAKDOWN, iter, repl, -1.0, g2); return; }
-        const double beta = restart ? 0.0 : (rho_new / rho) * (alpha / omega);
-        restart = false;
-#pragma clang loop unroll(full)
-        for (int i = 0; i < RPT; i++)
-            if (tile_row(i) < tr1) {
-                pr[i] = rr[i] + beta * (pr[i] - omega * s_v[i * TPB + tid]);
-                publish(row_of(i), pr[i]);
-            }
-        rho = rho_new;
-        if (!R.barrier(A)) return;
-    }
-}
-
-// ------------------------------------------------------------ dispatch
-using KernelFn = void (*)(SolveArgs);
-
-struct KernelChoice {
-    KernelFn fn = nullptr;
-    int tpb = 0, rpt = 0;
-};
-
-// 512 threads (8 waves per CU, 256 VGPRs per wave) hold the register-resident
-// vectors of up to 16 rows per thread without spilling.
-template <int B, int IW>
-static KernelChoice pick_kernel(int method, long long rows_per_wg)
-{
-    KernelChoice k;
-    const int q = (int)((rows_per_wg + 511) / 512);
-    if (method == MF_CG) {
-        if (q <= 1) k = {k_cg<B, IW, 512, 1>, 512, 1};
-        else if (q <= 2) k = {k_cg<B, IW, 512, 2>, 512, 2};
-        else if (q <= 4) k = {k_cg<B, IW, 512, 4>, 512, 4};
-        else if (q <= 8) k = {k_cg<B, IW, 512, 8>, 512, 8};
-        else if (q <= 16) k = {k_cg<B, IW, 512, 16>, 512, 16};
-    } else {
-        if (q <= 1) k = {k_bicgstab<B, IW, 512, 1>, 512, 1};
-        else if (q <= 2) k = {k_bicgstab<B, IW, 512, 2>, 512, 2};
-        else if (q <= 4) k = {k_bicgstab<B, IW, 512, 4>, 512, 4};
-        else if (q <= 8) k = {k_bicgstab<B, IW, 512, 8>, 512, 8};
-    }
-    return k;
-}
-
 static KernelChoice pick(int B, int iw, int method, long long rows_per_wg)
 {
-    if (B == 16)
-        return iw == 1   ? pick_kernel<16, 1>(method, rows_per_wg)
-               : iw == 2 ? pick_kernel<16, 2>(method, rows_per_wg)
-                         : pick_kernel<16, 4>(method, rows_per_wg);
-    return KernelChoice{};
+    switch (B) {
+    case 16: return pick_tile<16>(iw, method, rows_per_wg);
+    case 8: return pick_b8(iw, method, rows_per_wg);
+    case 32: return pick_b32(iw, method, rows_per_wg);
+    default: return KernelChoice{};
+    }
 }
 
 void Workspace::release()

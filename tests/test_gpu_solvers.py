@@ -394,3 +394,31 @@ def test_gpu_dropin_bicgstab_csv(gpu, tmp_path):
         # path,itr,time,time,nnzR,l2_norm,residual,each_block_nnz,1
         assert len(row) == 9 and row[0] == mtx and int(row[1]) > 0 and int(row[4]) == int(g["nnz"])
         assert float(row[5]) <= TOL and int(row[8]) == 1
+
+
+@pytest.mark.parametrize("B", [8, 32])
+def test_gpu_solvers_tile_sizes(gpu, orc, B):
+    # the persistent CG and BiCGSTAB built for 8- and 32-row tiles reach the
+    # tolerance in the oracle's iteration counts
+    rp, ci, va = gpu.gen_poisson3d(24, 24, 20)
+    n = len(rp) - 1
+    b = orc.csr_spmv(rp, ci, va, gpu.gen_uniform(12, n, -1, 1))
+    A = gpu.Matrix(rp, ci, va, B=B)
+    x, rep = A.solve(b, method=gpu.MF_CG, tol=TOL)
+    xo, orep = orc.cg(rp, ci, va, b, TOL, 20000)
+    print(f"B={B} cg it={rep['iterations']} oracle {orep.iterations}", flush=True)
+    assert rep["status"] == 0 and _true_relres(rp, ci, va, x, b) <= 1.5 * TOL
+    assert _iters_close(rep["iterations"], orep.iterations)
+    xm, repm = A.solve(b, method=gpu.MF_CG, tol=TOL, precision=gpu.MF_MIXED, tau=0.0, rr_factor=1e-2)
+    assert repm["status"] == 0 and _true_relres(rp, ci, va, xm, b) <= 1.5 * TOL
+    A.close()
+    rp, ci, va = gpu.gen_convdiff3d(18, 10.0, 1.0, 0)
+    n = len(rp) - 1
+    b = orc.csr_spmv(rp, ci, va, gpu.gen_uniform(13, n, -1, 1))
+    A = gpu.Matrix(rp, ci, va, B=B)
+    x, rep = A.solve(b, method=gpu.MF_BICGSTAB, tol=TOL)
+    xo, orep = orc.bicgstab(rp, ci, va, b, TOL, 20000)
+    print(f"B={B} bicgstab it={rep['iterations']} oracle {orep.iterations}", flush=True)
+    assert rep["status"] == 0 and _true_relres(rp, ci, va, x, b) <= 1.5 * TOL
+    assert _iters_close(rep["iterations"], orep.iterations)
+    A.close()

@@ -839,7 +839,9 @@ static int batch_capture(mf_batch_s *bt)
     A.sys_out = bt->d_sys_out;
     A.sys_relres = bt->d_relres;
     A.tol = bt->o.tol;
-    A.rr_factor = bt->o.rr_factor;
+    // an inner layout without low-precision tiles is the FP64 operator with
+    // FP64 arithmetic: replacements only at convergence, as in FP64 mode
+    A.rr_factor = Li.haslow ? bt->o.rr_factor : 0.0;
     A.maxit = bt->o.maxit;
     A.haslow = Li.haslow ? 1 : 0;
     A.spin_limit = bt->o.spin_limit;

@@ -194,7 +194,7 @@ typedef struct {
     int index_width;    /* slab index width, as mf_matrix_set_index_width
                            (0 = the matrix's setting; batched: 0 = automatic,
                            one width that every system fits) */
-    int profile;        /* CG: 1 = time the phases of the persistent launch
+    int profile;        /* 1 = time the phases of the persistent launch
                            (workgroup 0's clock) into mf_report.phase_us */
     int reorder;        /* MF_REORDER_AUTO (default): the solvers run on P A P^T
                            with rows sorted by length and tile rows dealt to the
@@ -212,9 +212,11 @@ typedef struct {
     double ms_solve;     /* time to tolerance (device events, inputs resident) */
     double ms_setup;     /* layout build */
     int grid, block, rows_per_thread;
-    double phase_us[5];  /* opts.profile (CG): per iteration SpMV, sync after it,
-                            vector update, sync after it; [4] = true-residual
-                            passes, total; of the last persistent launch */
+    double phase_us[5];  /* opts.profile, per iteration of the last persistent
+                            launch: CG: SpMV, sync after it, vector update, sync
+                            after it; BiCGSTAB: both SpMVs, the reductions after
+                            them, vector updates, the other three syncs;
+                            [4] = true-residual passes, total */
     int reordered;       /* 1: solved on the row-reordered copy */
     double pad_factor;   /* stored slab entries / nonzeros of the natural FP64 layout */
 } mf_report;

@@ -476,14 +476,17 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
     double rho = bnorm2, ref2 = bnorm2;
     bool restart = false;
     int iter = 0, repl = 0;
+    PhaseClock clk(A.phase);
     for (;;) {
         rbase = opaque(rbase);
         spmv_into(s_v);  // v = A p
+        clk.lap(0);
         double a1[1] = {0.0};
 #pragma clang loop unroll(full)
         for (int i = 0; i < RPT; i++)
             if (tile_row(i) < tr1) a1[0] += rh[i] * s_v[i * TPB + tid];
         if (!R.allreduce(A, a1, false)) return;
+        clk.lap(1);
         if (!(a1[0] != 0.0) || !isfinite(a1[0])) { finish(MF_E_BREAKDOWN, iter, repl, -1.0, -1.0); return; }
         const double alpha = rho / a1[0];
 #pragma clang loop unroll(full)
@@ -492,8 +495,11 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
                 sr[i] = rr[i] - alpha * s_v[i * TPB + tid];
                 publish(row_of(i), sr[i]);
             }
+        clk.lap(2);
         if (!R.barrier(A)) return;
+        clk.lap(3);
         spmv_into(s_t);  // t = A s
+        clk.lap(0);
         double a2[2] = {0.0, 0.0};
 #pragma clang loop unroll(full)
         for (int i = 0; i < RPT; i++)
@@ -503,6 +509,7 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
                 a2[1] += t * t;
             }
         if (!R.allreduce(A, a2, false)) return;
+        clk.lap(1);
         const double omega = a2[1] > 0.0 ? a2[0] / a2[1] : 0.0;
         double a3[2] = {0.0, 0.0};
 #pragma clang loop unroll(full)
@@ -513,7 +520,9 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
                 a3[0] += rr[i] * rh[i];
                 a3[1] += rr[i] * rr[i];
             }
+        clk.lap(2);
         if (!R.allreduce(A, a3, false)) return;
+        clk.lap(3);
         double rho_new = a3[0], g2 = a3[1];
         ++iter;
         if (!finite_all(rho_new, g2)) { finish(MF_E_NONFINITE, iter, repl, -1.0, g2); return; }
@@ -549,6 +558,7 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
                     t2[1] += rr[i] * rh[i];
                 }
             if (!R.allreduce(A, t2, false)) return;
+            clk.lap(4);
             ++repl;
             if (!isfinite(t2[0])) { finish(MF_E_NONFINITE, iter, repl, t2[0], g2); return; }
             if (t2[0] <= tol2) { finish(MF_OK, iter, repl, t2[0], g2); return; }
@@ -567,7 +577,9 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
                 publish(row_of(i), pr[i]);
             }
         rho = rho_new;
+        clk.lap(2);
         if (!R.barrier(A)) return;
+        clk.lap(3);
     }
 }
 

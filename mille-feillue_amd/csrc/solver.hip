@@ -226,7 +226,7 @@ static int launch_inner(mf_matrix m, const Layout &L, const Layout *L64, int met
     static const bool phase_profile = std::getenv("MF_PHASE_PROFILE") != nullptr;
     static unsigned long long *d_phase = nullptr;
     const size_t phase_words = 16 + 8 * 1024;
-    const bool prof = phase_profile || (o.profile && method == MF_CG);
+    const bool prof = phase_profile || o.profile;
     if (prof && !d_phase) MF_HIP(hipMalloc((void **)&d_phase, phase_words * sizeof(unsigned long long)));
     A.phase = prof ? d_phase : nullptr;
     if (A.phase) MF_HIP(hipMemsetAsync(d_phase, 0, phase_words * sizeof(unsigned long long), st));

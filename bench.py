@@ -35,9 +35,10 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(rp, ci, va, b, gpu_iters):
+def cpu_baseline(rp, ci, va, b, gpu_iters, gpu_x=None):
     """The oracle's tile-SpMV CG (FP64, OpenMP over the host cores) run to the
-    same tolerance on the same system."""
+    same tolerance on the same system; gpu_x maps a label to a GPU solution,
+    whose relative error against the oracle's solution is reported."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from orc import Oracle
 
@@ -61,6 +62,8 @@ def cpu_baseline(rp, ci, va, b, gpu_iters):
         "iterations": rep.iterations,
         "petsc_cg": {"ms": round(dp, 3), "iterations": prep.iterations, "relres_true": prep.relres_true,
                      "kind": "port", "cores": cores},
+        "gpu_err_vs_oracle_x": {k: float(np.linalg.norm(v - x) / np.linalg.norm(x))
+                                for k, v in (gpu_x or {}).items()},
     }
 
 
@@ -277,8 +280,9 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
-        cpu = cpu_baseline(rp, ci, va, b, it64)
-        log(f"cpu: {cpu['value']} ms (oracle tile CG), {cpu['petsc_cg']['ms']} ms (PETSc CG restatement)")
+        cpu = cpu_baseline(rp, ci, va, b, it64, {"fp64": x64, "mixed": xmx})
+        log(f"cpu: {cpu['value']} ms (oracle tile CG), {cpu['petsc_cg']['ms']} ms (PETSc CG restatement); "
+            f"GPU x vs oracle x: {cpu['gpu_err_vs_oracle_x']}")
 
     out = {
         "metric": METRIC,

@@ -174,6 +174,7 @@ int mf_matrix_set_index_width(mf_matrix m, int width);
 enum { MF_CG = 0, MF_BICGSTAB = 1 };
 enum { MF_FP64 = 0, MF_MIXED = 1 };
 enum { MF_REORDER_AUTO = 0, MF_REORDER_OFF = 1, MF_REORDER_ON = 2 };
+enum { MF_SYNC_AUTO = 0, MF_SYNC_GLOBAL = 1 };
 typedef struct {
     int method;         /* MF_CG or MF_BICGSTAB */
     int precision;      /* MF_FP64 or MF_MIXED */
@@ -200,6 +201,10 @@ typedef struct {
                            with rows sorted by length and tile rows dealt to the
                            workgroups when the natural slabs hold more than
                            twice the nonzeros; _OFF: never; _ON: always */
+    int sync;           /* BiCGSTAB, MF_SYNC_AUTO (default): after publishing p
+                           or s a workgroup waits only for the workgroups whose
+                           rows its tile columns reference (when none references
+                           more than 64); MF_SYNC_GLOBAL: a grid barrier */
 } mf_opts;
 void mf_opts_default(mf_opts *o);
 
@@ -219,6 +224,8 @@ typedef struct {
                             [4] = true-residual passes, total */
     int reordered;       /* 1: solved on the row-reordered copy */
     double pad_factor;   /* stored slab entries / nonzeros of the natural FP64 layout */
+    int sync_deps;       /* BiCGSTAB neighbour syncs: most workgroups one waits
+                            for after a publish; -1: grid barriers */
 } mf_report;
 
 /* Host buffers in/out (copies included; the timed ms_solve excludes them). */

@@ -111,6 +111,7 @@ void mf_opts_default(mf_opts *o)
     o->spin_limit = 1LL << 22;  // a few seconds of polling
     o->index_width = 0;
     o->reorder = MF_REORDER_AUTO;
+    o->sync = MF_SYNC_AUTO;
 }
 
 int mf_matrix_create(int rowA, int colA, int nnz, const int *rowptr, const int *colidx,

@@ -50,6 +50,13 @@ struct Workspace {
     double *out = nullptr;
     double *r_outer = nullptr, *d = nullptr, *tmp = nullptr;
     double *red = nullptr;
+    // neighbour syncs of the persistent solvers, for the grid (dep_G chunks
+    // of dep_trw tile rows): per chunk the other chunks whose rows its tile
+    // columns reference (dep_ptr[dep_G + 1], dep_idx), the longest such list
+    // (dep_max), and one epoch word per chunk, 64 bytes apart (flags)
+    int dep_G = 0, dep_trw = 0, dep_max = -1;
+    int *dep_ptr = nullptr, *dep_idx = nullptr;
+    unsigned *flags = nullptr;
     void release();
 };
 

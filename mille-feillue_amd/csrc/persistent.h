@@ -22,7 +22,7 @@ struct SolveArgs {
     double *g64;       // published Krylov vector (FP64)
     float *g32;        // its FP32 copy for low-precision tiles
     double *h64, *q64; // publishing slots of the true-residual pass
-    unsigned long long *slots;  // sync words [2][G][SYNC_K][2]
+    unsigned long long *slots;  // sync words [SLOT_BUFS][G][SLOT_WORDS]
     int *ctl;          // 0 abort, 1 status, 2 iterations, 3 replacements
     double *out;       // 0 ||b||, 1 final ||r_true||^2, 2 final updated ||r||^2
     double tol, rr_factor;
@@ -31,6 +31,12 @@ struct SolveArgs {
     unsigned long long *phase;  // optional: [0] spmv, [1] reduce after spmv,
                                 // [2] vector update, [3] reduce after update,
                                 // [4] true-residual passes (100 MHz ticks, block 0)
+    // neighbour syncs (nbr = 1): chunk c's SpMV gathers rows of the chunks
+    // dep_idx[dep_ptr[c] .. dep_ptr[c + 1]) (at most 64) besides its own;
+    // flags[16 c] is chunk c's publish epoch
+    const int *dep_ptr, *dep_idx;
+    unsigned *flags;
+    int nbr;
 };
 
 // Phase clock of block 0 (diagnostic; enabled when A.phase is set).
@@ -65,20 +71,28 @@ __device__ __forceinline__ void st_agent(float *p, float v)
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Sync words: per epoch buffer, each workgroup owns one 64-byte line of
+// SLOT_WORDS 64-bit words; an empty word holds SLOT_EMPTY, a signalling-NaN
+// pattern that no arithmetic produces (operations return quiet NaNs).
+constexpr int SLOT_WORDS = 8, SLOT_BUFS = 3;
+constexpr unsigned long long SLOT_EMPTY = 0x7ff0dead5ee00001ull;
+static_assert(SYNC_K <= SLOT_WORDS, "sync line");
+
 // Grid synchronisation and deterministic allreduce without read-modify-write
-// atomics or cache write-back.  Every workgroup writes each FP64 partial as
-// two 64-bit words, each holding the sync epoch in its high half and one half
-// of the partial in its low half; a word is stored and loaded whole, so a
-// word showing the current epoch holds the current half whatever order the
-// words arrive in.  Wave 0 of every workgroup polls all G x N x 2 words until
-// every epoch matches, then sums the partials in workgroup order (lane-strided
-// sums, then a fixed butterfly), so all workgroups obtain identical bits and
-// take identical branches.  Two word buffers alternate by epoch parity: a
-// workgroup writes buffer p at epoch e only after passing epoch e - 1, which
-// every workgroup reaches only after it has read buffer p at epoch e - 2.
+// atomics or cache write-back.  At epoch e every workgroup stores its N FP64
+// partials into its line of buffer e mod 3; wave 0 of every workgroup polls
+// the G x N words until none is empty, then sums the partials in workgroup
+// order (lane-strided sums, then a fixed butterfly), so all workgroups obtain
+// identical bits and take identical branches.  After the poll a workgroup
+// empties its own line of buffer (e + 2) mod 3: that buffer was last read at
+// epoch e - 1, which every workgroup has left once all epoch-e words are
+// there, and is next filled at epoch e + 2.  Before its epoch e + 1 stores a
+// workgroup waits for its own earlier stores (the emptying among them) to
+// complete, so a reader that has seen a workgroup's epoch e + 1 words finds
+// its epoch e + 2 line either empty or current.
 // publish: vector entries written before the sync (with st_agent) are read
 // by other workgroups after it -- each wave drains its stores before the
-// workgroup barrier that precedes the epoch words, and the readers invalidate
+// workgroup barrier that precedes the sync words, and the readers invalidate
 // their caches (agent-scope acquire) after the poll.  The wait is bounded:
 // after spin_limit polls the workgroup raises the abort word and every
 // workgroup leaves (returns false).
@@ -110,18 +124,14 @@ struct Reducer {
         if (stamp) st[0] = wall_clock64();
         if (tid < 64) {
             const unsigned G = (unsigned)A.G;
-            // each workgroup's words fill one 64-byte line of their own (stores
-            // of different workgroups never share a line)
-            unsigned long long *W = A.slots + (size_t)(ep & 1u) * G * SYNC_K * 2;
-            const unsigned long long tag = (unsigned long long)ep << 32;
-            if (lane < 2 * N) {
-                const int k = lane >> 1;
+            unsigned long long *W = A.slots + (size_t)(ep % SLOT_BUFS) * G * SLOT_WORDS;
+            __builtin_amdgcn_s_waitcnt(0);  // the emptying of this line has completed
+            if (lane < N) {
                 double s = 0.0;
-                for (int w = 0; w < TPB / 64; w++) s += s_red[w * K + k];
-                const unsigned long long bits = (unsigned long long)__double_as_longlong(s);
-                const unsigned long long half = (lane & 1) ? (bits >> 32) : (bits & 0xffffffffull);
-                __hip_atomic_store(W + ((size_t)blockIdx.x * SYNC_K + k) * 2 + (lane & 1), tag | half,
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                for (int w = 0; w < TPB / 64; w++) s += s_red[w * K + lane];
+                __hip_atomic_store(W + (size_t)blockIdx.x * SLOT_WORDS + lane,
+                                   (unsigned long long)__double_as_longlong(s), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
             }
             double acc[N];
             long long spins = 0;
@@ -133,11 +143,10 @@ struct Reducer {
                 for (unsigned g = lane; g < G; g += 64) {
 #pragma unroll
                     for (int k = 0; k < N; k++) {
-                        const unsigned long long *w = W + ((size_t)g * SYNC_K + k) * 2;
-                        const unsigned long long lo = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        const unsigned long long hi = __hip_atomic_load(w + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        ok = ok && (lo >> 32) == ep && (hi >> 32) == ep;
-                        acc[k] += __longlong_as_double((long long)((hi << 32) | (lo & 0xffffffffull)));
+                        const unsigned long long w =
+                            __hip_atomic_load(W + (size_t)g * SLOT_WORDS + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        ok = ok && w != SLOT_EMPTY;
+                        acc[k] += __longlong_as_double((long long)w);
                     }
                 }
                 if (__all(ok)) break;
@@ -155,6 +164,10 @@ struct Reducer {
                     const double s = wave_sum(acc[k]);
                     if (lane == 0) s_bc[k] = s;
                 }
+                if (lane < SLOT_WORDS)
+                    __hip_atomic_store(A.slots + (size_t)((ep + 2) % SLOT_BUFS) * G * SLOT_WORDS +
+                                           (size_t)blockIdx.x * SLOT_WORDS + lane,
+                                       SLOT_EMPTY, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 if (publish) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             }
             if (lane == 0) *s_flag = ab;
@@ -172,6 +185,51 @@ struct Reducer {
     {
         double z[1] = {0.0};
         return allreduce(A, z, true);
+    }
+
+    // Publish without a grid rendezvous: after this workgroup's vector stores
+    // have drained, its chunk's epoch word is raised, and the workgroup waits
+    // only until the chunks its SpMV gathers from show the same epoch (then
+    // invalidates its caches, as after a publishing allreduce).  No vector is
+    // overwritten before the next allreduce, which every workgroup reaches
+    // only after its SpMV, so readers never see a later value than the one
+    // published here.  Bounded like allreduce.
+    unsigned nep = 0;
+    __device__ bool neighbours(const SolveArgs &A, int chunk)
+    {
+        const int tid = threadIdx.x, lane = tid & 63;
+        __builtin_amdgcn_s_waitcnt(0);  // this wave's stores are complete
+        __syncthreads();
+        ++nep;
+        if (tid < 64) {
+            if (lane == 0)
+                __hip_atomic_store(A.flags + (size_t)chunk * 16, nep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const int d0 = A.dep_ptr[chunk], nd = A.dep_ptr[chunk + 1] - d0;
+            const unsigned *f = lane < nd ? A.flags + (size_t)A.dep_idx[d0 + lane] * 16 : nullptr;
+            long long spins = 0;
+            int ab = 0;
+            for (;;) {
+                const bool ok = !f || ld_agent(f) >= nep;
+                if (__all(ok)) break;
+                if (__builtin_amdgcn_readfirstlane(ld_agent(A.ctl))) { ab = 1; break; }
+                if (++spins > A.spin_limit) {
+                    if (lane == 0) __hip_atomic_store(A.ctl, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ab = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            if (!ab) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            if (lane == 0) *s_flag = ab;
+        }
+        __syncthreads();
+        return !*s_flag;
+    }
+
+    // The sync after publishing the vector the next SpMV gathers.
+    __device__ bool published(const SolveArgs &A, int chunk)
+    {
+        return A.nbr ? neighbours(A, chunk) : barrier(A);
     }
 };
 
@@ -194,10 +252,14 @@ __device__ __forceinline__ int opaque(int v)
     return v;
 }
 
-// CG (Chronopoulos-Gear): registers hold x and p of the thread's rows, LDS
-// holds r and s = A p.  Per iteration: [r published, (r,r) reduced] ->
-// convergence test -> w = A r with s = w + beta s and (w, r) in the same
-// pass -> reduce (w, r) -> p, x, r updates -> publish r, reduce (r, r).
+// CG (Hestenes-Stiefel with s = A p carried by recurrence): registers hold
+// x and p of the thread's rows, LDS holds r and s.  Per iteration: [r
+// published, (r, r) reduced] -> convergence test -> w = A r, s = w + beta s,
+// p = r + beta p and (p, s) in the same pass -> reduce (p, s) -> x, r updates
+// -> publish r, reduce (r, r).  The denominator (p, A p) is summed directly
+// rather than recurred (Chronopoulos-Gear's (A r, r) - beta (r, r) / alpha
+// form), which holds after a residual replacement that keeps the direction
+// and, with lossy tiles, halves the iterations on rough coefficients.
 template <int B, int IW, int TPB, int RPT>
 __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
 {
@@ -269,11 +331,9 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
     if (blockIdx.x == 0 && tid == 0) A.out[0] = sqrt(bnorm2);
     const double tol2 = A.tol * A.tol * bnorm2;
     if (bnorm2 == 0.0) { finish(MF_OK, 0, 0, 0.0, 0.0); return; }
-    double gamma = bnorm2, ref2 = bnorm2, gprev = 1.0, aprev = 1.0;
+    double gamma = bnorm2, ref2 = bnorm2, gprev = 1.0;
     bool restart = false;   // next iteration starts a new direction (beta = 0)
-    bool replaced = false;  // r was replaced before this iteration
     bool force = false;     // a breakdown asks for a true-residual pass + restart
-    double rs_repl = 0.0, ps_repl = 0.0;
     int iter = 0, repl = 0;
     PhaseClock clk(A.phase);
     for (;;) {
@@ -290,8 +350,7 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
                     st_agent(A.q64 + row_of(i), pr[i]);
                 }
             if (!R.barrier(A)) return;
-            // (r, r), (r, s), (p, s) of the replaced r and refreshed s = A p
-            double t1[3] = {0.0, 0.0, 0.0};
+            double t1[1] = {0.0};
 #pragma unroll 1
             for (int i = 0; i < RPT; i++) {
                 const int tr = tile_row(i);
@@ -311,8 +370,6 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
                     s_s[i * TPB + tid] = acc[1];
                     publish(row, rv);
                     t1[0] += rv * rv;
-                    t1[1] += rv * acc[1];
-                    t1[2] += pr[i] * acc[1];
                 }
             }
             if (!R.allreduce(A, t1, true)) return;
@@ -320,25 +377,19 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
             ++repl;
             if (!isfinite(t1[0])) { finish(MF_E_NONFINITE, iter, repl, t1[0], gamma); return; }
             if (t1[0] <= tol2) { finish(MF_OK, iter, repl, t1[0], gamma); return; }
-            // reliable update: keep the direction p.  The next step's
-            // (p', A p') with p' = r + beta p, A p' = A r + beta s is then
-            // (A r, r) + 2 beta (r, s) + beta^2 (p, s) (A symmetric), not the
-            // recurrence's form.  When the recurrence had drifted by more
-            // than a factor 2 in norm, restart from the true residual.
+            // reliable update: keep the direction p (s = A p was refreshed with
+            // it); when the recurrence had drifted by more than a factor 2 in
+            // norm, restart from the true residual
             restart = force || t1[0] > 4.0 * gamma;
             force = false;
-            replaced = true;
-            rs_repl = t1[1];
-            ps_repl = t1[2];
             gamma = ref2 = t1[0];
             continue;
         }
         if (iter >= A.maxit) { finish(MF_E_MAXIT, iter, repl, -1.0, gamma); return; }
-        const bool fresh = iter == 0 || restart, after_repl = replaced && !fresh;
-        restart = replaced = false;
+        const bool fresh = iter == 0 || restart;
+        restart = false;
         const double beta = fresh ? 0.0 : gamma / gprev;
-        // w = A r; s = w + beta s; (w, r)
-        double red[1] = {0.0};
+        // w = A r; s = w + beta s
 #pragma unroll 1
         for (int i0 = 0; i0 < RPT; i0 += NR) {
             double w[NR];
@@ -346,19 +397,21 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
 #pragma unroll
             for (int k = 0; k < NR; k++) {
                 const int i = i0 + k;
-                if (tile_row(i) < tr1) {
-                    red[0] += w[k] * s_r[i * TPB + tid];
-                    s_s[i * TPB + tid] = w[k] + beta * s_s[i * TPB + tid];
-                }
+                if (tile_row(i) < tr1) s_s[i * TPB + tid] = w[k] + beta * s_s[i * TPB + tid];
             }
         }
+        // p = r + beta p; (p, s)
+        double red[1] = {0.0};
+#pragma clang loop unroll(full)
+        for (int i = 0; i < RPT; i++)
+            if (tile_row(i) < tr1) {
+                pr[i] = s_r[i * TPB + tid] + beta * pr[i];
+                red[0] += pr[i] * s_s[i * TPB + tid];
+            }
         clk.lap(0);
         if (!R.allreduce(A, red, false)) return;
         clk.lap(1);
-        const double delta = red[0];
-        const double denom = fresh        ? delta
-                             : after_repl ? delta + beta * (2.0 * rs_repl + beta * ps_repl)
-                                          : delta - beta * gamma / aprev;
+        const double denom = red[0];
         if (!isfinite(denom)) { finish(MF_E_NONFINITE, iter, repl, -1.0, gamma); return; }
         if (!(denom > 0.0)) {
             // (p, A p) <= 0 from rounding: recompute the residual and restart;
@@ -370,10 +423,7 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
         const double alpha = gamma / denom;
 #pragma clang loop unroll(full)
         for (int i = 0; i < RPT; i++)
-            if (tile_row(i) < tr1) {
-                pr[i] = s_r[i * TPB + tid] + beta * pr[i];
-                xr[i] = xr[i] + alpha * pr[i];
-            }
+            if (tile_row(i) < tr1) xr[i] = xr[i] + alpha * pr[i];
         double g1[1] = {0.0};
 #pragma unroll 1
         for (int i = 0; i < RPT; i++)
@@ -384,7 +434,6 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
                 g1[0] += rv * rv;
             }
         gprev = gamma;
-        aprev = alpha;
         ++iter;
         clk.lap(2);
         if (!R.allreduce(A, g1, true)) return;
@@ -496,7 +545,7 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
                 publish(row_of(i), sr[i]);
             }
         clk.lap(2);
-        if (!R.barrier(A)) return;
+        if (!R.published(A, chunk)) return;
         clk.lap(3);
         spmv_into(s_t);  // t = A s
         clk.lap(0);
@@ -578,7 +627,7 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
             }
         rho = rho_new;
         clk.lap(2);
-        if (!R.barrier(A)) return;
+        if (!R.published(A, chunk)) return;
         clk.lap(3);
     }
 }

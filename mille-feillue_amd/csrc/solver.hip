@@ -121,7 +121,7 @@ static KernelChoice pick(int B, int iw, int method, long long rows_per_wg)
 
 void Workspace::release()
 {
-    void *ps[] = {g64, h64, q64, g32, slots, ctl, out, r_outer, d, tmp, red};
+    void *ps[] = {g64, h64, q64, g32, slots, ctl, out, r_outer, d, tmp, red, dep_ptr, dep_idx, flags};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     *this = Workspace();
@@ -139,7 +139,7 @@ static int ensure_ws(mf_matrix m, int G)
     MF_HIP(hipMalloc((void **)&w.h64, sizeof(double) * ncol));
     MF_HIP(hipMalloc((void **)&w.q64, sizeof(double) * ncol));
     MF_HIP(hipMalloc((void **)&w.g32, sizeof(float) * ncol));
-    MF_HIP(hipMalloc((void **)&w.slots, sizeof(unsigned long long) * 2 * SYNC_K * 2 * w.G));
+    MF_HIP(hipMalloc((void **)&w.slots, sizeof(unsigned long long) * SLOT_BUFS * SLOT_WORDS * w.G));
     MF_HIP(hipMalloc((void **)&w.ctl, sizeof(int) * 8));
     MF_HIP(hipMalloc((void **)&w.out, sizeof(double) * 4));
     MF_HIP(hipMalloc((void **)&w.r_outer, sizeof(double) * ncol));
@@ -158,17 +158,74 @@ struct InnerResult {
     int status = MF_OK, iterations = 0, replacements = 0;
     double bnorm = 0, rt2 = 0, g2 = 0;
     double phase_us[5] = {0, 0, 0, 0, 0};  // CG, block 0: per iteration x4, passes total
+    int sync_deps = -1;
 };
 
-// Zeroes the barrier counters (global + 8 groups, one 128-byte line apart)
-// and the control words with a kernel (no memset nodes).
-// Clears the sync words (epoch 0 never occurs in a launch) and the control
-// words before every persistent launch (a fill kernel, so the launch
-// sequence can be captured in a graph).
-__global__ void k_reset_ctl(unsigned long long *slots, int nslots, int *ctl)
+// Empties the sync words, clears the chunk epoch words (epoch 0 never occurs
+// in a launch) and the control words before every persistent launch (a fill
+// kernel, so the launch sequence can be captured in a graph).
+__global__ void k_reset_ctl(unsigned long long *slots, int nslots, unsigned *flags, int nflags, int *ctl)
 {
-    for (int i = threadIdx.x; i < nslots; i += blockDim.x) slots[i] = 0ull;
+    for (int i = threadIdx.x; i < nslots; i += blockDim.x) slots[i] = SLOT_EMPTY;
+    for (int i = threadIdx.x; i < nflags; i += blockDim.x) flags[i] = 0u;
     if (threadIdx.x < 8) ctl[threadIdx.x] = 0;
+}
+
+// bits[c * G + d] = 1 when a tile of chunk c (tile rows [c trw, (c + 1) trw))
+// lies in a tile column of chunk d, i.e. chunk c's SpMV gathers rows of d
+__global__ void k_chunk_deps(const int *__restrict__ tile_ptr, const int *__restrict__ tile_col, int tilem, int trw,
+                             int G, uint8_t *__restrict__ bits)
+{
+    const int tr = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tr >= tilem) return;
+    const size_t c = (size_t)(tr / trw);
+    for (int t = tile_ptr[tr]; t < tile_ptr[tr + 1]; t++) bits[c * G + min(tile_col[t] / trw, G - 1)] = 1;
+}
+
+// The neighbour lists of the grid (G chunks of trw tile rows), built once per
+// grid shape: the solvers wait only for these chunks after a publish when no
+// list is longer than one wave (64).
+static int ensure_deps(mf_matrix m, int G, int trw)
+{
+    Workspace &w = m->ws;
+    if (w.flags && w.dep_G == G && w.dep_trw == trw) return MF_OK;
+    hipStream_t st = stream();
+    for (void *p : {(void *)w.dep_ptr, (void *)w.dep_idx, (void *)w.flags})
+        if (p) MF_HIP(hipFree(p));
+    w.dep_ptr = w.dep_idx = nullptr;
+    w.flags = nullptr;
+    w.dep_G = w.dep_trw = 0;
+    w.dep_max = -1;
+    const size_t GG = (size_t)G * G;
+    uint8_t *d_bits = nullptr;
+    MF_HIP(hipMalloc((void **)&d_bits, GG));
+    MF_HIP(hipMemsetAsync(d_bits, 0, GG, st));
+    k_chunk_deps<<<(unsigned)((m->tilem + 255) / 256), 256, 0, st>>>(m->d_tile_ptr, m->d_tile_columnidx, m->tilem, trw,
+                                                                        G, d_bits);
+    MF_HIP(hipGetLastError());
+    std::vector<uint8_t> bits(GG);
+    MF_HIP(hipMemcpyAsync(bits.data(), d_bits, GG, hipMemcpyDeviceToHost, st));
+    MF_HIP(hipStreamSynchronize(st));
+    MF_HIP(hipFree(d_bits));
+    std::vector<int> ptr(G + 1, 0), idx;
+    int mx = 0;
+    for (int c = 0; c < G; c++) {
+        for (int d = 0; d < G; d++)
+            if (d != c && bits[(size_t)c * G + d]) idx.push_back(d);
+        ptr[c + 1] = (int)idx.size();
+        mx = std::max(mx, ptr[c + 1] - ptr[c]);
+    }
+    if (idx.empty()) idx.push_back(0);
+    MF_HIP(hipMalloc((void **)&w.dep_ptr, sizeof(int) * (G + 1)));
+    MF_HIP(hipMalloc((void **)&w.dep_idx, sizeof(int) * idx.size()));
+    MF_HIP(hipMalloc((void **)&w.flags, sizeof(unsigned) * 16 * (size_t)G));
+    MF_HIP(hipMemcpyAsync(w.dep_ptr, ptr.data(), sizeof(int) * (G + 1), hipMemcpyHostToDevice, st));
+    MF_HIP(hipMemcpyAsync(w.dep_idx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice, st));
+    MF_HIP(hipStreamSynchronize(st));
+    w.dep_G = G;
+    w.dep_trw = trw;
+    w.dep_max = mx;
+    return MF_OK;
 }
 
 // One persistent launch: solves L x = b (x overwritten) on the stream.
@@ -199,6 +256,8 @@ static int launch_inner(mf_matrix m, const Layout &L, const Layout *L64, int met
                     "system too large for register-resident vectors at this grid size");
     int rc = ensure_ws(m, G);
     if (rc) return rc;
+    rc = ensure_deps(m, G, trw);
+    if (rc) return rc;
     Workspace &w = m->ws;
     SolveArgs A;
     A.L = L.view(m->d_tile_ptr);
@@ -223,6 +282,13 @@ static int launch_inner(mf_matrix m, const Layout &L, const Layout *L64, int met
     A.has64 = L.has64 ? 1 : 0;
     A.haslow = L.haslow ? 1 : 0;
     A.spin_limit = o.spin_limit;
+    A.dep_ptr = w.dep_ptr;
+    A.dep_idx = w.dep_idx;
+    A.flags = w.flags;
+    // BiCGSTAB's syncs after publishing p and s carry no reduction: they can
+    // be neighbour waits (CG's publish sync also reduces (r, r))
+    A.nbr = method == MF_BICGSTAB && o.sync != MF_SYNC_GLOBAL && w.dep_max <= 64 ? 1 : 0;
+    if (res) res->sync_deps = A.nbr ? w.dep_max : -1;
     static const bool phase_profile = std::getenv("MF_PHASE_PROFILE") != nullptr;
     static unsigned long long *d_phase = nullptr;
     const size_t phase_words = 16 + 8 * 1024;
@@ -230,7 +296,7 @@ static int launch_inner(mf_matrix m, const Layout &L, const Layout *L64, int met
     if (prof && !d_phase) MF_HIP(hipMalloc((void **)&d_phase, phase_words * sizeof(unsigned long long)));
     A.phase = prof ? d_phase : nullptr;
     if (A.phase) MF_HIP(hipMemsetAsync(d_phase, 0, phase_words * sizeof(unsigned long long), st));
-    k_reset_ctl<<<1, 1024, 0, st>>>(w.slots, 2 * SYNC_K * 2 * G, w.ctl);
+    k_reset_ctl<<<1, 1024, 0, st>>>(w.slots, SLOT_BUFS * SLOT_WORDS * G, w.flags, 16 * G, w.ctl);
     MF_HIP(hipGetLastError());
     void *args[] = {&A};
     // Cooperative launch guarantees that all G workgroups are resident.  Some
@@ -369,6 +435,12 @@ static int solve_direct(mf_matrix m, const double *d_b, double *d_x, const mf_op
     const bool exact = !mixed || o.tau == 0.0;
     rc = ensure_ws(m, std::max(1, o.grid > 0 ? o.grid : device_cu_count()));
     if (rc) return rc;
+    {  // the grid's neighbour lists, outside the timed region
+        int G0 = 1, trw0 = 1;
+        solver_grid(m->tilem, m->B, o.grid, &G0, &trw0);
+        rc = ensure_deps(m, G0, trw0);
+        if (rc) return rc;
+    }
     Workspace &w = m->ws;
     const size_t n = (size_t)m->tilem * m->B;
     hipEvent_t e0, e1;
@@ -438,6 +510,7 @@ static int solve_direct(mf_matrix m, const double *d_b, double *d_x, const mf_op
     rep->block = TPB;
     rep->rows_per_thread = RPT;
     for (int k = 0; k < 5; k++) rep->phase_us[k] = ir.phase_us[k];
+    rep->sync_deps = ir.sync_deps;
     return status == MF_E_TIMEOUT ? fail(MF_E_TIMEOUT, "grid barrier timeout") : MF_OK;
 }
 

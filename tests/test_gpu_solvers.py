@@ -422,3 +422,42 @@ def test_gpu_solvers_tile_sizes(gpu, orc, B):
     assert rep["status"] == 0 and _true_relres(rp, ci, va, x, b) <= 1.5 * TOL
     assert _iters_close(rep["iterations"], orep.iterations)
     A.close()
+
+
+@pytest.mark.parametrize("case", ["convdiff27_32", "poisson3d_48"])
+def test_gpu_neighbour_syncs_match_grid_barriers(gpu, orc, case):
+    """BiCGSTAB: after publishing p or s, each workgroup waits only for the
+    chunks its tile columns reference (a 3D stencil: the chunks of the planes
+    either side); the answer and the iterations are those of grid barriers,
+    within the oracle's 5 % + 2.  CG reports grid barriers (its publish sync
+    also reduces (r, r))."""
+    rp, ci, va = gpu.gen_convdiff3d(32, 10.0, 1.0, 0) if case.startswith("convdiff") else gpu.gen_poisson3d(48, 48, 48)
+    n = len(rp) - 1
+    b = orc.csr_spmv(rp, ci, va, gpu.gen_uniform(4, n, -1, 1))
+    reps = {}
+    for mode in (gpu.MF_SYNC_AUTO, gpu.MF_SYNC_GLOBAL):
+        print(f"case {case} n={n} sync={mode}", flush=True)
+        reps[mode] = _bicg_case(gpu, orc, rp, ci, va, b, sync=mode)
+    auto, glob = reps[gpu.MF_SYNC_AUTO], reps[gpu.MF_SYNC_GLOBAL]
+    print(f"sync_deps auto={auto['sync_deps']} global={glob['sync_deps']} grid={auto['grid']}", flush=True)
+    assert 0 < auto["sync_deps"] <= 8 and glob["sync_deps"] == -1
+    assert _iters_close(auto["iterations"], glob["iterations"])
+    A = gpu.Matrix(rp, ci, va)
+    _, rep = A.solve(b, method=gpu.MF_CG, tol=TOL)
+    assert rep["sync_deps"] == -1
+    A.close()
+
+
+def test_gpu_neighbour_syncs_fall_back_to_barriers(gpu, orc):
+    """Random columns make every chunk depend on (almost) every other: more
+    than 64 dependencies fall back to grid barriers, with the same answer."""
+    rp, ci, va = gpu.gen_random_spd(200000, seed=2)
+    n = len(rp) - 1
+    b = orc.csr_spmv(rp, ci, va, gpu.gen_uniform(5, n, -1, 1))
+    A = gpu.Matrix(rp, ci, va)
+    x, rep = A.solve(b, method=gpu.MF_BICGSTAB, tol=TOL, reorder=gpu.MF_REORDER_OFF)
+    print(f"random spd n={n} grid={rep['grid']} sync_deps={rep['sync_deps']} it={rep['iterations']} "
+          f"relres={rep['relres_true']:.2e}", flush=True)
+    assert rep["status"] == 0 and rep["relres_true"] <= TOL and _true_relres(rp, ci, va, x, b) <= 1.5 * TOL
+    assert rep["grid"] > 65 and rep["sync_deps"] == -1
+    A.close()

@@ -339,6 +339,13 @@ int mf_partition_build(const mf_ref_tiles *t, int nnzR, int block_nnz, int sm_nu
                        mf_partition *out);
 void mf_free(void *p);
 
+/* Sizes of the structs of this header as compiled into the library, in the
+ * order mf_ref_tiles, mf_matrix_info, mf_opts, mf_report, mf_batch_info_t,
+ * mf_memory, mf_compact, mf_partition (at most n written); returns the number
+ * of structs.  Lets a binding check its mirrors of them (no counterpart in
+ * the reference, which has no binding). */
+int mf_abi_sizes(size_t *sizes, int n);
+
 /* Synthetic matrices of the benchmark configurations, as host CSR
  * (malloc'ed; mf_free).  Rows in lexicographic order, x fastest. */
 int mf_gen_laplace2d(int nx, int ny, int *n, int **rowptr, int **colidx, double **val);

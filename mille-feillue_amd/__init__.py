@@ -72,6 +72,13 @@ class Report(C.Structure):
         return d
 
 
+class Mem(C.Structure):
+    _fields_ = [("ref_memory_MB", C.c_double), ("ref_memory_CP_MB", C.c_double),
+                ("csr_fp64_MB", C.c_double), ("layout_fp64_MB", C.c_double),
+                ("layout_mixed_MB", C.c_double), ("index_width", C.c_int),
+                ("nnz_per_code", C.c_longlong * 5)]
+
+
 class Compact(C.Structure):
     _fields_ = [("cnt_non_new", C.c_int), ("nonzero_row_new", _i), ("blockrowid_new", _u8),
                 ("blockcsr_ptr_new", _u8), ("block_signal", _i), ("tile_rowidx", _i)]
@@ -95,8 +102,20 @@ EXPORTS = (
     "mf_solve_batched", "mf_mc_classify", "mf_memory_report", "mf_compact_rows", "mf_partition_build", "mf_free",
     "mf_gen_laplace2d", "mf_gen_poisson3d", "mf_gen_convdiff3d", "mf_gen_random_spd",
     "mf_gen_stencil_lognormal", "mf_gen_uniform", "mf_gen_normal",
-    "mf_batch_create", "mf_batch_solve", "mf_batch_info", "mf_batch_destroy", "mf_batch_tune",
+    "mf_batch_create", "mf_batch_solve", "mf_batch_info", "mf_batch_destroy", "mf_batch_tune", "mf_abi_sizes",
 )
+
+
+def abi_sizes():
+    """Sizes of the header's structs as compiled into the library, beside
+    this binding's mirrors of them: {name: (library, binding)}."""
+    names = ("mf_ref_tiles", "mf_matrix_info", "mf_opts", "mf_report", "mf_batch_info_t", "mf_memory",
+             "mf_compact", "mf_partition")
+    mirrors = (RefTiles, Info, Opts, Report, BatchInfo, Mem, Compact, Partition)
+    out = (C.c_size_t * len(names))()
+    k = lib().mf_abi_sizes(out, len(names))
+    assert k == len(names), k
+    return {n: (int(out[i]), C.sizeof(m)) for i, (n, m) in enumerate(zip(names, mirrors))}
 
 _L = None
 
@@ -394,11 +413,6 @@ class Matrix:
         return rep.as_dict()
 
     def memory_report(self, tau=0.0, allow_mask=ALLOW_ALL_LOW):
-        class Mem(C.Structure):
-            _fields_ = [("ref_memory_MB", C.c_double), ("ref_memory_CP_MB", C.c_double),
-                        ("csr_fp64_MB", C.c_double), ("layout_fp64_MB", C.c_double),
-                        ("layout_mixed_MB", C.c_double), ("index_width", C.c_int),
-                        ("nnz_per_code", C.c_longlong * 5)]
         r = Mem()
         lib().mf_memory_report.argtypes = [C.c_void_p, C.c_double, C.c_int, C.POINTER(Mem)]
         _check(lib().mf_memory_report(self.h, tau, allow_mask, C.byref(r)))

@@ -478,6 +478,15 @@ static double layout_bytes(const mf_matrix m, const mf::Layout &L)
     return (double)L.blob_bytes + 32.0 * m->tilem + 4.0 * m->tilenum + 4.0 * (m->tilem + 1);
 }
 
+int mf_abi_sizes(size_t *sizes, int n)
+{
+    const size_t all[] = {sizeof(mf_ref_tiles), sizeof(mf_matrix_info), sizeof(mf_opts), sizeof(mf_report),
+                          sizeof(mf_batch_info_t), sizeof(mf_memory), sizeof(mf_compact), sizeof(mf_partition)};
+    const int k = (int)(sizeof(all) / sizeof(all[0]));
+    for (int i = 0; i < k && i < n && sizes; i++) sizes[i] = all[i];
+    return k;
+}
+
 int mf_memory_report(mf_matrix m, double tau, int allow_mask, mf_memory *out)
 {
     clear_error();

@@ -225,3 +225,10 @@ def test_binary_csr_cache_and_errors(mf, tmp_path):
     assert mf.read_cbd(str(cache))[0] == -1
     # other extensions leave the outputs untouched
     assert mf.read_matrix_32(str(tmp_path / "m.txt")) is None
+
+
+def test_binding_structs_match_library(mf):
+    # every ctypes mirror has the size of the struct compiled into the library
+    sizes = mf.abi_sizes()
+    bad = {k: v for k, v in sizes.items() if v[0] != v[1]}
+    assert not bad, bad

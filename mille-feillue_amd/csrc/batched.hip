@@ -8,8 +8,8 @@
 // of a row interleaved ([row][K]), so one slab read and one gather index feed
 // K FMAs: the matrix is streamed once per iteration for all right-hand sides.
 // System s runs on a group of W_s workgroups (W_s sized to its work); the
-// group's reductions are block-local when W_s = 1 and otherwise use the
-// tagged-word sync of the single-system kernels restricted to the group.
+// group's reductions are block-local when W_s = 1 and otherwise use a
+// tagged-word sync restricted to the group (GroupSync).
 // Each right-hand side has its own CG scalars; a column that has converged
 // is frozen (alpha = beta = 0) while the others continue, and the system
 // finishes when an FP64 true-residual pass over the FP64 layout shows every
@@ -59,9 +59,15 @@ __device__ __forceinline__ void st_ag(float *p, float v)
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Group allreduce over the W workgroups of one system (see the file comment
-// of solver.hip for the protocol of the tagged words; a group of one only
-// reduces across its waves).
+// Group allreduce over the W workgroups of one system (a group of one only
+// reduces across its waves).  Each FP64 partial goes out as two 64-bit words
+// whose high halves carry the sync epoch and whose low halves carry one half
+// of the value each, so a word showing the current epoch holds the current
+// half whatever order the words arrive in; every workgroup of the group polls
+// the group's words until all show the epoch and sums them in workgroup order
+// (identical bits everywhere).  Two buffers alternate by epoch parity: a
+// buffer is rewritten only after every member has passed the next sync, which
+// it can do only after reading the buffer.
 template <int TPB>
 struct GroupSync {
     double *s_red;  // [TPB/64][BK_MAXV]

@@ -1,5 +1,6 @@
 // Persistent CG / BiCGSTAB kernels (one workgroup per CU, vectors on chip,
-// tagged-word grid syncs) and their launch selection, shared by the
+// self-validating grid sync words, neighbour waits where a sync carries no
+// reduction) and their launch selection, shared by the
 // translation units that instantiate them for the tile sizes 16 (solver.hip),
 // 8 (solver_b8.hip) and 32 (solver_b32.hip).
 #pragma once

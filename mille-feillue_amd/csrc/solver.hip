@@ -9,14 +9,15 @@
 // are deterministic: every workgroup sums all workgroups' partials in the same
 // fixed order (Reducer), so all take identical control decisions.
 //
-// CG uses the Chronopoulos-Gear form of Hestenes-Stiefel CG (one SpMV per
-// iteration; (Ar, r) is reduced after it and (r, r) together with the
-// publication of r: two grid syncs per iteration).  The
+// CG is Hestenes-Stiefel CG with s = A p carried by recurrence (one SpMV per
+// iteration, w = A r, s = w + beta s; (p, s) is reduced after it and (r, r)
+// together with the publication of r: two grid syncs per iteration).  The
 // reference's recurrences (PC/Mille-feuille_CG_AMD.cu:607-912: q = A d,
 // alpha = snew/(d.q), x += alpha d, r -= alpha q, beta = snew/sold,
 // d = r + beta d) are the same method in exact arithmetic.
 // BiCGSTAB follows PC/Mille-feuille_BiCGSTAB_AMD.cu:61-425 (van der Vorst),
-// with the omega reduction of :735 computed as intended.
+// with the omega reduction of :735 computed as intended; its publications of
+// p and s wait only for the workgroups whose rows the next SpMV gathers.
 //
 // Both kernels stop on the true residual: when the updated residual reaches
 // the tolerance (or has dropped by rr_factor since the last replacement) they

@@ -220,13 +220,18 @@ __device__ __forceinline__ int load_tcr(const DevL &L, const int *t0s, int q, in
 // gather is issued, then the FMAs run in each row's own entry order, so each
 // row's sum is bit-identical to slab_spmv.  FP64 values go 1 quad x NR/2 rows
 // per step (their quads are 4x wider), the other classes 2 quads x NR rows.
-template <int B, int IW, int CLS, int NR, int QS, typename AccT, typename GT>
+// With four-byte indices (unstructured columns, where rows of thousands of
+// entries occur and one lane walks each row) more quads of a row are in
+// flight per step: FP64 UQ64 when one row is (NB = 1), the other classes UQL4.
+// A 2000-entry row is then ~125 dependent load -> gather steps, not ~500
+// (random SPD, 1M rows: FP64 CG 315 -> 218 ms).
+template <int B, int IW, int CLS, int NR, int QS, int UQL4, int UQ64, typename AccT, typename GT>
 __device__ __forceinline__ void slab_spmv_rows(const DevL &L, const int4 *hd, const int *t0s, int q0,
                                                const int (&tcr)[NR], int r, int lanebase,
                                                const GT *__restrict__ g, AccT (&acc)[NR])
 {
-    constexpr int UQ = CLS == 0 ? 1 : 2;
     constexpr int NB = CLS == 0 && NR > 1 ? NR / 2 : NR;
+    constexpr int UQ = CLS == 0 ? (NB == 1 && IW == 4 ? UQ64 : 1) : (IW == 4 ? UQL4 : 2);
 #pragma unroll
     for (int kb = 0; kb < NR; kb += NB) {
         int2 h[NB];
@@ -281,7 +286,7 @@ __device__ __forceinline__ void slab_spmv_rows(const DevL &L, const int4 *hd, co
 
 // y of NR tile rows (local rows q0 + k * QS, global tile rows trq + k * QS,
 // headers in LDS), the same sums as tile_row_spmv_c.
-template <int B, int IW, int NR, int QS>
+template <int B, int IW, int NR, int QS, int UQL4 = 2, int UQ64 = 4>
 __device__ __forceinline__ void tile_rows_spmv(const DevL &L, const int4 *hd, const int *t0s, int q0, int trq,
                                                int r, int lanebase, const double *__restrict__ g64,
                                                const float *__restrict__ g32, double (&y)[NR])
@@ -296,10 +301,10 @@ __device__ __forceinline__ void tile_rows_spmv(const DevL &L, const int4 *hd, co
         a64[k] = 0.0;
         a32[k] = 0.0f;
     }
-    slab_spmv_rows<B, IW, 0, NR, QS>(L, hd, t0s, q0, tcr, r, lanebase, g64, a64);
-    slab_spmv_rows<B, IW, 1, NR, QS>(L, hd, t0s, q0, tcr, r, lanebase, g32, a32);
-    slab_spmv_rows<B, IW, 2, NR, QS>(L, hd, t0s, q0, tcr, r, lanebase, g32, a32);
-    slab_spmv_rows<B, IW, 3, NR, QS>(L, hd, t0s, q0, tcr, r, lanebase, g32, a32);
+    slab_spmv_rows<B, IW, 0, NR, QS, UQL4, UQ64>(L, hd, t0s, q0, tcr, r, lanebase, g64, a64);
+    slab_spmv_rows<B, IW, 1, NR, QS, UQL4, UQ64>(L, hd, t0s, q0, tcr, r, lanebase, g32, a32);
+    slab_spmv_rows<B, IW, 2, NR, QS, UQL4, UQ64>(L, hd, t0s, q0, tcr, r, lanebase, g32, a32);
+    slab_spmv_rows<B, IW, 3, NR, QS, UQL4, UQ64>(L, hd, t0s, q0, tcr, r, lanebase, g32, a32);
 #pragma unroll
     for (int k = 0; k < NR; k++) y[k] = a64[k] + (double)a32[k];
 }

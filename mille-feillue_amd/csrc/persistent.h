@@ -394,7 +394,11 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
 #pragma unroll 1
         for (int i0 = 0; i0 < RPT; i0 += NR) {
             double w[NR];
-            tile_rows_spmv<B, IW, NR, GPB>(A.L, s_h, s_t0, i0 * GPB + grp, tile_row(i0), lane, lanebase, A.g64, A.g32, w);
+            // four-byte indices: four quads of a row in flight up to 8 rows per
+            // thread (at 16 the registers spill; BiCGSTAB's allow four only
+            // for FP64)
+            tile_rows_spmv<B, IW, NR, GPB, RPT <= 8 ? 4 : 2, RPT <= 8 ? 4 : 1>(
+                A.L, s_h, s_t0, i0 * GPB + grp, tile_row(i0), lane, lanebase, A.g64, A.g32, w);
 #pragma unroll
             for (int k = 0; k < NR; k++) {
                 const int i = i0 + k;

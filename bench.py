@@ -13,6 +13,7 @@ CG kernel), the batched solve of 512 systems x 8 right-hand sides, and the
 oracle's tile-SpMV CG timed on the host cores.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batched]
+    (profiling aids: --only-batched [--batched-no-tune], --only-bicgstab, --calib)
 """
 import argparse
 import json
@@ -123,6 +124,52 @@ def batched_run(mf, nsys=512, nrhs=8, tune=True):
             "nnz_per_code": info["nnz_per_code"]}
 
 
+def bicgstab_run(mf, torch, steps, warmup):
+    """27-point convection-diffusion 96^3 (lognormal diffusion, cell Peclet
+    10, the library generator, seed 0), b = A x*, x* ~ U(-1, 1) seed 0;
+    BiCGSTAB to true relres 1e-10 with FP64 tiles and with FP32-rounded tiles
+    (tau = 2^-24, replacement factor 1e-2; the true-residual passes use the
+    FP64 layout).  Unit: one BiCGSTAB iteration, 2 B_alg + 112 n bytes."""
+    import scipy.sparse as sp
+    rp, ci, va = mf.gen_convdiff3d(96, 10.0, 1.0, 0)
+    n = len(rp) - 1
+    xs = mf.gen_uniform(0, n, -1.0, 1.0)
+    b = sp.csr_matrix((va, ci, rp), shape=(n, n)) @ xs
+    A = mf.Matrix(rp, ci, va)
+    tau = 2.0 ** -24
+    A.layout(tau)
+    info = A.info()
+    dev = torch.device("cuda:0")
+    npad = info["tilem"] * info["B"]
+    d_b = torch.zeros(npad, dtype=torch.float64, device=dev)
+    d_b[:n] = torch.from_numpy(b).to(dev)
+    d_x = torch.zeros(npad, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    out = {"workload": "27-point convection-diffusion 96^3, BiCGSTAB to true relres 1e-10, b = A x*",
+           "n": n, "nnz": int(len(ci)), "unit_def": "one BiCGSTAB iteration: 2 B_alg + 112 n bytes"}
+    for label, prec in (("fp64", mf.MF_FP64), ("fp32_tiles", mf.MF_MIXED)):
+        opts = dict(method=mf.MF_BICGSTAB, tol=TOL, precision=prec)
+        if prec == mf.MF_MIXED:
+            opts.update(tau=tau, rr_factor=1e-2)
+        for _ in range(warmup):
+            A.solve_device(d_b.data_ptr(), d_x.data_ptr(), **opts)
+        reps = [A.solve_device(d_b.data_ptr(), d_x.data_ptr(), **opts) for _ in range(steps)]
+        for r in reps:
+            if r["status"] != 0 or r["relres_true"] > TOL:
+                raise RuntimeError(f"BiCGSTAB did not reach tolerance: {r}")
+        ms = statistics.median(r["ms_solve"] for r in reps)
+        it = reps[0]["iterations"]
+        balg = info["bytes_alg_fp64"] if prec == mf.MF_FP64 else info["bytes_alg_mixed"]
+        per_it = 2.0 * balg + 112.0 * n
+        out[label] = {"ms": round(ms, 4), "iterations": it, "replacements": reps[0]["replacements"],
+                      "bytes_per_iteration": per_it, "GBps": round(it * per_it / (ms * 1e-3) / 1e9, 1),
+                      "frac": round(it * per_it / (ms * 1e-3) / HBM_PEAK, 4), "sync_deps": reps[0]["sync_deps"]}
+        log(f"bicgstab {label}: {ms:.3f} ms, {it} it, {out[label]['GBps']} GB/s algorithmic")
+    out["speedup_fp32_tiles_vs_fp64"] = round(out["fp64"]["ms"] / out["fp32_tiles"]["ms"], 3)
+    A.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,6 +184,8 @@ def main():
     ap.add_argument("--only-batched", action="store_true",
                     help="profiling aid: only the batched measurement; prints its record, not the bench line")
     ap.add_argument("--batched-no-tune", action="store_true", help="batched solves on the default assignment")
+    ap.add_argument("--only-bicgstab", action="store_true",
+                    help="profiling aid: only the 96^3 convection-diffusion BiCGSTAB measurement; prints its record")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -152,6 +201,13 @@ def main():
     rc = mf.lib().mf_init(0)
     if rc != 0:
         raise RuntimeError(mf.lib().mf_last_error().decode())
+
+    if args.only_bicgstab:
+        res = bicgstab_run(mf, torch, args.steps, args.warmup)
+        if args.calib:
+            calibration_copy(torch)
+        print(json.dumps(res), flush=True)
+        return
 
     if args.only_batched:
         res = batched_run(mf, tune=not args.batched_no_tune)

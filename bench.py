@@ -189,7 +189,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
-              "the multi-GPU sharding is a design in DESIGN.md, not a runnable path", file=sys.stderr)
+              "row-block sharding over several GPUs is designed but not built", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -123,6 +123,23 @@ struct Reducer {
         const bool stamp = A.phase && tid == 0 && ep - PHASE_EP0 < 4u;
         unsigned long long *st = stamp ? A.phase + 16 + (size_t)(ep - PHASE_EP0) * 2 * A.G + 2 * blockIdx.x : nullptr;
         if (stamp) st[0] = wall_clock64();
+        if (A.G == 1) {
+            // one workgroup: the reduction stays in LDS; published entries
+            // are read back through this CU's L1, which the acquire refreshes
+            if (tid < 64) {
+                if (lane < N) {
+                    double sum = 0.0;
+                    for (int w = 0; w < TPB / 64; w++) sum += s_red[w * K + lane];
+                    s_bc[lane] = sum;
+                }
+                if (publish) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            }
+            if (stamp) st[1] = wall_clock64();
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < N; k++) v[k] = s_bc[k];
+            return true;
+        }
         if (tid < 64) {
             const unsigned G = (unsigned)A.G;
             unsigned long long *W = A.slots + (size_t)(ep % SLOT_BUFS) * G * SLOT_WORDS;

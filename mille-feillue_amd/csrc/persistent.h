@@ -6,12 +6,15 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "mf_matrix.h"
 
 namespace mf {
 
 // --------------------------------------------------------- solver kernels
 constexpr int SYNC_K = 4;  // values per grid allreduce, at most
+constexpr int LDS_GATHER = 2048;  // rows of a one-workgroup solve gathered from LDS
 
 struct SolveArgs {
     DevL L;
@@ -247,7 +250,7 @@ struct Reducer {
     // The sync after publishing the vector the next SpMV gathers.
     __device__ bool published(const SolveArgs &A, int chunk)
     {
-        return A.nbr ? neighbours(A, chunk) : barrier(A);
+        return A.nbr && A.G > 1 ? neighbours(A, chunk) : barrier(A);
     }
 };
 
@@ -309,159 +312,188 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
         }
         s_t0[q] = A.L.tile_ptr[min(tr, A.tilem)];
     }
+    // one workgroup with the whole gathered vector in LDS (up to LDS_GATHER
+    // rows): the SpMV reads published entries there, with no global stores
+    // to drain and no cache round trips
+    constexpr int LG = RPT <= 4 ? LDS_GATHER : 1;
+    __shared__ double s_g64[LG];
+    __shared__ float s_g32[LG];
+    const bool lds_g = RPT <= 4 && A.G == 1 && (long long)A.tilem * B <= LDS_GATHER;
+    if (lds_g)
+        for (int i = tid; i < LG; i += TPB) {
+            s_g64[i] = 0.0;
+            s_g32[i] = 0.0f;
+        }
     __syncthreads();
     int rbase = tr0 * B + tid;  // row of step i is rbase + i * TPB
     auto tile_row = [&](int i) { return tr0 + i * GPB + grp; };
     auto row_of = [&](int i) { return (size_t)(rbase + i * TPB); };
-    auto publish = [&](size_t row, double v) {
-        if (A.has64) st_agent(A.g64 + row, v);
-        if (A.haslow) st_agent(A.g32 + row, (float)v);
-    };
-    auto finish = [&](int status, int iter, int repl, double rt2, double g2) {
-#pragma clang loop unroll(full)
-        for (int i = 0; i < RPT; i++)
-            if (tile_row(i) < tr1) A.x[row_of(i)] = xr[i];
-        if (blockIdx.x == 0 && tid == 0) {
-            A.ctl[1] = status;
-            A.ctl[2] = iter;
-            A.ctl[3] = repl;
-            A.out[1] = rt2;
-            A.out[2] = g2;
-        }
-    };
+    // the solve, on the gathered vector in LDS or in global memory (two
+    // inlined copies, each with its address space known to the compiler)
+    auto solve = [&](double *g64, float *g32, auto lds_tag) {
+        constexpr bool lds = decltype(lds_tag)::value;
+        auto publish = [&](size_t row, double v) {
+            if (lds) {
+                if (A.has64) g64[row] = v;
+                if (A.haslow) g32[row] = (float)v;
+            } else {
+                if (A.has64) st_agent(g64 + row, v);
+                if (A.haslow) st_agent(g32 + row, (float)v);
+            }
+        };
+        auto finish = [&](int status, int iter, int repl, double rt2, double g2) {
+    #pragma clang loop unroll(full)
+            for (int i = 0; i < RPT; i++)
+                if (tile_row(i) < tr1) A.x[row_of(i)] = xr[i];
+            if (blockIdx.x == 0 && tid == 0) {
+                A.ctl[1] = status;
+                A.ctl[2] = iter;
+                A.ctl[3] = repl;
+                A.out[1] = rt2;
+                A.out[2] = g2;
+            }
+        };
 
-    double v1[1] = {0.0};
-#pragma clang loop unroll(full)
-    for (int i = 0; i < RPT; i++) {
-        xr[i] = pr[i] = 0.0;
-        s_r[i * TPB + tid] = 0.0;
-        s_s[i * TPB + tid] = 0.0;
-        if (tile_row(i) < tr1) {
-            const size_t row = row_of(i);
-            const double bv = A.b[row];
-            s_r[i * TPB + tid] = bv;
-            publish(row, bv);
-            v1[0] += bv * bv;
+        double v1[1] = {0.0};
+    #pragma clang loop unroll(full)
+        for (int i = 0; i < RPT; i++) {
+            xr[i] = pr[i] = 0.0;
+            s_r[i * TPB + tid] = 0.0;
+            s_s[i * TPB + tid] = 0.0;
+            if (tile_row(i) < tr1) {
+                const size_t row = row_of(i);
+                const double bv = A.b[row];
+                s_r[i * TPB + tid] = bv;
+                publish(row, bv);
+                v1[0] += bv * bv;
+            }
         }
-    }
-    if (!R.allreduce(A, v1, true)) return;
-    const double bnorm2 = v1[0];
-    if (blockIdx.x == 0 && tid == 0) A.out[0] = sqrt(bnorm2);
-    const double tol2 = A.tol * A.tol * bnorm2;
-    if (bnorm2 == 0.0) { finish(MF_OK, 0, 0, 0.0, 0.0); return; }
-    double gamma = bnorm2, ref2 = bnorm2, gprev = 1.0;
-    bool restart = false;   // next iteration starts a new direction (beta = 0)
-    bool force = false;     // a breakdown asks for a true-residual pass + restart
-    int iter = 0, repl = 0;
-    PhaseClock clk(A.phase);
-    for (;;) {
-        rbase = opaque(rbase);
-        if (!isfinite(gamma)) { finish(MF_E_NONFINITE, iter, repl, -1.0, gamma); return; }
-        const bool conv = gamma <= tol2;
-        const bool due = force || (A.rr_factor > 0.0 && gamma <= A.rr_factor * A.rr_factor * ref2);
-        if (conv || due) {
-            // true residual b - A x and s = A p in one FP64-accumulated pass
-#pragma clang loop unroll(full)
+        if (!R.allreduce(A, v1, true)) return;
+        const double bnorm2 = v1[0];
+        if (blockIdx.x == 0 && tid == 0) A.out[0] = sqrt(bnorm2);
+        const double tol2 = A.tol * A.tol * bnorm2;
+        if (bnorm2 == 0.0) { finish(MF_OK, 0, 0, 0.0, 0.0); return; }
+        double gamma = bnorm2, ref2 = bnorm2, gprev = 1.0;
+        bool restart = false;   // next iteration starts a new direction (beta = 0)
+        bool force = false;     // a breakdown asks for a true-residual pass + restart
+        int iter = 0, repl = 0;
+        PhaseClock clk(A.phase);
+        for (;;) {
+            rbase = opaque(rbase);
+            if (!isfinite(gamma)) { finish(MF_E_NONFINITE, iter, repl, -1.0, gamma); return; }
+            const bool conv = gamma <= tol2;
+            const bool due = force || (A.rr_factor > 0.0 && gamma <= A.rr_factor * A.rr_factor * ref2);
+            if (conv || due) {
+                // true residual b - A x and s = A p in one FP64-accumulated pass
+    #pragma clang loop unroll(full)
+                for (int i = 0; i < RPT; i++)
+                    if (tile_row(i) < tr1) {
+                        st_agent(A.h64 + row_of(i), xr[i]);
+                        st_agent(A.q64 + row_of(i), pr[i]);
+                    }
+                if (!R.barrier(A)) return;
+                double t1[1] = {0.0};
+    #pragma unroll 1
+                for (int i = 0; i < RPT; i++) {
+                    const int tr = tile_row(i);
+                    if (tr < tr1) {
+                        const double *g[2] = {A.h64, A.q64};
+                        double acc[2] = {0.0, 0.0};
+                        const int q = i * GPB + grp;
+                        if (A.use64)
+                            tile_row_spmv_f64_c<B, IW, 2>(A.L64, row_ctx<B, IW>(A.L64, tr, lane), lane, lanebase, g, acc);
+                        else
+                            tile_row_spmv_f64_c<B, IW, 2>(
+                                A.L, row_ctx_from<B, IW>(A.L, tr, s_t0[q], s_t0[q + 1] - s_t0[q], s_h[2 * q], s_h[2 * q + 1], lane),
+                                lane, lanebase, g, acc);
+                        const size_t row = (size_t)tr * B + lane;
+                        const double rv = A.b[row] - acc[0];
+                        s_r[i * TPB + tid] = rv;
+                        s_s[i * TPB + tid] = acc[1];
+                        publish(row, rv);
+                        t1[0] += rv * rv;
+                    }
+                }
+                if (!R.allreduce(A, t1, true)) return;
+                clk.lap(4);
+                ++repl;
+                if (!isfinite(t1[0])) { finish(MF_E_NONFINITE, iter, repl, t1[0], gamma); return; }
+                if (t1[0] <= tol2) { finish(MF_OK, iter, repl, t1[0], gamma); return; }
+                // reliable update: keep the direction p (s = A p was refreshed with
+                // it); when the recurrence had drifted by more than a factor 2 in
+                // norm, restart from the true residual
+                restart = force || t1[0] > 4.0 * gamma;
+                force = false;
+                gamma = ref2 = t1[0];
+                continue;
+            }
+            if (iter >= A.maxit) { finish(MF_E_MAXIT, iter, repl, -1.0, gamma); return; }
+            const bool fresh = iter == 0 || restart;
+            restart = false;
+            const double beta = fresh ? 0.0 : gamma / gprev;
+            // w = A r; s = w + beta s
+    #pragma unroll 1
+            for (int i0 = 0; i0 < RPT; i0 += NR) {
+                double w[NR];
+                // four-byte indices: four quads of a row in flight up to 8 rows per
+                // thread (at 16 the registers spill; BiCGSTAB's allow four only
+                // for FP64)
+                tile_rows_spmv<B, IW, NR, GPB, RPT <= 8 ? 4 : 2, RPT <= 8 ? 4 : 1>(
+                    A.L, s_h, s_t0, i0 * GPB + grp, tile_row(i0), lane, lanebase, g64, g32, w);
+    #pragma unroll
+                for (int k = 0; k < NR; k++) {
+                    const int i = i0 + k;
+                    if (tile_row(i) < tr1) s_s[i * TPB + tid] = w[k] + beta * s_s[i * TPB + tid];
+                }
+            }
+            // p = r + beta p; (p, s)
+            double red[1] = {0.0};
+    #pragma clang loop unroll(full)
             for (int i = 0; i < RPT; i++)
                 if (tile_row(i) < tr1) {
-                    st_agent(A.h64 + row_of(i), xr[i]);
-                    st_agent(A.q64 + row_of(i), pr[i]);
+                    pr[i] = s_r[i * TPB + tid] + beta * pr[i];
+                    red[0] += pr[i] * s_s[i * TPB + tid];
                 }
-            if (!R.barrier(A)) return;
-            double t1[1] = {0.0};
-#pragma unroll 1
-            for (int i = 0; i < RPT; i++) {
-                const int tr = tile_row(i);
-                if (tr < tr1) {
-                    const double *g[2] = {A.h64, A.q64};
-                    double acc[2] = {0.0, 0.0};
-                    const int q = i * GPB + grp;
-                    if (A.use64)
-                        tile_row_spmv_f64_c<B, IW, 2>(A.L64, row_ctx<B, IW>(A.L64, tr, lane), lane, lanebase, g, acc);
-                    else
-                        tile_row_spmv_f64_c<B, IW, 2>(
-                            A.L, row_ctx_from<B, IW>(A.L, tr, s_t0[q], s_t0[q + 1] - s_t0[q], s_h[2 * q], s_h[2 * q + 1], lane),
-                            lane, lanebase, g, acc);
-                    const size_t row = (size_t)tr * B + lane;
-                    const double rv = A.b[row] - acc[0];
+            clk.lap(0);
+            if (!R.allreduce(A, red, false)) return;
+            clk.lap(1);
+            const double denom = red[0];
+            if (!isfinite(denom)) { finish(MF_E_NONFINITE, iter, repl, -1.0, gamma); return; }
+            if (!(denom > 0.0)) {
+                // (p, A p) <= 0 from rounding: recompute the residual and restart;
+                // a breakdown right after a restart is final
+                if (fresh) { finish(MF_E_BREAKDOWN, iter, repl, -1.0, gamma); return; }
+                force = true;
+                continue;
+            }
+            const double alpha = gamma / denom;
+    #pragma clang loop unroll(full)
+            for (int i = 0; i < RPT; i++)
+                if (tile_row(i) < tr1) xr[i] = xr[i] + alpha * pr[i];
+            double g1[1] = {0.0};
+    #pragma unroll 1
+            for (int i = 0; i < RPT; i++)
+                if (tile_row(i) < tr1) {
+                    const double rv = s_r[i * TPB + tid] - alpha * s_s[i * TPB + tid];
                     s_r[i * TPB + tid] = rv;
-                    s_s[i * TPB + tid] = acc[1];
-                    publish(row, rv);
-                    t1[0] += rv * rv;
+                    publish(row_of(i), rv);
+                    g1[0] += rv * rv;
                 }
-            }
-            if (!R.allreduce(A, t1, true)) return;
-            clk.lap(4);
-            ++repl;
-            if (!isfinite(t1[0])) { finish(MF_E_NONFINITE, iter, repl, t1[0], gamma); return; }
-            if (t1[0] <= tol2) { finish(MF_OK, iter, repl, t1[0], gamma); return; }
-            // reliable update: keep the direction p (s = A p was refreshed with
-            // it); when the recurrence had drifted by more than a factor 2 in
-            // norm, restart from the true residual
-            restart = force || t1[0] > 4.0 * gamma;
-            force = false;
-            gamma = ref2 = t1[0];
-            continue;
+            gprev = gamma;
+            ++iter;
+            clk.lap(2);
+            if (!R.allreduce(A, g1, true)) return;
+            clk.lap(3);
+            gamma = g1[0];
         }
-        if (iter >= A.maxit) { finish(MF_E_MAXIT, iter, repl, -1.0, gamma); return; }
-        const bool fresh = iter == 0 || restart;
-        restart = false;
-        const double beta = fresh ? 0.0 : gamma / gprev;
-        // w = A r; s = w + beta s
-#pragma unroll 1
-        for (int i0 = 0; i0 < RPT; i0 += NR) {
-            double w[NR];
-            // four-byte indices: four quads of a row in flight up to 8 rows per
-            // thread (at 16 the registers spill; BiCGSTAB's allow four only
-            // for FP64)
-            tile_rows_spmv<B, IW, NR, GPB, RPT <= 8 ? 4 : 2, RPT <= 8 ? 4 : 1>(
-                A.L, s_h, s_t0, i0 * GPB + grp, tile_row(i0), lane, lanebase, A.g64, A.g32, w);
-#pragma unroll
-            for (int k = 0; k < NR; k++) {
-                const int i = i0 + k;
-                if (tile_row(i) < tr1) s_s[i * TPB + tid] = w[k] + beta * s_s[i * TPB + tid];
-            }
+    };
+    if constexpr (RPT <= 4) {
+        if (lds_g) {
+            solve(s_g64, s_g32, std::true_type{});
+            return;
         }
-        // p = r + beta p; (p, s)
-        double red[1] = {0.0};
-#pragma clang loop unroll(full)
-        for (int i = 0; i < RPT; i++)
-            if (tile_row(i) < tr1) {
-                pr[i] = s_r[i * TPB + tid] + beta * pr[i];
-                red[0] += pr[i] * s_s[i * TPB + tid];
-            }
-        clk.lap(0);
-        if (!R.allreduce(A, red, false)) return;
-        clk.lap(1);
-        const double denom = red[0];
-        if (!isfinite(denom)) { finish(MF_E_NONFINITE, iter, repl, -1.0, gamma); return; }
-        if (!(denom > 0.0)) {
-            // (p, A p) <= 0 from rounding: recompute the residual and restart;
-            // a breakdown right after a restart is final
-            if (fresh) { finish(MF_E_BREAKDOWN, iter, repl, -1.0, gamma); return; }
-            force = true;
-            continue;
-        }
-        const double alpha = gamma / denom;
-#pragma clang loop unroll(full)
-        for (int i = 0; i < RPT; i++)
-            if (tile_row(i) < tr1) xr[i] = xr[i] + alpha * pr[i];
-        double g1[1] = {0.0};
-#pragma unroll 1
-        for (int i = 0; i < RPT; i++)
-            if (tile_row(i) < tr1) {
-                const double rv = s_r[i * TPB + tid] - alpha * s_s[i * TPB + tid];
-                s_r[i * TPB + tid] = rv;
-                publish(row_of(i), rv);
-                g1[0] += rv * rv;
-            }
-        gprev = gamma;
-        ++iter;
-        clk.lap(2);
-        if (!R.allreduce(A, g1, true)) return;
-        clk.lap(3);
-        gamma = g1[0];
     }
+    solve(A.g64, A.g32, std::false_type{});
 }
 
 // BiCGSTAB: registers hold x, r, r^, p, s; LDS holds v = A p and t = A s.
@@ -496,162 +528,191 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
         }
         s_t0[q] = A.L.tile_ptr[min(tr, A.tilem)];
     }
+    // one workgroup with the whole gathered vector in LDS (up to LDS_GATHER
+    // rows): the SpMV reads published entries there, with no global stores
+    // to drain and no cache round trips
+    constexpr int LG = RPT <= 4 ? LDS_GATHER : 1;
+    __shared__ double s_g64[LG];
+    __shared__ float s_g32[LG];
+    const bool lds_g = RPT <= 4 && A.G == 1 && (long long)A.tilem * B <= LDS_GATHER;
+    if (lds_g)
+        for (int i = tid; i < LG; i += TPB) {
+            s_g64[i] = 0.0;
+            s_g32[i] = 0.0f;
+        }
     __syncthreads();
     int rbase = tr0 * B + tid;  // row of step i is rbase + i * TPB
     auto tile_row = [&](int i) { return tr0 + i * GPB + grp; };
     auto row_of = [&](int i) { return (size_t)(rbase + i * TPB); };
-    auto publish = [&](size_t row, double v) {
-        if (A.has64) st_agent(A.g64 + row, v);
-        if (A.haslow) st_agent(A.g32 + row, (float)v);
-    };
-    auto finish = [&](int status, int iter, int repl, double rt2, double g2) {
-#pragma clang loop unroll(full)
-        for (int i = 0; i < RPT; i++)
-            if (tile_row(i) < tr1) A.x[row_of(i)] = xr[i];
-        if (blockIdx.x == 0 && tid == 0) {
-            A.ctl[1] = status;
-            A.ctl[2] = iter;
-            A.ctl[3] = repl;
-            A.out[1] = rt2;
-            A.out[2] = g2;
-        }
-    };
-    auto spmv_into = [&](double *dst) {
-#pragma unroll 1
-        for (int i0 = 0; i0 < RPT; i0 += NR) {
-            double w[NR];
-            tile_rows_spmv<B, IW, NR, GPB>(A.L, s_h, s_t0, i0 * GPB + grp, tile_row(i0), lane, lanebase, A.g64, A.g32, w);
-#pragma unroll
-            for (int k = 0; k < NR; k++)
-                if (tile_row(i0 + k) < tr1) dst[(i0 + k) * TPB + tid] = w[k];
-        }
-    };
-
-    double v1[1] = {0.0};
-#pragma clang loop unroll(full)
-    for (int i = 0; i < RPT; i++) {
-        xr[i] = rr[i] = rh[i] = pr[i] = sr[i] = 0.0;
-        s_v[i * TPB + tid] = 0.0;
-        if (tile_row(i) < tr1) {
-            const size_t row = row_of(i);
-            rr[i] = rh[i] = pr[i] = A.b[row];
-            publish(row, pr[i]);
-            v1[0] += rr[i] * rr[i];
-        }
-    }
-    if (!R.allreduce(A, v1, true)) return;
-    const double bnorm2 = v1[0];
-    if (blockIdx.x == 0 && tid == 0) A.out[0] = sqrt(bnorm2);
-    const double tol2 = A.tol * A.tol * bnorm2;
-    if (bnorm2 == 0.0) { finish(MF_OK, 0, 0, 0.0, 0.0); return; }
-    double rho = bnorm2, ref2 = bnorm2;
-    bool restart = false;
-    int iter = 0, repl = 0;
-    PhaseClock clk(A.phase);
-    for (;;) {
-        rbase = opaque(rbase);
-        spmv_into(s_v);  // v = A p
-        clk.lap(0);
-        double a1[1] = {0.0};
-#pragma clang loop unroll(full)
-        for (int i = 0; i < RPT; i++)
-            if (tile_row(i) < tr1) a1[0] += rh[i] * s_v[i * TPB + tid];
-        if (!R.allreduce(A, a1, false)) return;
-        clk.lap(1);
-        if (!(a1[0] != 0.0) || !isfinite(a1[0])) { finish(MF_E_BREAKDOWN, iter, repl, -1.0, -1.0); return; }
-        const double alpha = rho / a1[0];
-#pragma clang loop unroll(full)
-        for (int i = 0; i < RPT; i++)
-            if (tile_row(i) < tr1) {
-                sr[i] = rr[i] - alpha * s_v[i * TPB + tid];
-                publish(row_of(i), sr[i]);
+    // the solve, on the gathered vector in LDS or in global memory (two
+    // inlined copies, each with its address space known to the compiler)
+    auto solve = [&](double *g64, float *g32, auto lds_tag) {
+        constexpr bool lds = decltype(lds_tag)::value;
+        auto publish = [&](size_t row, double v) {
+            if (lds) {
+                if (A.has64) g64[row] = v;
+                if (A.haslow) g32[row] = (float)v;
+            } else {
+                if (A.has64) st_agent(g64 + row, v);
+                if (A.haslow) st_agent(g32 + row, (float)v);
             }
-        clk.lap(2);
-        if (!R.published(A, chunk)) return;
-        clk.lap(3);
-        spmv_into(s_t);  // t = A s
-        clk.lap(0);
-        double a2[2] = {0.0, 0.0};
-#pragma clang loop unroll(full)
-        for (int i = 0; i < RPT; i++)
-            if (tile_row(i) < tr1) {
-                const double t = s_t[i * TPB + tid];
-                a2[0] += t * sr[i];
-                a2[1] += t * t;
-            }
-        if (!R.allreduce(A, a2, false)) return;
-        clk.lap(1);
-        const double omega = a2[1] > 0.0 ? a2[0] / a2[1] : 0.0;
-        double a3[2] = {0.0, 0.0};
-#pragma clang loop unroll(full)
-        for (int i = 0; i < RPT; i++)
-            if (tile_row(i) < tr1) {
-                xr[i] = xr[i] + alpha * pr[i] + omega * sr[i];
-                rr[i] = sr[i] - omega * s_t[i * TPB + tid];
-                a3[0] += rr[i] * rh[i];
-                a3[1] += rr[i] * rr[i];
-            }
-        clk.lap(2);
-        if (!R.allreduce(A, a3, false)) return;
-        clk.lap(3);
-        double rho_new = a3[0], g2 = a3[1];
-        ++iter;
-        if (!finite_all(rho_new, g2)) { finish(MF_E_NONFINITE, iter, repl, -1.0, g2); return; }
-        const bool conv = g2 <= tol2;
-        const bool due = A.rr_factor > 0.0 && g2 <= A.rr_factor * A.rr_factor * ref2;
-        if (conv || due) {
-#pragma clang loop unroll(full)
+        };
+        auto finish = [&](int status, int iter, int repl, double rt2, double g2) {
+    #pragma clang loop unroll(full)
             for (int i = 0; i < RPT; i++)
-                if (tile_row(i) < tr1) st_agent(A.h64 + row_of(i), xr[i]);
-            if (!R.barrier(A)) return;
-#pragma unroll 1
-            for (int i = 0; i < RPT; i++) {
-                const int tr = tile_row(i);
-                if (tr < tr1) {
-                    const double *g[1] = {A.h64};
-                    double acc[1] = {0.0};
-                    const int q = i * GPB + grp;
-                    if (A.use64)
-                        tile_row_spmv_f64_c<B, IW, 1>(A.L64, row_ctx<B, IW>(A.L64, tr, lane), lane, lanebase, g, acc);
-                    else
-                        tile_row_spmv_f64_c<B, IW, 1>(
-                            A.L, row_ctx_from<B, IW>(A.L, tr, s_t0[q], s_t0[q + 1] - s_t0[q], s_h[2 * q], s_h[2 * q + 1], lane),
-                            lane, lanebase, g, acc);
-                    s_t[i * TPB + tid] = A.b[(size_t)tr * B + lane] - acc[0];
-                }
+                if (tile_row(i) < tr1) A.x[row_of(i)] = xr[i];
+            if (blockIdx.x == 0 && tid == 0) {
+                A.ctl[1] = status;
+                A.ctl[2] = iter;
+                A.ctl[3] = repl;
+                A.out[1] = rt2;
+                A.out[2] = g2;
             }
-            double t2[2] = {0.0, 0.0};
-#pragma clang loop unroll(full)
+        };
+        auto spmv_into = [&](double *dst) {
+    #pragma unroll 1
+            for (int i0 = 0; i0 < RPT; i0 += NR) {
+                double w[NR];
+                tile_rows_spmv<B, IW, NR, GPB>(A.L, s_h, s_t0, i0 * GPB + grp, tile_row(i0), lane, lanebase, g64, g32, w);
+    #pragma unroll
+                for (int k = 0; k < NR; k++)
+                    if (tile_row(i0 + k) < tr1) dst[(i0 + k) * TPB + tid] = w[k];
+            }
+        };
+
+        double v1[1] = {0.0};
+    #pragma clang loop unroll(full)
+        for (int i = 0; i < RPT; i++) {
+            xr[i] = rr[i] = rh[i] = pr[i] = sr[i] = 0.0;
+            s_v[i * TPB + tid] = 0.0;
+            if (tile_row(i) < tr1) {
+                const size_t row = row_of(i);
+                rr[i] = rh[i] = pr[i] = A.b[row];
+                publish(row, pr[i]);
+                v1[0] += rr[i] * rr[i];
+            }
+        }
+        if (!R.allreduce(A, v1, true)) return;
+        const double bnorm2 = v1[0];
+        if (blockIdx.x == 0 && tid == 0) A.out[0] = sqrt(bnorm2);
+        const double tol2 = A.tol * A.tol * bnorm2;
+        if (bnorm2 == 0.0) { finish(MF_OK, 0, 0, 0.0, 0.0); return; }
+        double rho = bnorm2, ref2 = bnorm2;
+        bool restart = false;
+        int iter = 0, repl = 0;
+        PhaseClock clk(A.phase);
+        for (;;) {
+            rbase = opaque(rbase);
+            spmv_into(s_v);  // v = A p
+            clk.lap(0);
+            double a1[1] = {0.0};
+    #pragma clang loop unroll(full)
+            for (int i = 0; i < RPT; i++)
+                if (tile_row(i) < tr1) a1[0] += rh[i] * s_v[i * TPB + tid];
+            if (!R.allreduce(A, a1, false)) return;
+            clk.lap(1);
+            if (!(a1[0] != 0.0) || !isfinite(a1[0])) { finish(MF_E_BREAKDOWN, iter, repl, -1.0, -1.0); return; }
+            const double alpha = rho / a1[0];
+    #pragma clang loop unroll(full)
             for (int i = 0; i < RPT; i++)
                 if (tile_row(i) < tr1) {
-                    rr[i] = s_t[i * TPB + tid];
-                    t2[0] += rr[i] * rr[i];
-                    t2[1] += rr[i] * rh[i];
+                    sr[i] = rr[i] - alpha * s_v[i * TPB + tid];
+                    publish(row_of(i), sr[i]);
                 }
-            if (!R.allreduce(A, t2, false)) return;
-            clk.lap(4);
-            ++repl;
-            if (!isfinite(t2[0])) { finish(MF_E_NONFINITE, iter, repl, t2[0], g2); return; }
-            if (t2[0] <= tol2) { finish(MF_OK, iter, repl, t2[0], g2); return; }
-            restart = t2[0] > 4.0 * g2;  // recurrence drifted: new direction p = r
-            ref2 = t2[0];
-            rho_new = t2[1];
-        }
-        if (iter >= A.maxit) { finish(MF_E_MAXIT, iter, repl, -1.0, g2); return; }
-        if (omega == 0.0 || rho == 0.0) { finish(MF_E_BREAKDOWN, iter, repl, -1.0, g2); return; }
-        const double beta = restart ? 0.0 : (rho_new / rho) * (alpha / omega);
-        restart = false;
-#pragma clang loop unroll(full)
-        for (int i = 0; i < RPT; i++)
-            if (tile_row(i) < tr1) {
-                pr[i] = rr[i] + beta * (pr[i] - omega * s_v[i * TPB + tid]);
-                publish(row_of(i), pr[i]);
+            clk.lap(2);
+            if (!R.published(A, chunk)) return;
+            clk.lap(3);
+            spmv_into(s_t);  // t = A s
+            clk.lap(0);
+            double a2[2] = {0.0, 0.0};
+    #pragma clang loop unroll(full)
+            for (int i = 0; i < RPT; i++)
+                if (tile_row(i) < tr1) {
+                    const double t = s_t[i * TPB + tid];
+                    a2[0] += t * sr[i];
+                    a2[1] += t * t;
+                }
+            if (!R.allreduce(A, a2, false)) return;
+            clk.lap(1);
+            const double omega = a2[1] > 0.0 ? a2[0] / a2[1] : 0.0;
+            double a3[2] = {0.0, 0.0};
+    #pragma clang loop unroll(full)
+            for (int i = 0; i < RPT; i++)
+                if (tile_row(i) < tr1) {
+                    xr[i] = xr[i] + alpha * pr[i] + omega * sr[i];
+                    rr[i] = sr[i] - omega * s_t[i * TPB + tid];
+                    a3[0] += rr[i] * rh[i];
+                    a3[1] += rr[i] * rr[i];
+                }
+            clk.lap(2);
+            if (!R.allreduce(A, a3, false)) return;
+            clk.lap(3);
+            double rho_new = a3[0], g2 = a3[1];
+            ++iter;
+            if (!finite_all(rho_new, g2)) { finish(MF_E_NONFINITE, iter, repl, -1.0, g2); return; }
+            const bool conv = g2 <= tol2;
+            const bool due = A.rr_factor > 0.0 && g2 <= A.rr_factor * A.rr_factor * ref2;
+            if (conv || due) {
+    #pragma clang loop unroll(full)
+                for (int i = 0; i < RPT; i++)
+                    if (tile_row(i) < tr1) st_agent(A.h64 + row_of(i), xr[i]);
+                if (!R.barrier(A)) return;
+    #pragma unroll 1
+                for (int i = 0; i < RPT; i++) {
+                    const int tr = tile_row(i);
+                    if (tr < tr1) {
+                        const double *g[1] = {A.h64};
+                        double acc[1] = {0.0};
+                        const int q = i * GPB + grp;
+                        if (A.use64)
+                            tile_row_spmv_f64_c<B, IW, 1>(A.L64, row_ctx<B, IW>(A.L64, tr, lane), lane, lanebase, g, acc);
+                        else
+                            tile_row_spmv_f64_c<B, IW, 1>(
+                                A.L, row_ctx_from<B, IW>(A.L, tr, s_t0[q], s_t0[q + 1] - s_t0[q], s_h[2 * q], s_h[2 * q + 1], lane),
+                                lane, lanebase, g, acc);
+                        s_t[i * TPB + tid] = A.b[(size_t)tr * B + lane] - acc[0];
+                    }
+                }
+                double t2[2] = {0.0, 0.0};
+    #pragma clang loop unroll(full)
+                for (int i = 0; i < RPT; i++)
+                    if (tile_row(i) < tr1) {
+                        rr[i] = s_t[i * TPB + tid];
+                        t2[0] += rr[i] * rr[i];
+                        t2[1] += rr[i] * rh[i];
+                    }
+                if (!R.allreduce(A, t2, false)) return;
+                clk.lap(4);
+                ++repl;
+                if (!isfinite(t2[0])) { finish(MF_E_NONFINITE, iter, repl, t2[0], g2); return; }
+                if (t2[0] <= tol2) { finish(MF_OK, iter, repl, t2[0], g2); return; }
+                restart = t2[0] > 4.0 * g2;  // recurrence drifted: new direction p = r
+                ref2 = t2[0];
+                rho_new = t2[1];
             }
-        rho = rho_new;
-        clk.lap(2);
-        if (!R.published(A, chunk)) return;
-        clk.lap(3);
+            if (iter >= A.maxit) { finish(MF_E_MAXIT, iter, repl, -1.0, g2); return; }
+            if (omega == 0.0 || rho == 0.0) { finish(MF_E_BREAKDOWN, iter, repl, -1.0, g2); return; }
+            const double beta = restart ? 0.0 : (rho_new / rho) * (alpha / omega);
+            restart = false;
+    #pragma clang loop unroll(full)
+            for (int i = 0; i < RPT; i++)
+                if (tile_row(i) < tr1) {
+                    pr[i] = rr[i] + beta * (pr[i] - omega * s_v[i * TPB + tid]);
+                    publish(row_of(i), pr[i]);
+                }
+            rho = rho_new;
+            clk.lap(2);
+            if (!R.published(A, chunk)) return;
+            clk.lap(3);
+        }
+    };
+    if constexpr (RPT <= 4) {
+        if (lds_g) {
+            solve(s_g64, s_g32, std::true_type{});
+            return;
+        }
     }
+    solve(A.g64, A.g32, std::false_type{});
 }
 
 // ------------------------------------------------------------ dispatch

@@ -461,3 +461,33 @@ def test_gpu_neighbour_syncs_fall_back_to_barriers(gpu, orc):
     assert rep["status"] == 0 and rep["relres_true"] <= TOL and _true_relres(rp, ci, va, x, b) <= 1.5 * TOL
     assert rep["grid"] > 65 and rep["sync_deps"] == -1
     A.close()
+
+
+@pytest.mark.parametrize("method", ["cg", "bicgstab"])
+def test_gpu_one_workgroup_paths(gpu, orc, method):
+    """A one-workgroup solve keeps reductions and the gathered vector in LDS
+    (up to 2048 rows); two workgroups go through global memory.  Both reach
+    1e-10 with the oracle's iteration count (5 % + 2), and a one-workgroup
+    solve above 2048 rows (gathered vector in global memory) as well."""
+    if method == "cg":
+        g = load_golden("1138_bus")
+        rp, ci, va = g["rowptr"], g["col"], g["val"]
+        case = _cg_case
+    else:
+        rp, ci, va = gpu.gen_convdiff3d(10, 10.0, 1.0, 0)
+        case = _bicg_case
+    n = len(rp) - 1
+    b = orc.csr_spmv(rp, ci, va, gpu.gen_uniform(6, n, -1, 1))
+    reps = {}
+    for grid in (1, 2):
+        print(f"case {method} n={n} grid={grid}", flush=True)
+        reps[grid] = case(gpu, orc, rp, ci, va, b, grid=grid)
+        assert reps[grid]["grid"] == grid
+    assert _iters_close(reps[1]["iterations"], reps[2]["iterations"])
+    rp, ci, va = gpu.gen_poisson3d(16, 16, 12) if method == "cg" else gpu.gen_convdiff3d(14, 10.0, 1.0, 0)
+    n = len(rp) - 1
+    assert n > 2048
+    b = orc.csr_spmv(rp, ci, va, gpu.gen_uniform(7, n, -1, 1))
+    print(f"case {method} n={n} grid=1 (global gather)", flush=True)
+    rep = case(gpu, orc, rp, ci, va, b, grid=1)
+    assert rep["grid"] == 1

@@ -9,11 +9,14 @@ x resident in HBM (device buffers owned by torch, passed to mf_solve_device).
 Reported: time to tolerance of the per-tile mixed-precision solve (the value),
 the same persistent kernel run FP64-only, their ratio, the standalone tiled
 SpMV kernel's bandwidth, the roofline of the dominant kernel (the persistent
-CG kernel), the batched solve of 512 systems x 8 right-hand sides, and the
-oracle's tile-SpMV CG timed on the host cores.
+CG kernel: one SpMV at its algorithmic bytes per unit), the batched solve of
+512 systems x 8 right-hand sides, the oracle's tile-SpMV CG and the restated
+PETSc CG timed on the host cores, and the same records for the other single
+system workloads: the 256^2 Laplacian CG, the 96^3 convection-diffusion
+BiCGSTAB and the random SPD (n = 1M) CG.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batched]
-    (profiling aids: --only-batched [--batched-no-tune], --only-bicgstab, --calib)
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batched] [--no-configs]
+    (profiling aids: --only-batched [--batched-no-tune], --only-bicgstab, --only-config NAME, --calib)
 """
 import argparse
 import json
@@ -36,16 +39,41 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def host_cpu():
+    """The host the CPU baselines run on: logical CPUs, the CPUs this process
+    may run on, the model name, OMP_NUM_THREADS, and the thread count used
+    (OMP_NUM_THREADS when set -- a GPU box sets it to its CPU share -- else
+    every CPU of the affinity mask)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    aff = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    threads = int(omp) if omp and omp.isdigit() and int(omp) > 0 else aff
+    return {"nproc": os.cpu_count(), "affinity_cpus": aff, "model": model, "OMP_NUM_THREADS": omp,
+            "threads": threads}
+
+
+def _oracle():
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from orc import Oracle
+    return Oracle()
+
+
 def cpu_baseline(rp, ci, va, b, gpu_iters, gpu_x=None):
     """The oracle's tile-SpMV CG (FP64, OpenMP over the host cores) run to the
     same tolerance on the same system; gpu_x maps a label to a GPU solution,
     whose relative error against the oracle's solution is reported."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from orc import Oracle
-
-    O = Oracle()
+    O = _oracle()
     n = len(b)
-    cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16), 64)
+    host = host_cpu()
+    cores = host["threads"]
     T = O.tiles(16, n, n, rp, ci, va)
     t0 = time.perf_counter()
     x, rep = O.cg(rp, ci, va, b, TOL, 20000, tiles=T, nthreads=cores)
@@ -63,6 +91,7 @@ def cpu_baseline(rp, ci, va, b, gpu_iters, gpu_x=None):
         "iterations": rep.iterations,
         "petsc_cg": {"ms": round(dp, 3), "iterations": prep.iterations, "relres_true": prep.relres_true,
                      "kind": "port", "cores": cores},
+        "host": host,
         "gpu_err_vs_oracle_x": {k: float(np.linalg.norm(v - x) / np.linalg.norm(x))
                                 for k, v in (gpu_x or {}).items()},
     }
@@ -124,20 +153,63 @@ def batched_run(mf, nsys=512, nrhs=8, tune=True):
             "nnz_per_code": info["nnz_per_code"]}
 
 
-def bicgstab_run(mf, torch, steps, warmup):
-    """27-point convection-diffusion 96^3 (lognormal diffusion, cell Peclet
-    10, the library generator, seed 0), b = A x*, x* ~ U(-1, 1) seed 0;
-    BiCGSTAB to true relres 1e-10 with FP64 tiles and with FP32-rounded tiles
-    (tau = 2^-24, replacement factor 1e-2; the true-residual passes use the
-    FP64 layout).  Unit: one BiCGSTAB iteration, 2 B_alg + 112 n bytes."""
+def _roofline(iters, spmvs_per_iter, bytes_alg, ms):
+    """One SpMV at its algorithmic bytes (B_alg = min(tile, CSR) bytes: values
+    and indices, tile metadata, the gathered vector read once, y written once)
+    is the unit; a solve runs iterations x SpMVs per iteration of them."""
+    ach = iters * spmvs_per_iter * bytes_alg / (ms * 1e-3)
+    return {"bound": "hbm", "achieved": round(ach / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK, 4), "bytes_per_unit": bytes_alg,
+            "units_per_launch": iters * spmvs_per_iter}
+
+
+def _configs(mf):
+    """The single-system workloads besides the headline, as generators:
+    (label, workload text, method, builder -> (rp, ci, va, x*), variants)."""
+    def laplace():
+        rp, ci, va = mf.gen_laplace2d(256, 256)
+        return rp, ci, va, np.ones(len(rp) - 1)
+
+    def convdiff():
+        rp, ci, va = mf.gen_convdiff3d(96, 10.0, 1.0, 0)
+        return rp, ci, va, mf.gen_uniform(0, len(rp) - 1, -1.0, 1.0)
+
+    def random_spd():
+        rp, ci, va = mf.gen_random_spd(1000000, 2.0, 2, 2000, 0)
+        return rp, ci, va, mf.gen_uniform(0, len(rp) - 1, -1.0, 1.0)
+
+    return {
+        "laplace256_cg": ("2D 5-point Laplacian 256^2, CG to true relres 1e-10, b = A 1", mf.MF_CG, laplace,
+                          {"fp64": dict(precision=mf.MF_FP64),
+                           "mixed": dict(precision=mf.MF_MIXED, tau=0.0, rr_factor=1e-2)}),
+        "convdiff96_bicgstab": ("27-point convection-diffusion 96^3 (cell Peclet 10, lognormal(1) diffusion), "
+                                "BiCGSTAB to true relres 1e-10, b = A x*, x* ~ U(-1,1) seed 0", mf.MF_BICGSTAB,
+                                convdiff,
+                                {"fp64": dict(precision=mf.MF_FP64),
+                                 "fp32_tiles": dict(precision=mf.MF_MIXED, tau=2.0 ** -24, rr_factor=1e-2),
+                                 "e4m3_tiles": dict(precision=mf.MF_MIXED, tau=2.0 ** -4, rr_factor=1e-2)}),
+        "random_spd_1m_cg": ("random SPD n = 1,000,000 (Zipf(2) row lengths in [2, 2000], B + B^T, diagonal "
+                             "1.05 x row abs-sum), CG to true relres 1e-10, b = A x*, x* ~ U(-1,1) seed 0", mf.MF_CG,
+                             random_spd,
+                             {"fp64": dict(precision=mf.MF_FP64),
+                              "mixed": dict(precision=mf.MF_MIXED, tau=2.0 ** -24, rr_factor=1e-2)}),
+    }
+
+
+def config_run(mf, torch, name, steps, warmup, cpu=True):
+    """One of the other single-system workloads: every variant of the
+    persistent solver (median time to tolerance over the steps, b and x in
+    HBM), its roofline on one SpMV at B_alg, the vendor-library baseline, and
+    the oracle's tile solver and the restated PETSc solver on the host cores
+    (to tolerance, or capped for the random SPD system, whose CSR solve runs
+    seconds per hundred iterations on the host)."""
     import scipy.sparse as sp
-    rp, ci, va = mf.gen_convdiff3d(96, 10.0, 1.0, 0)
+    text, method, build, variants = _configs(mf)[name]
+    t0 = time.perf_counter()
+    rp, ci, va, xs = build()
     n = len(rp) - 1
-    xs = mf.gen_uniform(0, n, -1.0, 1.0)
     b = sp.csr_matrix((va, ci, rp), shape=(n, n)) @ xs
     A = mf.Matrix(rp, ci, va)
-    tau = 2.0 ** -24
-    A.layout(tau)
     info = A.info()
     dev = torch.device("cuda:0")
     npad = info["tilem"] * info["B"]
@@ -145,28 +217,75 @@ def bicgstab_run(mf, torch, steps, warmup):
     d_b[:n] = torch.from_numpy(b).to(dev)
     d_x = torch.zeros(npad, dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
-    out = {"workload": "27-point convection-diffusion 96^3, BiCGSTAB to true relres 1e-10, b = A x*",
-           "n": n, "nnz": int(len(ci)), "unit_def": "one BiCGSTAB iteration: 2 B_alg + 112 n bytes"}
-    for label, prec in (("fp64", mf.MF_FP64), ("fp32_tiles", mf.MF_MIXED)):
-        opts = dict(method=mf.MF_BICGSTAB, tol=TOL, precision=prec)
-        if prec == mf.MF_MIXED:
-            opts.update(tau=tau, rr_factor=1e-2)
+    log(f"{name}: n={n} nnz={len(ci)} set up in {time.perf_counter() - t0:.1f}s")
+    spmvs = 2 if method == mf.MF_BICGSTAB else 1
+    out = {"workload": text, "n": n, "nnz": int(len(ci)), "method": "bicgstab" if spmvs == 2 else "cg",
+           "unit_def": "one SpMV at B_alg = min(tile, CSR) algorithmic bytes", "spmvs_per_iteration": spmvs}
+    k = steps if n * 20 < 30e6 else max(2, min(steps, 3))  # the random SPD solve takes ~0.2 s
+    xsol = {}
+    for label, opts in variants.items():
+        if opts.get("precision") == mf.MF_MIXED:
+            A.layout(opts["tau"])
+        balg = A.info()["bytes_alg_mixed" if opts.get("precision") == mf.MF_MIXED else "bytes_alg_fp64"]
+        o = dict(method=method, tol=TOL, **opts)
         for _ in range(warmup):
-            A.solve_device(d_b.data_ptr(), d_x.data_ptr(), **opts)
-        reps = [A.solve_device(d_b.data_ptr(), d_x.data_ptr(), **opts) for _ in range(steps)]
-        for r in reps:
-            if r["status"] != 0 or r["relres_true"] > TOL:
-                raise RuntimeError(f"BiCGSTAB did not reach tolerance: {r}")
+            A.solve_device(d_b.data_ptr(), d_x.data_ptr(), **o)
+        reps = [A.solve_device(d_b.data_ptr(), d_x.data_ptr(), **o) for _ in range(k)]
+        ok = all(r["status"] == 0 and r["relres_true"] <= TOL for r in reps)
         ms = statistics.median(r["ms_solve"] for r in reps)
         it = reps[0]["iterations"]
-        balg = info["bytes_alg_fp64"] if prec == mf.MF_FP64 else info["bytes_alg_mixed"]
-        per_it = 2.0 * balg + 112.0 * n
-        out[label] = {"ms": round(ms, 4), "iterations": it, "replacements": reps[0]["replacements"],
-                      "bytes_per_iteration": per_it, "GBps": round(it * per_it / (ms * 1e-3) / 1e9, 1),
-                      "frac": round(it * per_it / (ms * 1e-3) / HBM_PEAK, 4), "sync_deps": reps[0]["sync_deps"]}
-        log(f"bicgstab {label}: {ms:.3f} ms, {it} it, {out[label]['GBps']} GB/s algorithmic")
-    out["speedup_fp32_tiles_vs_fp64"] = round(out["fp64"]["ms"] / out["fp32_tiles"]["ms"], 3)
+        x = d_x[:n].cpu().numpy()
+        xsol[label] = x
+        rec = {"ms": round(ms, 4), "iterations": it, "replacements": reps[0]["replacements"],
+               "relres_true": reps[0]["relres_true"], "converged": ok,
+               "err_vs_xstar": float(np.linalg.norm(x - xs) / np.linalg.norm(xs)),
+               "grid": reps[0]["grid"], "reordered": reps[0]["reordered"], "sync_deps": reps[0]["sync_deps"],
+               "nnz_per_code": A.info()["nnz_per_code"] if opts.get("precision") == mf.MF_MIXED else None,
+               "roofline": _roofline(it, spmvs, balg, ms)}
+        out[label] = rec
+        log(f"{name} {label}: {ms:.3f} ms, {it} it, converged {ok}, roofline {rec['roofline']['frac']}")
+    for label in variants:
+        if label != "fp64" and out[label]["converged"]:
+            out[f"speedup_{label}_vs_fp64"] = round(out["fp64"]["ms"] / out[label]["ms"], 3)
+    v = A.vendor_solve_device(d_b.data_ptr(), d_x.data_ptr(), method=method, tol=TOL)
+    out["vendor_baseline"] = {"ms": round(v["ms_solve"], 3), "iterations": v["iterations"],
+                              "relres_true": v["relres_true"],
+                              "kind": "rocSPARSE CSR SpMV (adaptive) + rocBLAS dots, host loop"}
+    log(f"{name} vendor: {v['ms_solve']:.3f} ms, {v['iterations']} it")
     A.close()
+    del d_b, d_x
+    if cpu:
+        O = _oracle()
+        host = host_cpu()
+        th = host["threads"]
+        capped = n >= 1000000
+        cap = 50 if capped else 20000
+        T = O.tiles(16, n, n, rp, ci, va)
+        solve = O.bicgstab if spmvs == 2 else O.cg
+        psolve = O.petsc_bcgs if spmvs == 2 else O.petsc_cg
+        t0 = time.perf_counter()
+        xo, rep = solve(rp, ci, va, b, TOL, cap, tiles=T, nthreads=th)
+        dt = (time.perf_counter() - t0) * 1e3
+        del T
+        t0 = time.perf_counter()
+        _, prep = psolve(rp, ci, va, b, TOL, cap, nthreads=th)
+        dp = (time.perf_counter() - t0) * 1e3
+        out["cpu_baseline"] = {
+            "tile_solver": {"ms": round(dt, 3), "iterations": rep.iterations, "converged": bool(rep.converged),
+                            "ms_per_iteration": round(dt / max(rep.iterations, 1), 4),
+                            "relres_true": rep.relres_true, "kind": "port"},
+            "petsc": {"ms": round(dp, 3), "iterations": prep.iterations, "converged": bool(prep.converged),
+                      "ms_per_iteration": round(dp / max(prep.iterations, 1), 4),
+                      "relres_true": prep.relres_true, "kind": "port (PETSc algorithm restated)"},
+            "threads": th, "host": host,
+            "sample": (f"capped at {cap} iterations (time per iteration reported)" if capped
+                       else "full solve to true relres 1e-10"),
+        }
+        if not capped and rep.converged:
+            out["cpu_baseline"]["gpu_err_vs_oracle_x"] = {
+                k2: float(np.linalg.norm(v2 - xo) / np.linalg.norm(xo)) for k2, v2 in xsol.items()}
+        log(f"{name} cpu: tile {dt:.1f} ms ({rep.iterations} it), petsc {dp:.1f} ms ({prep.iterations} it), "
+            f"{th} threads")
     return out
 
 
@@ -185,8 +304,16 @@ def main():
                     help="profiling aid: only the batched measurement; prints its record, not the bench line")
     ap.add_argument("--batched-no-tune", action="store_true", help="batched solves on the default assignment")
     ap.add_argument("--only-bicgstab", action="store_true",
-                    help="profiling aid: only the 96^3 convection-diffusion BiCGSTAB measurement; prints its record")
+                    help="profiling aid: only the 96^3 convection-diffusion BiCGSTAB record (--only-config "
+                         "convdiff96_bicgstab)")
+    ap.add_argument("--only-config", default=None,
+                    help="profiling aid: only this workload's record (laplace256_cg, convdiff96_bicgstab, "
+                         "random_spd_1m_cg); prints it, not the bench line")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the records of the other single-system workloads")
     args = ap.parse_args()
+    if args.only_bicgstab:
+        args.only_config = "convdiff96_bicgstab"
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
               "row-block sharding over several GPUs is designed but not built", file=sys.stderr)
@@ -202,8 +329,8 @@ def main():
     if rc != 0:
         raise RuntimeError(mf.lib().mf_last_error().decode())
 
-    if args.only_bicgstab:
-        res = bicgstab_run(mf, torch, args.steps, args.warmup)
+    if args.only_config:
+        res = config_run(mf, torch, args.only_config, args.steps, args.warmup, cpu=not args.no_cpu)
         if args.calib:
             calibration_copy(torch)
         print(json.dumps(res), flush=True)
@@ -304,12 +431,12 @@ def main():
         f"fp64 {vendor['speedup_fp64']}x faster)")
     log(f"spmv phase in the solver: fp64 {phases['fp64']['spmv_frac']:.3f}, mixed {phases['mixed']['spmv_frac']:.3f} of peak")
 
-    # dominant kernel: the persistent CG launch.  Unit = one CG iteration,
-    # algorithmic bytes B_alg + 56 n (the SpMV plus x, r, d read and written
-    # and q re-read by a kernel that streams its vectors); true-residual
-    # passes are not counted.
-    iter_bytes = bmx + 56.0 * n
-    ach = itmx * iter_bytes / (msmx * 1e-3)
+    # dominant kernel: the persistent CG launch (one per solve).  Unit = one
+    # SpMV at its algorithmic bytes B_alg (values and indices at their stored
+    # widths, tile metadata, the gathered vector read once, y written once);
+    # the solve runs one SpMV per iteration.  The other Krylov vectors stay in
+    # registers and LDS for the whole solve and are not counted, nor are the
+    # true-residual passes.
     traffic = None
     pmc_file = os.path.join(ROOT, "profiles", "k_cg_poisson128_pmc.json")
     if g == 128 and os.path.exists(pmc_file):
@@ -318,20 +445,28 @@ def main():
         # calibrated FETCH_SIZE + WRITE_SIZE of the mixed launch, per iteration
         traffic = round(pmc["mixed"]["traffic_bytes_per_iteration"] * itmx)
     iw = info["index_width"]
-    roof = {"bound": "hbm", "achieved": round(ach / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK, 4), "traffic": traffic,
-            "kernel": f"k_cg<16,{iw},512,16> (persistent mixed CG, {iw}-byte slab indices, one launch per solve)",
-            "working_set_bytes": round(info["bytes_layout_mixed"] + 40.0 * n),
-            "note": "the working set (slabs, gather and solver vectors) fits in the 256 MB Infinity Cache, "
-                    "so re-reads of it are served there and the algorithmic rate can exceed the HBM peak; "
-                    "traffic is what L2 fetched and wrote per launch",
-            "bytes_per_unit": iter_bytes, "unit_def": "one CG iteration: B_alg + 56 n bytes",
-            "units_per_launch": itmx, "launch_ms": round(msmx, 4),
-            "spmv_bytes_only_GBps": round(itmx * bmx / (msmx * 1e-3) / 1e9, 1),
-            "traffic_source": "profiles/k_cg_poisson128_pmc.json (rocprofv3 FETCH_SIZE x2.000 + WRITE_SIZE x1.000)"
-            if traffic is not None else None}
+    roof = _roofline(itmx, 1, bmx, msmx)
+    roof.update({
+        "traffic": traffic,
+        "kernel": f"k_cg<16,{iw},512,16> (persistent mixed CG, {iw}-byte slab indices, one launch per solve)",
+        "unit_def": "one SpMV at B_alg (SpMV algorithmic bytes of the mixed layout); one per CG iteration",
+        "launch_ms": round(msmx, 4),
+        "per_step_GBps": round(itmx * bmx / (wallmx * 1e-3) / 1e9, 1),
+        "fp64": _roofline(it64, 1, b64, ms64),
+        "traffic_source": "profiles/k_cg_poisson128_pmc.json (rocprofv3 FETCH_SIZE, WRITE_SIZE, calibrated "
+                          "against a device copy of known size)" if traffic is not None else None,
+    })
+    if roof["achieved"] > HBM_PEAK / 1e9 or roof["per_step_GBps"] > HBM_PEAK / 1e9:
+        raise RuntimeError(f"algorithmic rate above the HBM peak: {roof}")
 
+    A.close()
+    del d_b, d_x
     batched = None if args.no_batched else batched_run(mf)
+
+    configs = None
+    if not args.no_configs:
+        configs = {name: config_run(mf, torch, name, args.steps, args.warmup, cpu=not args.no_cpu)
+                   for name in _configs(mf)}
 
     cpu = None
     if not args.no_cpu:
@@ -374,10 +509,10 @@ def main():
         },
         "spmv_phase_in_solver": phases,
         "vendor_baseline": vendor,
-        "solver_GBps": {"fp64": round(it64 * (b64 + 56.0 * n) / ms64 / 1e6, 1), "mixed": round(ach / 1e9, 1)},
         "roofline": roof,
         "batched": batched,
         "cpu_baseline": cpu,
+        "configs": configs,
     }
     print(json.dumps(out), flush=True)
 

@@ -52,6 +52,9 @@ int mmio_info(int *m, int *n, int *nnz, int *isSymmetric, char *filename);
 /* PC/mmio_highlevel.h:275 -- CSR in file order (no column sort, duplicates
  * kept), symmetric/hermitian off-diagonals mirrored. */
 int mmio_data(int *csrRowPtr, int *csrColIdx, double *csrVal, char *filename);
+/* PC/mmio_highlevel.h:594 -- mmio_data with float values (each value
+ * converted from its double parse, round to nearest). */
+int mmio_data_Low(int *csrRowPtr, int *csrColIdx, float *csrVal, char *filename);
 /* Binary CSR cache, PC/biio2.0/src/biio.h (format: int rows, int cols, int64
  * nnz, int64 row_ptr[rows+1], int col_idx[nnz], double val[nnz]).
  * binary_read_csr (:22) returns 0 or -1 (missing or short file) and mallocs
@@ -87,6 +90,18 @@ void Tile_create(Tile_matrix *matrix, int rowA, int colA, int nnzA, int *csrRowP
                  int *csrColIdxA, double *csrValA, float *csrValA_Low);
 /* PC/format.h:88 -- frees what Tile_create allocated. */
 void Tile_destroy(Tile_matrix *matrix);
+/* PC/blockspmv_cpu.h:2 -- the reference's host set-up routine over a
+ * Tile_create'd matrix (16-row tiles): ptroffset1/ptroffset2[tilenum]
+ * (nonzeros / row slots before each tile), the row-block split of tile rows
+ * with more than 4 tiles (*rowblkblock entries in three malloc'ed arrays,
+ * bit 31 marking split tile rows, free with free/mf_free), and y[tilem*16] =
+ * the tile product A x summed per tile.  Columns are decoded from
+ * Tile_csr_Col (the reference reads the never-filled csr_compressedIdx).
+ * The CSR arguments and y_golden are unused, as in the reference. */
+void blockspmv_cpu(Tile_matrix *matrix, int *ptroffset1, int *ptroffset2, int *rowblkblock,
+                   unsigned int **blkcoostylerowidx, int **blkcoostylerowidx_colstart,
+                   int **blkcoostylerowidx_colstop, int rowA, int colA, int nnzA, int *csrRowPtrA,
+                   int *csrColIdxA, double *csrValA, double *x, double *y, double *y_golden);
 /* PC/Mille-feuille_CG_AMD.cu:1240 -- FP64 CG.  Rows are truncated to a
  * multiple of 16 as in the reference (:1246-1248).  Solves to true relative
  * residual MF_DROPIN_TOL (1e-10) with at most max(maxiter, 10000) iterations
@@ -127,6 +142,13 @@ typedef struct mf_matrix_s *mf_matrix;
 
 /* Library set-up on a device (also done lazily by every entry point). */
 int mf_init(int device);
+/* Releases every matrix and batch still held, the library's device buffers
+ * and its stream (e.g. at process exit, before the runtime and any profiler
+ * attached to it shut down).  Released handles may still be passed to the
+ * destroy functions, which then do nothing; the next call sets the device up
+ * again.  No reference counterpart (the reference frees per solve and leaks
+ * some device buffers). */
+void mf_finalize(void);
 
 /* Uploads a CSR matrix and builds the reference tile layout on the GPU
  * (PC/csr2block.h:267 semantics, bit-exact).  B in {8, 16, 32}.  val_low may
@@ -341,8 +363,8 @@ void mf_free(void *p);
 
 /* Sizes of the structs of this header as compiled into the library, in the
  * order mf_ref_tiles, mf_matrix_info, mf_opts, mf_report, mf_batch_info_t,
- * mf_memory, mf_compact, mf_partition (at most n written); returns the number
- * of structs.  Lets a binding check its mirrors of them (no counterpart in
+ * mf_memory, mf_compact, mf_partition, Tile_matrix (at most n written);
+ * returns the number of structs.  Lets a binding check its mirrors of them (no counterpart in
  * the reference, which has no binding). */
 int mf_abi_sizes(size_t *sizes, int n);
 

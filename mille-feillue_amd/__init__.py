@@ -5,6 +5,7 @@ gfx950 and C++ host code.  This module only moves numpy arrays across that
 boundary for the tests and the benchmark.  It raises if the library is
 missing or if a call returns an error -- there is no fallback path.
 """
+import atexit
 import ctypes as C
 import os
 
@@ -91,12 +92,38 @@ class Partition(C.Structure):
                 ("non_each_block_offset", _i), ("balance_tile_ptr_shared_end", _i)]
 
 
+class TileMatrix(C.Structure):
+    """The reference's Tile_matrix record (include/mf_tile_matrix.h, field
+    order of PC/format.h:14-86), for Tile_create/Tile_destroy/blockspmv_cpu."""
+    _p = C.c_void_p
+    _fields_ = [
+        ("tilem", C.c_int), ("tilen", C.c_int), ("tilenum", C.c_int), ("tile_ptr", _i), ("tile_columnidx", _i),
+        ("tile_nnz", _i), ("Format", C.POINTER(C.c_char)), ("blknnz", _i), ("blknnznnz", _u8),
+        ("dnsrowptr", _p), ("dnscolptr", _p), ("tilewidth", C.POINTER(C.c_char)), ("csr_offset", _i),
+        ("csrptr_offset", _i), ("coo_offset", _p), ("ell_offset", _p), ("hyb_offset", _p), ("hyb_coocount", _p),
+        ("dns_offset", _p), ("dnsrow_offset", _p), ("dnscol_offset", _p), ("new_coocount", _p),
+        ("Blockcsr_Val", _d), ("Blockcsr_Val_Low", _f), ("Tile_csr_Col", _u8), ("Blockcsr_Val_mix", _p),
+        ("Blockcsr_Ptr", _u8), ("csr_compressedIdx", _u8), ("csrsize", C.c_int), ("csrptrlen", C.c_int),
+        ("Blockcoo_Val", _p), ("Blockcoo_Val_Low", _p), ("coo_compressed_Idx", _p), ("coosize", C.c_int),
+        ("Blockell_Val", _p), ("Blockell_Val_Low", _p), ("ell_compressedIdx", _p), ("ellsize", C.c_int),
+        ("Blockhyb_Val", _p), ("Blockhyb_Val_Low", _p), ("hybIdx", _p), ("hybsize", C.c_int),
+        ("hybellsize", C.c_int), ("hybcoosize", C.c_int), ("Blockdense_Val", _p), ("Blockdense_Val_Low", _p),
+        ("dnssize", C.c_int), ("Blockdenserow_Val", _p), ("Blockdenserow_Val_Low", _p), ("denserowid", _p),
+        ("dnsrowsize", C.c_int), ("Blockdensecol_Val", _p), ("Blockdensecol_Val_Low", _p), ("densecolid", _p),
+        ("dnscolsize", C.c_int), ("coototal", C.c_int), ("deferredcoo_val", _p), ("deferredcoo_val_Low", _p),
+        ("deferredcoo_colidx", _p), ("deferredcoo_ptr", _p), ("rowblkblock", C.c_int),
+        ("flag_bal_tile_rowidx", _p), ("tile_bal_rowidx_colstart", _p), ("tile_bal_rowidx_colstop", _p),
+        ("flag_tilerow_start", _p), ("flag_tilerow_stop", _p), ("tile_bal_rowidx_colstart_v2", _p),
+        ("tile_bal_rowidx_colstop_v2", _p), ("map", _p),
+    ]
+
+
 # symbols declared in include/mf.h (checked by the CPU tests)
 EXPORTS = (
-    "mmio_info", "mmio_data", "encode", "decode", "exclusive_scan", "exclusive_scan_char",
+    "mmio_info", "mmio_data", "mmio_data_Low", "blockspmv_cpu", "encode", "decode", "exclusive_scan", "exclusive_scan_char",
     "binary_read_csr", "binary_write_csr", "read_Dmatrix", "read_Dmatrix_32", "read_Dmatrix_convert", "mf_write_mtx",
     "Tile_create", "Tile_destroy", "cg_solve_inc", "bicgstab", "mf_status_str", "mf_last_error",
-    "mf_init", "mf_matrix_create", "mf_matrix_destroy", "mf_matrix_export", "mf_ref_tiles_free",
+    "mf_init", "mf_finalize", "mf_matrix_create", "mf_matrix_destroy", "mf_matrix_export", "mf_ref_tiles_free",
     "mf_matrix_encode_cols", "mf_matrix_classes", "mf_matrix_get_info", "mf_opts_default",
     "mf_solve", "mf_solve_device", "mf_vendor_solve", "mf_vendor_solve_device", "mf_matrix_layout", "mf_matrix_set_index_width", "mf_spmv", "mf_spmv_bench",
     "mf_solve_batched", "mf_mc_classify", "mf_memory_report", "mf_compact_rows", "mf_partition_build", "mf_free",
@@ -110,8 +137,8 @@ def abi_sizes():
     """Sizes of the header's structs as compiled into the library, beside
     this binding's mirrors of them: {name: (library, binding)}."""
     names = ("mf_ref_tiles", "mf_matrix_info", "mf_opts", "mf_report", "mf_batch_info_t", "mf_memory",
-             "mf_compact", "mf_partition")
-    mirrors = (RefTiles, Info, Opts, Report, BatchInfo, Mem, Compact, Partition)
+             "mf_compact", "mf_partition", "Tile_matrix")
+    mirrors = (RefTiles, Info, Opts, Report, BatchInfo, Mem, Compact, Partition, TileMatrix)
     out = (C.c_size_t * len(names))()
     k = lib().mf_abi_sizes(out, len(names))
     assert k == len(names), k
@@ -174,7 +201,13 @@ def lib():
     L.mf_batch_info.argtypes = [C.c_void_p, C.POINTER(BatchInfo)]
     L.mf_batch_destroy.argtypes = [C.c_void_p]
     L.mf_batch_tune.argtypes = [C.c_void_p, C.POINTER(_d), C.POINTER(Report)]
+    L.mf_finalize.argtypes = []
+    L.mf_finalize.restype = None
     _L = L
+    # release device state at interpreter exit, before the runtime (and a
+    # profiler attached to it) shuts down; objects collected later find
+    # their handles released and do nothing
+    atexit.register(L.mf_finalize)
     return L
 
 
@@ -619,3 +652,66 @@ def decode(packed, n):
     res = np.zeros(n, np.uint8)
     lib().decode(_p(packed, C.c_ubyte), _p(res, C.c_ubyte), n)
     return res
+
+
+def tile_create(rowA, colA, rowptr, col, val, val_low=None):
+    """Tile_create (PC/csr2block.h:267) into a TileMatrix (16-row tiles); the
+    caller releases it with tile_destroy."""
+    L = lib()
+    rp = np.ascontiguousarray(rowptr, np.int32)
+    ci = np.ascontiguousarray(col, np.int32)
+    va = np.ascontiguousarray(val, np.float64)
+    low = None if val_low is None else np.ascontiguousarray(val_low, np.float32)
+    t = TileMatrix()
+    L.Tile_create.argtypes = [C.POINTER(TileMatrix), C.c_int, C.c_int, C.c_int, _i, _i, _d, _f]
+    L.Tile_create(C.byref(t), rowA, colA, int(rp[rowA]), _p(rp, C.c_int), _p(ci, C.c_int), _p(va, C.c_double),
+                  _p(low, C.c_float) if low is not None else None)
+    return t
+
+
+def tile_destroy(t):
+    L = lib()
+    L.Tile_destroy.argtypes = [C.POINTER(TileMatrix)]
+    L.Tile_destroy(C.byref(t))
+
+
+def blockspmv_cpu(t, rowA, colA, x):
+    """blockspmv_cpu (PC/blockspmv_cpu.h:2) over a TileMatrix: dict of the
+    per-tile offsets, the row-block split and y (tilem * 16 entries)."""
+    L = lib()
+    n1 = max(t.tilenum, 1)
+    p1 = np.zeros(n1, np.int32)
+    p2 = np.zeros(n1, np.int32)
+    rbb = C.c_int()
+    br, cs, ce = C.POINTER(C.c_uint)(), _i(), _i()
+    xx = np.zeros(max(t.tilen * 16, colA), np.float64)
+    xx[: len(x)] = x
+    y = np.zeros(t.tilem * 16, np.float64)
+    yg = np.zeros(max(rowA, 1), np.float64)
+    L.blockspmv_cpu.argtypes = [C.POINTER(TileMatrix), _i, _i, C.POINTER(C.c_int), C.POINTER(C.POINTER(C.c_uint)),
+                                C.POINTER(_i), C.POINTER(_i), C.c_int, C.c_int, C.c_int, _i, _i, _d, _d, _d, _d]
+    L.blockspmv_cpu(C.byref(t), _p(p1, C.c_int), _p(p2, C.c_int), C.byref(rbb), C.byref(br), C.byref(cs),
+                    C.byref(ce), rowA, colA, 0, None, None, None, _p(xx, C.c_double), _p(y, C.c_double),
+                    _p(yg, C.c_double))
+    k = rbb.value
+    out = dict(rowblkblock=k, ptroffset1=p1[: t.tilenum], ptroffset2=p2[: t.tilenum],
+               blkrow=_copy(br, k, np.uint32), colstart=_copy(cs, k, np.int32), colstop=_copy(ce, k, np.int32),
+               y=y, y_golden=yg)
+    for q in (br, cs, ce):
+        L.mf_free(C.cast(q, C.c_void_p))
+    return out
+
+
+def mmio_low(path):
+    """mmio_data_Low (PC/mmio_highlevel.h:594): (rc, dict(rowptr, col, val float32))."""
+    L = lib()
+    m, n, nnz, sym = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    rc = L.mmio_info(C.byref(m), C.byref(n), C.byref(nnz), C.byref(sym), path.encode())
+    if rc:
+        return rc, None
+    rp = np.zeros(m.value + 1, np.int32)
+    ci = np.zeros(max(nnz.value, 1), np.int32)
+    va = np.zeros(max(nnz.value, 1), np.float32)
+    L.mmio_data_Low.argtypes = [_i, _i, _f, C.c_char_p]
+    rc = L.mmio_data_Low(_p(rp, C.c_int), _p(ci, C.c_int), _p(va, C.c_float), path.encode())
+    return rc, dict(rowptr=rp, col=ci[: nnz.value], val=va[: nnz.value])

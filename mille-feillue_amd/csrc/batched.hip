@@ -1063,13 +1063,14 @@ int mf_batch_create(int nsys, const int *n, const int *const *rowptr, const int 
         return rc;
     }
     bt->ms_setup = now_ms() - t0;
+    live_add(bt, 1);
     *out = bt;
     return MF_OK;
 }
 
 void mf_batch_destroy(mf_batch bt)
 {
-    if (!bt) return;
+    if (!bt || !live_remove(bt, 1)) return;
     batch_release(bt);
     delete bt;
 }

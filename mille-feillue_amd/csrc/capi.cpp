@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -29,6 +30,24 @@ static int g_device = -1, g_cus = 0;
 static hipStream_t g_stream = nullptr;
 
 hipStream_t stream() { return g_stream; }
+
+static std::mutex g_live_mu;
+static std::set<void *> g_live[2];
+void live_add(void *h, int kind)
+{
+    std::lock_guard<std::mutex> lk(g_live_mu);
+    g_live[kind].insert(h);
+}
+bool live_remove(void *h, int kind)
+{
+    std::lock_guard<std::mutex> lk(g_live_mu);
+    return g_live[kind].erase(h) > 0;
+}
+bool live_has(void *h, int kind)
+{
+    std::lock_guard<std::mutex> lk(g_live_mu);
+    return g_live[kind].count(h) > 0;
+}
 int device_cu_count() { return g_cus; }
 
 static int init_device(int dev)
@@ -172,15 +191,41 @@ int mf_matrix_create(int rowA, int colA, int nnz, const int *rowptr, const int *
     rc = build_reference_tiles(m);
     if (rc) return bail(rc);
     m->ms_build = now_ms() - t0;
+    live_add(m, 0);
     *out = m;
     return MF_OK;
 }
 
 void mf_matrix_destroy(mf_matrix m)
 {
-    if (!m) return;
+    if (!m || !live_remove(m, 0)) return;
     free_dev(m);
     delete m;
+}
+
+// Releases every matrix and batch still held, the library's device buffers
+// and its stream, before the process tears the runtime down (bindings call it
+// at interpreter exit).  Handles released here may still be passed to the
+// destroy functions, which then do nothing.  The next entry point sets the
+// device up again.
+void mf_finalize(void)
+{
+    std::vector<void *> bs, ms;
+    {
+        std::lock_guard<std::mutex> lk(g_live_mu);
+        bs.assign(g_live[1].begin(), g_live[1].end());
+        ms.assign(g_live[0].begin(), g_live[0].end());
+    }
+    for (void *b : bs) mf_batch_destroy((mf_batch)b);
+    for (void *m : ms) mf_matrix_destroy((mf_matrix)m);  // children go with their parents
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_device >= 0) {
+        (void)hipSetDevice(g_device);
+        mf::release_phase_buffer();
+        if (g_stream) (void)hipStreamDestroy(g_stream);
+        g_stream = nullptr;
+        g_device = -1;
+    }
 }
 
 int mf_matrix_export(mf_matrix m, mf_ref_tiles *t)
@@ -481,7 +526,8 @@ static double layout_bytes(const mf_matrix m, const mf::Layout &L)
 int mf_abi_sizes(size_t *sizes, int n)
 {
     const size_t all[] = {sizeof(mf_ref_tiles), sizeof(mf_matrix_info), sizeof(mf_opts), sizeof(mf_report),
-                          sizeof(mf_batch_info_t), sizeof(mf_memory), sizeof(mf_compact), sizeof(mf_partition)};
+                          sizeof(mf_batch_info_t), sizeof(mf_memory), sizeof(mf_compact), sizeof(mf_partition),
+                          sizeof(Tile_matrix)};
     const int k = (int)(sizeof(all) / sizeof(all[0]));
     for (int i = 0; i < k && i < n && sizes; i++) sizes[i] = all[i];
     return k;

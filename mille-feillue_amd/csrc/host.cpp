@@ -286,6 +286,21 @@ int mmio_data(int *csrRowPtr, int *csrColIdx, double *csrVal, char *filename)
     return 0;
 }
 
+// PC/mmio_highlevel.h:594 -- mmio_data with single-precision values: the
+// same CSR, each value converted from its double parse (round to nearest).
+int mmio_data_Low(int *csrRowPtr, int *csrColIdx, float *csrVal, char *filename)
+{
+    MMData d;
+    int rc = mm_parse(filename, &d);
+    if (rc) return rc;
+    std::vector<double> v((size_t)std::max(1LL, mm_nnz_expanded(d)));
+    mm_to_csr(d, csrRowPtr, csrColIdx, v.data());
+    const long long nz = mm_nnz_expanded(d);
+    for (long long k = 0; k < nz; k++) csrVal[k] = (float)v[(size_t)k];
+    mm_free(&d);
+    return 0;
+}
+
 // ------------------------------------------------------ binary CSR (.cbd)
 // Format of PC/biio2.0/src/biio.h:22-94: int rows, int cols, int64 nnz,
 // int64 row_ptr[rows + 1], int col_idx[nnz], double val[nnz] (native byte
@@ -498,6 +513,78 @@ void exclusive_scan_char(unsigned char *input, int length)
         unsigned v = input[i];
         input[i] = (unsigned char)acc;
         acc = (acc + v) & 255u;
+    }
+}
+
+// PC/blockspmv_cpu.h:2-112 -- the reference's host routine that its drivers
+// call for the per-tile offsets (PC/Mille-feuille_CG_AMD.cu:1276):
+//  - rowblkblock and the row-block split: a tile row of more than 4 tiles
+//    (PREFETCH_SMEM_TH) becomes ceil(tiles / 4) pieces of ceil(tiles /
+//    pieces) tiles, each tagged with bit 31 and given its [colstart, colstop)
+//    tile range; shorter tile rows are one untagged entry (malloc'ed arrays);
+//  - ptroffset1[t] = nonzeros before tile t, ptroffset2[t] = row slots before
+//    tile t (rows of its tile row per tile);
+//  - y (tilem * 16 entries) = the tile-layout product A x, each tile row's
+//    rows summed per tile in element order.  The reference decodes the
+//    columns from csr_compressedIdx, which it never fills (the encode call is
+//    commented out), so its y uses column 0 of every tile; here the columns
+//    come from Tile_csr_Col, which holds them.  The CSR arguments and
+//    y_golden are unused, as in the reference.  A host routine of the
+//    reference's set-up path; the library's own SpMV runs on the GPU
+//    (mf_spmv).
+void blockspmv_cpu(Tile_matrix *matrix, int *ptroffset1, int *ptroffset2, int *rowblkblock,
+                   unsigned int **blkcoostylerowidx, int **blkcoostylerowidx_colstart,
+                   int **blkcoostylerowidx_colstop, int rowA, int colA, int nnzA, int *csrRowPtrA,
+                   int *csrColIdxA, double *csrValA, double *x, double *y, double *y_golden)
+{
+    (void)colA; (void)nnzA; (void)csrRowPtrA; (void)csrColIdxA; (void)csrValA; (void)y_golden;
+    constexpr int B = MF_TILE_BLOCK_SIZE, TH = 4;
+    const Tile_matrix &t = *matrix;
+    auto pieces = [&](int bi) {
+        const int nb = t.tile_ptr[bi + 1] - t.tile_ptr[bi];
+        return nb <= TH ? 1 : (nb + TH - 1) / TH;
+    };
+    int rbb = 0;
+    for (int bi = 0; bi < t.tilem; bi++) rbb += pieces(bi);
+    *rowblkblock = rbb;
+    unsigned int *br = (unsigned int *)std::calloc((size_t)std::max(rbb, 1), sizeof(unsigned int));
+    int *cs = (int *)std::calloc((size_t)std::max(rbb, 1), sizeof(int));
+    int *ce = (int *)std::calloc((size_t)std::max(rbb, 1), sizeof(int));
+    for (int bi = 0, k = 0; bi < t.tilem; bi++) {
+        const int t0 = t.tile_ptr[bi], nb = t.tile_ptr[bi + 1] - t0, np = pieces(bi);
+        if (nb <= TH) {
+            br[k++] = (unsigned int)bi;
+            continue;
+        }
+        const int len = (nb + np - 1) / np;
+        for (int q = 0; q < np; q++, k++) {
+            br[k] = (unsigned int)bi | 0x80000000u;
+            cs[k] = t0 + q * len;
+            ce[k] = q == np - 1 ? t0 + nb : t0 + (q + 1) * len;
+        }
+    }
+    *blkcoostylerowidx = br;
+    *blkcoostylerowidx_colstart = cs;
+    *blkcoostylerowidx_colstop = ce;
+    int off1 = 0, off2 = 0;
+    for (int bi = 0; bi < t.tilem; bi++) {
+        const int rowlen = bi == t.tilem - 1 ? rowA - (t.tilem - 1) * B : B;
+        for (int ri = 0; ri < B; ri++) y[(size_t)bi * B + ri] = 0.0;
+        for (int tid = t.tile_ptr[bi]; tid < t.tile_ptr[bi + 1]; tid++) {
+            const int nz = t.blknnz[tid + 1] - t.blknnz[tid];
+            const size_t xo = (size_t)t.tile_columnidx[tid] * B;
+            ptroffset1[tid] = off1;
+            ptroffset2[tid] = off2;
+            for (int ri = 0; ri < rowlen; ri++) {
+                const int stop = ri == rowlen - 1 ? nz : t.Blockcsr_Ptr[off2 + ri + 1];
+                double sum = 0.0;
+                for (int rj = t.Blockcsr_Ptr[off2 + ri]; rj < stop; rj++)
+                    sum += x[xo + t.Tile_csr_Col[off1 + rj]] * t.Blockcsr_Val[off1 + rj];
+                y[(size_t)bi * B + ri] += sum;
+            }
+            off1 += nz;
+            off2 += rowlen;
+        }
     }
 }
 

@@ -108,4 +108,10 @@ int vendor_solve_device(mf_matrix m, const double *d_b, double *d_x, const mf_op
 int encode_cols_device(mf_matrix m, uint8_t *d_out);  // tiles.hip
 int classes_device(mf_matrix m, double tau, int allow_mask, uint8_t *d_codes);  // tiles.hip
 hipStream_t stream();
+// Live handles, so that mf_finalize can release what callers still hold and
+// later destroy calls on released handles do nothing (capi.cpp).
+void live_add(void *h, int kind);     // kind 0 matrix, 1 batch
+bool live_remove(void *h, int kind);  // false when h is not live
+bool live_has(void *h, int kind);
+void release_phase_buffer();          // solver.hip
 }  // namespace mf

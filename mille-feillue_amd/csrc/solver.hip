@@ -250,6 +250,14 @@ static void solver_grid(int tilem, int B, int grid, int *G_out, int *trw_out)
     *trw_out = trw;
 }
 
+// diagnostic phase clock words of the persistent launches (opts.profile)
+static unsigned long long *d_phase = nullptr;
+void release_phase_buffer()
+{
+    if (d_phase) (void)hipFree(d_phase);
+    d_phase = nullptr;
+}
+
 static int launch_inner(mf_matrix m, const Layout &L, const Layout *L64, int method, const double *d_b,
                         double *d_x, double tol, const mf_opts &o, bool sync, InnerResult *res, int *grid_out,
                         int *block_out, int *rpt_out)
@@ -297,7 +305,6 @@ static int launch_inner(mf_matrix m, const Layout &L, const Layout *L64, int met
     A.nbr = method == MF_BICGSTAB && o.sync != MF_SYNC_GLOBAL && w.dep_max <= 64 ? 1 : 0;
     if (res) res->sync_deps = A.nbr ? w.dep_max : -1;
     static const bool phase_profile = std::getenv("MF_PHASE_PROFILE") != nullptr;
-    static unsigned long long *d_phase = nullptr;
     const size_t phase_words = 16 + 8 * 1024;
     const bool prof = phase_profile || o.profile;
     if (prof && !d_phase) MF_HIP(hipMalloc((void **)&d_phase, phase_words * sizeof(unsigned long long)));

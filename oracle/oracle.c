@@ -839,6 +839,15 @@ static double vdot(int n, const double *a, const double *b, int nt)
     return s;
 }
 
+/* residual history (orc_report.hist): entry k = updated ||r_k|| */
+static void hist_put(orc_report *rep, int k, double v)
+{
+    if (rep->hist && k < rep->hist_cap) {
+        rep->hist[k] = v;
+        if (k + 1 > rep->hist_len) rep->hist_len = k + 1;
+    }
+}
+
 static double true_relres(const orc_op *A, const double *b, const double *x, double *tmp, double bnorm)
 {
     op_apply(A, x, tmp);
@@ -860,6 +869,8 @@ int orc_cg(int n, const int *rowptr, const int *colidx, const double *val,
     double snew = vdot(n, r, r, nt);
     int it = 0, conv = 0;
     double tr = 1.0;
+    rep->hist_len = 0;
+    hist_put(rep, 0, sqrt(snew));
     if (bnorm == 0.0) { conv = 1; tr = 0.0; }
     while (!conv && it < maxit) {
         op_apply(&A, d, q);                      /* q = A d */
@@ -874,6 +885,7 @@ int orc_cg(int n, const int *rowptr, const int *colidx, const double *val,
 #pragma omp parallel for num_threads(nt) schedule(static) if (nt > 1)
         for (int i = 0; i < n; i++) d[i] = r[i] + beta * d[i];
         it++;
+        hist_put(rep, it, sqrt(snew));
         if (sqrt(snew) <= tol * bnorm) {
             tr = true_relres(&A, b, x, tmp, bnorm);
             if (tr <= tol) conv = 1;
@@ -902,6 +914,8 @@ int orc_bicgstab(int n, const int *rowptr, const int *colidx, const double *val,
     double rho = vdot(n, r, rh, nt), rr = rho;
     int it = 0, conv = 0;
     double tr = 1.0;
+    rep->hist_len = 0;
+    hist_put(rep, 0, sqrt(rr));
     if (bnorm == 0.0) { conv = 1; tr = 0.0; }
     while (!conv && it < maxit) {
         op_apply(&A, p, v);                       /* v = A p */
@@ -918,6 +932,7 @@ int orc_bicgstab(int n, const int *rowptr, const int *colidx, const double *val,
         double rho_new = vdot(n, r, rh, nt);
         rr = vdot(n, r, r, nt);
         it++;
+        hist_put(rep, it, sqrt(rr));
         if (sqrt(rr) <= tol * bnorm) {
             tr = true_relres(&A, b, x, tmp, bnorm);
             if (tr <= tol) { conv = 1; break; }
@@ -949,6 +964,8 @@ int orc_petsc_cg(int n, const int *rowptr, const int *colidx, const double *val,
     double dp = sqrt(vdot(n, R, R, nt)), rnorm0 = dp;
     double beta = vdot(n, R, R, nt), betaold = 1.0, dpi = 0.0, dpiold;
     int its = 0, conv = dp == 0.0, i = 0;
+    rep->hist_len = 0;
+    hist_put(rep, 0, dp);
     while (!conv && i < maxit) {
         its = i + 1;
         if (beta == 0.0) { conv = 1; break; }
@@ -965,8 +982,9 @@ int orc_petsc_cg(int n, const int *rowptr, const int *colidx, const double *val,
         if (dpi == 0.0 || (i > 0 && dpi * dpiold < 0.0)) break;
         double a = beta / dpi;
         for (int k = 0; k < n; k++) { x[k] += a * P[k]; R[k] -= a * W[k]; }
-        dp = sqrt(vdot(n, R, R, nt));
-        beta = dp * dp;
+        beta = vdot(n, R, R, nt); /* beta <- z'r with z = r (PCNONE), cg.c:363 */
+        dp = sqrt(beta);          /* dp <- ||r||, cg.c:316 */
+        hist_put(rep, i + 1, dp);
         if (dp < rtol * rnorm0) { conv = 1; break; }
         i++;
     }
@@ -991,6 +1009,8 @@ int orc_petsc_bcgs(int n, const int *rowptr, const int *colidx, const double *va
     double dp = sqrt(vdot(n, R, R, nt)), rnorm0 = dp;
     double rhoold = 1.0, alpha = 1.0, omegaold = 1.0;
     int its = 0, conv = dp == 0.0, i = 0;
+    rep->hist_len = 0;
+    hist_put(rep, 0, dp);
     while (!conv && i < maxit) {
         double rho = vdot(n, R, RP, nt);
         double beta = (rho / rhoold) * (alpha / omegaold);
@@ -1004,14 +1024,19 @@ int orc_petsc_bcgs(int n, const int *rowptr, const int *colidx, const double *va
         d1 = vdot(n, S, T, nt);
         double d2 = vdot(n, T, T, nt);
         if (d2 == 0.0) {
+            /* t = 0: converged only when s = 0 too, else a singular operator (bcgs.c:96-110) */
+            if (vdot(n, S, S, nt) != 0.0) break;
             for (int k = 0; k < n; k++) x[k] += alpha * P[k];
-            its++; dp = 0.0; conv = 1; break;
+            its++; dp = 0.0; conv = 1;
+            hist_put(rep, its, dp);
+            break;
         }
         double omega = d1 / d2;
         for (int k = 0; k < n; k++) { x[k] += alpha * P[k] + omega * S[k]; R[k] = S[k] - omega * T[k]; }
         dp = sqrt(vdot(n, R, R, nt));
         rhoold = rho; omegaold = omega;
         its++;
+        hist_put(rep, its, dp);
         if (dp < rtol * rnorm0) { conv = 1; break; }
         if (rho == 0.0) break;
         i++;

@@ -17,7 +17,6 @@
  *                         16x16 reconstruction.
  *   orc_encode/decode     oracle/_ref build of PC/encode.h; round trip.
  *   orc_blockspmv_offsets oracle/_ref build of PC/blockspmv_cpu.h.
- *   orc_compact_rows      invariants (row lists reproduce Blockcsr_Ptr rows).
  *   orc_partition         known answer index=149 for 1138_bus at each_nnz=16
  *                         (MP/data/cg_syncfree_reduce_2757.csv); invariants.
  *   orc_mc_classify       closed form on hand-built values at the thresholds.
@@ -30,9 +29,16 @@
  *   orc_cg / orc_bicgstab scipy.sparse.linalg.spsolve solution; textbook
  *                         properties (CG converges in <= n steps on tiny SPD,
  *                         exact after k steps with k distinct eigenvalues).
- *   orc_petsc_cg/_bcgs    parity unpinned against PETSc itself (PETSc cannot
- *                         be built here); only checked to agree with orc_cg
- *                         within one iteration and to converge.
+ *   orc_petsc_cg/_bcgs    PETSc itself cannot be built here; pinned by what the
+ *                         mathematics fixes: KSPCG / KSPBCGS and the textbook
+ *                         Hestenes-Stiefel CG / van der Vorst BiCGSTAB have the
+ *                         same iterates in exact arithmetic, so their residual
+ *                         histories agree to rounding with orc_cg / orc_bicgstab
+ *                         over the first iterations; finite termination on
+ *                         matrices with two or three distinct eigenvalues.
+ *   orc_compact_rows      stops and listed rows brute-forced from the reference
+ *                         tile arrays (golden Blockcsr_Ptr, uint8 wrap included)
+ *                         and checked against the CSR row counts mod 256.
  */
 #ifndef MF_ORACLE_H
 #define MF_ORACLE_H
@@ -145,6 +151,11 @@ typedef struct {
     double relres_true;    /* ||b - A x|| / ||b|| at exit */
     double relres_updated; /* recursively updated residual at exit */
     int converged;
+    /* input, optional: when hist is set, hist[k] receives the updated
+     * residual norm ||r_k|| after iteration k (hist[0] = ||r_0||) for
+     * k < hist_cap; hist_len = entries written */
+    double *hist;
+    int hist_cap, hist_len;
 } orc_report;
 /* Hestenes-Stiefel CG, x0 = 0 (PC/Mille-feuille_CG_AMD.cu:607-912 algebra).
  * Stop: updated ||r|| <= tol||b|| and then true ||b-Ax|| <= tol||b||.

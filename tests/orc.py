@@ -43,7 +43,27 @@ class Partition(C.Structure):
 
 class Report(C.Structure):
     _fields_ = [("iterations", C.c_int), ("relres_true", C.c_double),
-                ("relres_updated", C.c_double), ("converged", C.c_int)]
+                ("relres_updated", C.c_double), ("converged", C.c_int),
+                ("hist", _d), ("hist_cap", C.c_int), ("hist_len", C.c_int)]
+
+
+def _report(hist):
+    """A zeroed report; with hist > 0 it records that many residual norms
+    (available afterwards as rep.history)."""
+    rep = Report()
+    rep.history = None
+    if hist:
+        buf = np.zeros(hist, np.float64)
+        rep._buf = buf
+        rep.hist = buf.ctypes.data_as(_d)
+        rep.hist_cap = hist
+    return rep
+
+
+def _history(rep):
+    if getattr(rep, "_buf", None) is not None:
+        rep.history = rep._buf[: rep.hist_len].copy()
+    return rep
 
 
 def _arr(ptr, n, dtype):
@@ -164,15 +184,15 @@ class Oracle:
                             _p(np.ascontiguousarray(x, np.float64), C.c_double), _p(y, C.c_double))
         return y
 
-    def _solve(self, name, rowptr, col, val, b, tol, maxit, tiles=None, nthreads=1):
+    def _solve(self, name, rowptr, col, val, b, tol, maxit, tiles=None, nthreads=1, hist=0):
         n = len(b)
         x = np.zeros(n, np.float64)
-        rep = Report()
+        rep = _report(hist)
         tp = C.byref(tiles.t) if tiles is not None else None
         getattr(self.L, name)(n, _p(rowptr, C.c_int), _p(col, C.c_int), _p(val, C.c_double), tp,
                               _p(np.ascontiguousarray(b, np.float64), C.c_double), _p(x, C.c_double),
                               tol, maxit, nthreads, C.byref(rep))
-        return x, rep
+        return x, _history(rep)
 
     def cg(self, *a, **k):
         return self._solve("orc_cg", *a, **k)
@@ -180,14 +200,14 @@ class Oracle:
     def bicgstab(self, *a, **k):
         return self._solve("orc_bicgstab", *a, **k)
 
-    def _psolve(self, name, rowptr, col, val, b, rtol, maxit, nthreads=1):
+    def _psolve(self, name, rowptr, col, val, b, rtol, maxit, nthreads=1, hist=0):
         n = len(b)
         x = np.zeros(n, np.float64)
-        rep = Report()
+        rep = _report(hist)
         getattr(self.L, name)(n, _p(rowptr, C.c_int), _p(col, C.c_int), _p(val, C.c_double),
                               _p(np.ascontiguousarray(b, np.float64), C.c_double), _p(x, C.c_double),
                               rtol, maxit, nthreads, C.byref(rep))
-        return x, rep
+        return x, _history(rep)
 
     def petsc_cg(self, *a, **k):
         return self._psolve("orc_petsc_cg", *a, **k)

@@ -130,8 +130,8 @@ def test_gpu_solvers_index_widths_identical(gpu, orc, method):
     A.close()
 
 
-def _cg_case(gpu, orc, rp, ci, va, b, **opts):
-    A = gpu.Matrix(rp, ci, va)
+def _cg_case(gpu, orc, rp, ci, va, b, B=16, **opts):
+    A = gpu.Matrix(rp, ci, va, B=B)
     x, rep = A.solve(b, method=gpu.MF_CG, tol=TOL, **opts)
     xo, orep = orc.cg(rp, ci, va, b, TOL, 20000)
     print(f"cg n={len(b)} gpu it={rep['iterations']} repl={rep['replacements']} oracle it={orep.iterations} "
@@ -143,9 +143,9 @@ def _cg_case(gpu, orc, rp, ci, va, b, **opts):
     return rep
 
 
-def _cg_case_loose(gpu, orc, rp, ci, va, b, **opts):
+def _cg_case_loose(gpu, orc, rp, ci, va, b, B=16, **opts):
     # lossy tiles change the iterations; the answer must still be the FP64 one
-    A = gpu.Matrix(rp, ci, va)
+    A = gpu.Matrix(rp, ci, va, B=B)
     x, rep = A.solve(b, method=gpu.MF_CG, tol=TOL, **opts)
     xo, orep = orc.cg(rp, ci, va, b, TOL, 20000)
     print(f"cg lossy n={len(b)} gpu it={rep['iterations']} repl={rep['replacements']} oracle it={orep.iterations} "
@@ -204,10 +204,10 @@ def test_gpu_cg_deterministic(gpu, orc):
     assert r1["iterations"] == r2["iterations"] and np.array_equal(x1, x2)
 
 
-def _bicg_case(gpu, orc, rp, ci, va, b, check_iters=True, **opts):
-    A = gpu.Matrix(rp, ci, va)
+def _bicg_case(gpu, orc, rp, ci, va, b, check_iters=True, B=16, nthreads=1, **opts):
+    A = gpu.Matrix(rp, ci, va, B=B)
     x, rep = A.solve(b, method=gpu.MF_BICGSTAB, tol=TOL, **opts)
-    xo, orep = orc.bicgstab(rp, ci, va, b, TOL, 20000)
+    xo, orep = orc.bicgstab(rp, ci, va, b, TOL, 20000, nthreads=nthreads)
     print(f"bicgstab n={len(b)} gpu it={rep['iterations']} outer={rep['outer']} repl={rep['replacements']} "
           f"oracle it={orep.iterations} relres={rep['relres_true']:.2e} ms={rep['ms_solve']:.2f}", flush=True)
     assert rep["status"] == 0
@@ -491,3 +491,63 @@ def test_gpu_one_workgroup_paths(gpu, orc, method):
     print(f"case {method} n={n} grid=1 (global gather)", flush=True)
     rep = case(gpu, orc, rp, ci, va, b, grid=1)
     assert rep["grid"] == 1
+
+
+@pytest.mark.parametrize("method", ["cg", "bicgstab"])
+@pytest.mark.parametrize("tau", [0.0, 2.0 ** -11])
+def test_gpu_one_workgroup_mixed(gpu, orc, method, tau):
+    """One-workgroup solves with per-tile classes: the FP32 copy of the
+    gathered vector sits in LDS beside the FP64 one, and with lossy tiles the
+    true-residual passes read the FP64 layout.  The default grid puts these
+    systems on one workgroup; both solves reach 1e-10 and the oracle's x."""
+    if method == "cg":
+        g = load_golden("1138_bus")
+        rp, ci, va = g["rowptr"], g["col"], g["val"]
+    else:
+        rp, ci, va = gpu.gen_convdiff3d(10, 10.0, 1.0, 0)
+    n = len(rp) - 1
+    assert n <= 2048
+    b = orc.csr_spmv(rp, ci, va, gpu.gen_uniform(8, n, -1, 1))
+    codes = orc.tiles(16, n, n, rp, ci, va).classes(tau)
+    print(f"case {method} n={n} tau={tau} codes={np.bincount(codes, minlength=5).tolist()}", flush=True)
+    assert np.bincount(codes, minlength=5)[1:].sum() > 0  # low-precision tiles present
+    opts = dict(precision=gpu.MF_MIXED, tau=tau, rr_factor=1e-2, maxit=40000)
+    case = (_cg_case_loose if tau else _cg_case) if method == "cg" else _bicg_case
+    extra = {} if method == "cg" else {"check_iters": tau == 0.0}
+    for grid in (0, 1):
+        rep = case(gpu, orc, rp, ci, va, b, grid=grid, **opts, **extra)
+        assert rep["grid"] == 1
+
+
+@pytest.mark.parametrize("B", [8, 32])
+@pytest.mark.parametrize("method", ["cg", "bicgstab"])
+def test_gpu_one_workgroup_tile_sizes(gpu, orc, B, method):
+    """8- and 32-row tiles on one workgroup (LDS-resident gathered vector),
+    FP64 and lossless mixed, against the oracle."""
+    rp, ci, va = gpu.gen_poisson3d(12, 12, 12) if method == "cg" else gpu.gen_convdiff3d(12, 10.0, 1.0, 0)
+    n = len(rp) - 1
+    assert n <= 2048
+    b = orc.csr_spmv(rp, ci, va, gpu.gen_uniform(9, n, -1, 1))
+    case = _cg_case if method == "cg" else _bicg_case
+    for prec in (gpu.MF_FP64, gpu.MF_MIXED):
+        print(f"case {method} B={B} n={n} precision={prec}", flush=True)
+        rep = case(gpu, orc, rp, ci, va, b, B=B, grid=1, precision=prec, tau=0.0, rr_factor=1e-2)
+        assert rep["grid"] == 1
+
+
+def test_gpu_bicgstab_convdiff96_lossy_e4m3(gpu, orc):
+    """The 27-point convection-diffusion system at 96^3 (n = 884,736) with
+    every tile whose values E4M3 rounds within 2^-4 stored in 8 bits: BiCGSTAB
+    with FP32 accumulation and in-kernel reliable updates against the FP64
+    layout reaches 1e-10 and the oracle's FP64 solution."""
+    rp, ci, va = gpu.gen_convdiff3d(96, 10.0, 1.0, 0)
+    n = len(rp) - 1
+    b = orc.csr_spmv(rp, ci, va, gpu.gen_uniform(0, n, -1, 1))
+    A = gpu.Matrix(rp, ci, va)
+    A.layout(2.0 ** -4)
+    codes = A.info()["nnz_per_code"]
+    print(f"case convdiff96 e4m3 nnz_per_code={codes}", flush=True)
+    assert codes[2] > 0.5 * sum(codes)  # most nonzeros in E4M3 tiles
+    A.close()
+    _bicg_case(gpu, orc, rp, ci, va, b, check_iters=False, nthreads=16, precision=gpu.MF_MIXED, tau=2.0 ** -4,
+               rr_factor=1e-2)

@@ -7,7 +7,7 @@ the oracle exactly."""
 import numpy as np
 import pytest
 
-from fixtures import golden_cases, golden_tiles, golden_variants, load_golden
+from fixtures import golden_cases, golden_tiles, golden_variants, load_golden, tile_matrix_arrays
 from orc import REF_ARRAYS
 
 pytestmark = pytest.mark.gpu
@@ -25,6 +25,30 @@ def test_gpu_tile_build_matches_reference(gpu, name):
             assert np.array_equal(d[k], ref[k]), (key, k)
         assert np.array_equal(A.encode_cols(), g[f"{key}_encode"]), key
         A.close()
+
+
+@pytest.mark.parametrize("name", golden_cases())
+def test_gpu_tile_create_drop_in(gpu, name):
+    """Tile_create (PC/csr2block.h:267) through the C-ABI fills the caller's
+    Tile_matrix with arrays equal to the reference's own (golden fixtures),
+    blockspmv_cpu reads that struct, and Tile_destroy releases and zeroes it."""
+    g = load_golden(name)
+    for key, B, rowA in golden_variants(g):
+        if B != 16:
+            continue
+        print(f"case {name} {key}", flush=True)
+        t = gpu.tile_create(rowA, int(g["n"]), g["rowptr"], g["col"], g["val"])
+        assert t.tilem == (rowA + 15) // 16 and t.tilen == (int(g["n"]) + 15) // 16
+        d = tile_matrix_arrays(gpu, t)
+        ref = golden_tiles(g, key)
+        for k in REF_ARRAYS:
+            assert np.array_equal(d[k], ref[k]), (key, k)
+        assert t.csrsize == len(ref["Blockcsr_Val"]) and t.csrptrlen == len(ref["Blockcsr_Ptr"])
+        off = gpu.blockspmv_cpu(t, rowA, int(g["n"]), np.ones(int(g["n"])))
+        for k in ("ptroffset1", "ptroffset2", "blkrow", "colstart", "colstop"):
+            assert np.array_equal(np.asarray(off[k]), g[f"{key}_off_{k}"]), (key, k)
+        gpu.tile_destroy(t)
+        assert t.tilem == 0 and not t.tile_ptr and not t.Blockcsr_Val
 
 
 def _configs(mf):

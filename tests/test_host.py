@@ -8,7 +8,8 @@ import re
 import numpy as np
 import pytest
 
-from fixtures import GOLDEN, golden_cases, golden_tiles, golden_variants, load_golden, to_ref_struct
+from fixtures import (GOLDEN, golden_cases, golden_tiles, golden_variants, load_golden, to_ref_struct,
+                      to_tile_matrix)
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -232,3 +233,49 @@ def test_binding_structs_match_library(mf):
     sizes = mf.abi_sizes()
     bad = {k: v for k, v in sizes.items() if v[0] != v[1]}
     assert not bad, bad
+
+
+@pytest.mark.parametrize("name", golden_cases())
+def test_blockspmv_cpu_matches_reference(mf, orc, name):
+    """blockspmv_cpu over the reference-built Tile_matrix arrays (golden
+    fixtures): the per-tile offsets and the row-block split equal the
+    reference's own outputs bit for bit; y equals the oracle's tile product
+    over the same arrays, and the CSR product where no tile holds more than
+    255 entries (beyond, the reference's uint8 row starts wrap and the tile
+    arrays no longer hold every entry).  The reference's own y decodes an
+    unfilled column array and is not comparable."""
+    import scipy.sparse as sp
+    g = load_golden(name)
+    n = int(g["n"])
+    rng = np.random.default_rng(7)
+    for key, B, rowA in golden_variants(g):
+        if B != 16:
+            continue
+        t = to_tile_matrix(mf, golden_tiles(g, key))
+        x = rng.standard_normal(n)
+        out = mf.blockspmv_cpu(t, rowA, n, x)
+        assert out["rowblkblock"] == int(g[f"{key}_off_rowblkblock"]), key
+        for k in ("ptroffset1", "ptroffset2", "blkrow", "colstart", "colstop"):
+            assert np.array_equal(np.asarray(out[k]), g[f"{key}_off_{k}"]), (key, k)
+        ot = orc.tiles(B, rowA, n, g["rowptr"], g["col"], g["val"])
+        xx = np.zeros(ot.d["tilen"] * B)
+        xx[:n] = x
+        want = ot.spmv(xx)[:rowA]
+        np.testing.assert_allclose(out["y"][:rowA], want, rtol=1e-12, atol=1e-12 * np.abs(want).max(), err_msg=key)
+        if np.diff(g[f"{key}_blknnz"]).max() <= 255:
+            A = sp.csr_matrix((g["val"], g["col"], g["rowptr"]), shape=(int(g["m"]), n))[:rowA]
+            np.testing.assert_allclose(out["y"][:rowA], A @ x, rtol=1e-12, atol=1e-12 * np.abs(want).max(),
+                                       err_msg=key)
+        assert not out["y"][rowA:].any()
+        assert not out["y_golden"].any()  # left untouched, as in the reference
+
+
+@pytest.mark.parametrize("name", ["1138_bus", "ani1_nonsymm", "LFAT5"])
+def test_mmio_data_low_matches_reference_reader(mf, name):
+    # PC/mmio_highlevel.h:594: the same CSR, values converted to float
+    rc, d = mf.mmio_low(os.path.join(GOLDEN, f"{name}.mtx"))
+    g = load_golden(name)
+    assert rc == 0
+    assert np.array_equal(d["rowptr"], g["rowptr"]) and np.array_equal(d["col"], g["col"])
+    assert np.array_equal(d["val"], g["val"].astype(np.float32))
+    assert mf.mmio_low(os.path.join(GOLDEN, "missing.mtx"))[0] == -1

@@ -120,6 +120,53 @@ def test_compact_rows_reproduce_rows(orc):
     assert c["block_signal"][: d["tilem"]].tolist() == np.diff(d["tile_ptr"]).tolist()
 
 
+@pytest.mark.parametrize("name", golden_cases())
+def test_compact_rows_stops_against_reference_tiles(orc, name):
+    """Every (row, stop) pair of orc_compact_rows, brute-forced from the tile
+    arrays the reference's own csr2block.h built (golden fixtures): per tile a
+    (0, 0) sentinel, then each row whose stop differs from its uint8 start,
+    where the stop is the next row's uint8 start, or the tile's nonzero count
+    for the tile's last row (PC/Mille-feuille_CG_AMD.cu:1459-1480), stored
+    as uint8.  The golden row starts are first checked against the CSR: each
+    row's entries in the tile's column range, mod 256."""
+    g = load_golden(name)
+    n = int(g["n"])
+    wrapped = 0
+    for key, B, rowA in golden_variants(g):
+        ref = golden_tiles(g, key)
+        tp, tcol, bn, po, ptr = (ref[k] for k in ("tile_ptr", "tile_columnidx", "blknnz", "csrptr_offset",
+                                                   "Blockcsr_Ptr"))
+        c = orc.tiles(B, rowA, n, g["rowptr"], g["col"], g["val"]).compact()
+        tilem, tilenum = len(tp) - 1, len(tcol)
+        assert len(c["nonzero_row_new"]) == tilenum + 1
+        cols = g["col"]
+        for bi in range(tilem):
+            rowlen = B if bi < tilem - 1 else rowA - (tilem - 1) * B
+            for tid in range(tp[bi], tp[bi + 1]):
+                cc, nz = int(po[tid]), int(bn[tid + 1] - bn[tid])
+                lo, hi = int(tcol[tid]) * B, int(tcol[tid]) * B + B
+                cnt = [int(np.count_nonzero((cols[g["rowptr"][bi * B + ri]:g["rowptr"][bi * B + ri + 1]] >= lo)
+                                            & (cols[g["rowptr"][bi * B + ri]:g["rowptr"][bi * B + ri + 1]] < hi)))
+                       for ri in range(rowlen)]
+                assert sum(cnt) == nz, (key, tid)
+                if rowlen > 1:  # scanned starts: S_ri mod 256
+                    assert [int(ptr[cc + ri]) for ri in range(rowlen)] == \
+                        [sum(cnt[:ri]) % 256 for ri in range(rowlen)], (key, tid)
+                expect = [(0, 0)]
+                for ri in range(rowlen):
+                    start = int(ptr[cc + ri])
+                    stop = nz if ri == rowlen - 1 else int(ptr[cc + ri + 1])
+                    if stop != start:
+                        expect.append((ri, stop % 256))
+                    wrapped += stop != sum(cnt[:ri + 1])
+                seg = slice(int(c["nonzero_row_new"][tid]), int(c["nonzero_row_new"][tid + 1]))
+                got = list(zip(c["blockrowid_new"][seg].tolist(), c["blockcsr_ptr_new"][seg].tolist()))
+                assert got == expect, (key, tid)
+        assert c["cnt_non_new"] == int(c["nonzero_row_new"][-1])
+    if name == "dup_wrap":
+        assert wrapped > 0  # tiles of more than 255 entries: the uint8 wrap is exercised
+
+
 def test_mc_classifier_closed_form(orc):
     # MC/memory_Mille_feillue.cu:866-918 bins |v| <= 3.9375, 60, 6000
     vals = np.array([3.9375, -3.9375, 3.94, 60.0, -60.0000001, 6000.0, 6000.5, 1e-300, -1e9])
@@ -255,16 +302,96 @@ def test_bicgstab_oracle_against_direct_solve(orc):
     assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-7
 
 
-def test_petsc_restatements_converge(orc):
+def _csr(A):
+    A = sp.csr_matrix(A)
+    return A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.astype(np.float64)
+
+
+def _spd_cases():
     g = load_golden("isai_spd")
-    A = sp.csr_matrix((g["val"], g["col"], g["rowptr"]), shape=(int(g["m"]), int(g["n"])))
-    b = A @ np.ones(A.shape[0])
-    x1, r1 = orc.petsc_cg(g["rowptr"], g["col"], g["val"], b, 1e-10, 1000)
-    x2, r2 = orc.cg(g["rowptr"], g["col"], g["val"], b, 1e-10, 1000)
-    # same Krylov method: iteration counts agree to a step
-    assert r1.converged and abs(r1.iterations - r2.iterations) <= 1
-    x3, r3 = orc.petsc_bcgs(g["rowptr"], g["col"], g["val"], b, 1e-10, 1000)
-    assert r3.converged and r3.relres_true < 1e-9
+    yield "isai_spd", (g["rowptr"], g["col"], g["val"])
+    # 2D 5-point Laplacian 24 x 24
+    k = 24
+    T = sp.diags([-1.0, 2.0, -1.0], [-1, 0, 1], shape=(k, k))
+    yield "laplace24", _csr(sp.kron(sp.eye(k), T) + sp.kron(T, sp.eye(k)))
+
+
+def _nonsym_cases():
+    g = load_golden("ani1_nonsymm")
+    yield "ani1_nonsymm", (g["rowptr"], g["col"], g["val"])
+    # well conditioned nonsymmetric: upwinded 1D convection-diffusion blocks
+    rng = np.random.default_rng(5)
+    n = 300
+    A = sp.diags([-1.3, 3.0, -0.4], [-1, 0, 1], shape=(n, n)) + sp.random(n, n, density=0.01, random_state=6) * 0.2
+    A = A.tocsr()
+    A.data += 0.0 * rng.standard_normal(A.nnz)
+    yield "convdiff1d", _csr(A)
+
+
+@pytest.mark.parametrize("case", list(range(2)))
+def test_petsc_cg_iterates_match_textbook_cg(orc, case):
+    """KSPSolve_CG (cg.c:112, unpreconditioned) and Hestenes-Stiefel CG have the
+    same iterates in exact arithmetic: the residual histories agree to
+    rounding, iteration by iteration (a dropped or wrong-signed term in
+    either restatement changes them from the second iteration on)."""
+    name, (rp, ci, va) = list(_spd_cases())[case]
+    n = len(rp) - 1
+    A = sp.csr_matrix((va, ci, rp), shape=(n, n))
+    b = A @ np.linspace(-1.0, 2.0, n)
+    xp, rp_ = orc.petsc_cg(rp, ci, va, b, 1e-10, 2000, hist=2001)
+    xc, rc_ = orc.cg(rp, ci, va, b, 1e-10, 2000, hist=2001)
+    hp, hc = rp_.history, rc_.history
+    k = min(len(hp), len(hc), 11)
+    assert k >= 10, (name, len(hp), len(hc))
+    np.testing.assert_allclose(hp[:k], hc[:k], rtol=1e-8, err_msg=name)
+    assert rp_.converged and rc_.converged
+    assert abs(rp_.iterations - rc_.iterations) <= 1, name
+    xd = spl.spsolve(A.tocsc(), b)
+    assert np.linalg.norm(xp - xd) / np.linalg.norm(xd) < 1e-8, name
+
+
+@pytest.mark.parametrize("case", list(range(2)))
+def test_petsc_bcgs_iterates_match_bicgstab(orc, case):
+    """KSPSolve_BCGS (bcgs.c:19-150) and van der Vorst BiCGSTAB are the same
+    recurrences (p = r + beta (p - omega v) written as r - omega beta v +
+    beta p): their first ten residual norms agree to 1e-8 relative on well
+    conditioned nonsymmetric systems, and both reach the direct solution."""
+    name, (rp, ci, va) = list(_nonsym_cases())[case]
+    n = len(rp) - 1
+    A = sp.csr_matrix((va, ci, rp), shape=(n, n))
+    b = A @ np.linspace(-1.0, 1.0, n)
+    xp, rp_ = orc.petsc_bcgs(rp, ci, va, b, 1e-10, 2000, hist=2001)
+    xb, rb_ = orc.bicgstab(rp, ci, va, b, 1e-10, 2000, hist=2001)
+    hp, hb = rp_.history, rb_.history
+    k = min(len(hp), len(hb), 11)
+    assert k >= 10, (name, len(hp), len(hb))
+    np.testing.assert_allclose(hp[:k], hb[:k], rtol=1e-8, err_msg=name)
+    assert rp_.converged and rb_.converged
+    xd = spl.spsolve(A.tocsc(), b)
+    for x in (xp, xb):
+        assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-7, name
+
+
+def test_bicgstab_restatements_finite_termination(orc):
+    """A diagonal matrix with two distinct eigenvalues: BiCG's polynomial
+    annihilates r0 at step 2, so both BiCGSTAB restatements are exact after
+    two iterations (PETSc's through its d2 = (t, t) = 0 branch, bcgs.c:104)."""
+    n = 40
+    d = np.array([1.0, 3.0] * (n // 2))
+    rp = np.arange(n + 1, dtype=np.int32)
+    ci = np.arange(n, dtype=np.int32)
+    b = np.linspace(1.0, 2.0, n)
+    for solve in (orc.petsc_bcgs, orc.bicgstab):
+        x, rep = solve(rp, ci, d, b, 1e-12, 50, hist=51)
+        assert rep.converged and rep.iterations == 2, (solve, rep.iterations)
+        assert rep.history[-1] <= 1e-13 * rep.history[0]
+        np.testing.assert_allclose(x, b / d, rtol=1e-13)
+    # CG on three distinct eigenvalues: KSPCG exact after three iterations
+    d3 = np.array([1.0, 4.0, 9.0] * 16)
+    rp3 = np.arange(49, dtype=np.int32)
+    x, rep = orc.petsc_cg(rp3, rp3[:-1].copy(), d3, np.linspace(1, 2, 48), 1e-13, 100)
+    assert rep.converged and rep.iterations == 3
+    np.testing.assert_allclose(x, np.linspace(1, 2, 48) / d3, rtol=1e-12)
 
 
 @pytest.mark.skipif(not os.path.exists(os.path.join(REF_DIR, "libref_b16.so")), reason="oracle/_ref not built")

@@ -38,7 +38,9 @@ typedef enum {
     MF_E_BREAKDOWN = -20, /* Krylov breakdown (zero/negative curvature, rho=0) */
     MF_E_MAXIT = -21,     /* iteration limit before tolerance */
     MF_E_TIMEOUT = -22,   /* grid barrier spin limit reached; kernel aborted */
-    MF_E_NONFINITE = -23  /* NaN/Inf in a reduction */
+    MF_E_NONFINITE = -23, /* NaN/Inf in a reduction */
+    MF_E_DIVERGED = -24   /* lossy tiles: the true residual grew (rounded operator
+                             not definite); solves then fall back to FP64 tiles */
 } mf_status;
 
 const char *mf_status_str(int status);
@@ -197,6 +199,7 @@ enum { MF_CG = 0, MF_BICGSTAB = 1 };
 enum { MF_FP64 = 0, MF_MIXED = 1 };
 enum { MF_REORDER_AUTO = 0, MF_REORDER_OFF = 1, MF_REORDER_ON = 2 };
 enum { MF_SYNC_AUTO = 0, MF_SYNC_GLOBAL = 1 };
+enum { MF_VECTORS_AUTO = 0, MF_VECTORS_HBM = 1 };
 typedef struct {
     int method;         /* MF_CG or MF_BICGSTAB */
     int precision;      /* MF_FP64 or MF_MIXED */
@@ -227,6 +230,11 @@ typedef struct {
                            or s a workgroup waits only for the workgroups whose
                            rows its tile columns reference (when none references
                            more than 64); MF_SYNC_GLOBAL: a grid barrier */
+    int vectors;        /* MF_VECTORS_AUTO (default): the Krylov vectors of a
+                           workgroup's rows stay in registers and LDS where they
+                           fit (CG up to 16 rows per thread at B >= 16, else 8;
+                           BiCGSTAB 8) and are streamed from HBM beyond that;
+                           MF_VECTORS_HBM: always streamed (same iterates) */
 } mf_opts;
 void mf_opts_default(mf_opts *o);
 
@@ -248,6 +256,15 @@ typedef struct {
     double pad_factor;   /* stored slab entries / nonzeros of the natural FP64 layout */
     int sync_deps;       /* BiCGSTAB neighbour syncs: most workgroups one waits
                             for after a publish; -1: grid barriers */
+    int streamed;        /* 1: the Krylov vectors lived in HBM (systems beyond
+                            register/LDS residency, or opts.vectors) */
+    int fallback;        /* 1: the solve on lossy tiles diverged, broke down or
+                            hit maxit, and was redone on the FP64 layout (its
+                            time and iterations are included); batched: the
+                            systems that left their lossy tiles for the FP64
+                            layout inside the launch */
+    int split_rows;      /* reordered solves: rows longer than 64 entries,
+                            each split over 2..16 lanes of its tile row */
 } mf_report;
 
 /* Host buffers in/out (copies included; the timed ms_solve excludes them). */

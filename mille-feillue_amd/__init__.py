@@ -18,6 +18,7 @@ MF_CG, MF_BICGSTAB = 0, 1
 MF_FP64, MF_MIXED = 0, 1
 MF_REORDER_AUTO, MF_REORDER_OFF, MF_REORDER_ON = 0, 1, 2
 MF_SYNC_AUTO, MF_SYNC_GLOBAL = 0, 1
+MF_VECTORS_AUTO, MF_VECTORS_HBM = 0, 1
 CODE_FP64, CODE_FP8, CODE_FP16, CODE_FP32 = 0, 2, 3, 4
 ALLOW_ALL_LOW = (1 << 2) | (1 << 3) | (1 << 4)
 
@@ -57,7 +58,7 @@ class Opts(C.Structure):
                 ("tau", C.c_double), ("allow_mask", C.c_int), ("rr_factor", C.c_double),
                 ("ir_max_outer", C.c_int), ("ir_inner_tol", C.c_double), ("grid", C.c_int),
                 ("block", C.c_int), ("spin_limit", C.c_longlong), ("index_width", C.c_int), ("profile", C.c_int),
-                ("reorder", C.c_int), ("sync", C.c_int)]
+                ("reorder", C.c_int), ("sync", C.c_int), ("vectors", C.c_int)]
 
 
 class Report(C.Structure):
@@ -65,7 +66,8 @@ class Report(C.Structure):
                 ("replacements", C.c_int), ("relres_true", C.c_double), ("ms_solve", C.c_double),
                 ("ms_setup", C.c_double), ("grid", C.c_int), ("block", C.c_int),
                 ("rows_per_thread", C.c_int), ("phase_us", C.c_double * 5), ("reordered", C.c_int),
-                ("pad_factor", C.c_double), ("sync_deps", C.c_int)]
+                ("pad_factor", C.c_double), ("sync_deps", C.c_int), ("streamed", C.c_int),
+                ("fallback", C.c_int), ("split_rows", C.c_int)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}

@@ -281,8 +281,10 @@ struct BatchShared {
     double bc[BK_MAXV];
     int flag;
     double g[K], gp[K], ap[K], ref[K], b2[K], rs[K], ps[K], al[K], be[K];
+    double tprev[K];  // true (r, r) of each column at the last true-residual pass
     int mode[K];
     int bd;
+    int use64;        // 1: the system's SpMVs read the FP64 layout
 };
 
 // CG on one system (tile rows tr0 .. tr1 - 1 of it in this workgroup);
@@ -315,7 +317,8 @@ __device__ bool cg_system(const BatchArgs &A, BatchShared<K, TPB> &sh, int sys, 
         if (g == gw0 && tid == 0) {
             A.sys_out[4 * sys] = status;
             A.sys_out[4 * sys + 1] = iter;
-            A.sys_out[4 * sys + 2] = repl;
+            // bit 30: the system left its lossy tiles for the FP64 layout
+            A.sys_out[4 * sys + 2] = repl | (sh.use64 && A.haslow ? 1 << 30 : 0);
             // time of the system on its first workgroup (us; the clock runs at 100 MHz)
             A.sys_out[4 * sys + 3] = (int)min((wall_clock64() - t_start) / 100ull, 0x7fffffffull);
             A.sys_relres[sys] = relres;
@@ -340,10 +343,11 @@ __device__ bool cg_system(const BatchArgs &A, BatchShared<K, TPB> &sh, int sys, 
     }
     if (!S.allreduce(A, v, true)) return false;
     if (tid < K) {
-        sh_b2[tid] = sh_g[tid] = sh_ref[tid] = sh.bc[tid];
+        sh_b2[tid] = sh_g[tid] = sh_ref[tid] = sh.tprev[tid] = sh.bc[tid];
         sh_gp[tid] = sh_ap[tid] = 1.0;
         sh_mode[tid] = 1;
     }
+    if (tid == 0) sh.use64 = A.haslow ? 0 : 1;
     __syncthreads();
     int iter = 0, repl = 0;
     bool force = false;
@@ -407,6 +411,18 @@ __device__ bool cg_system(const BatchArgs &A, BatchShared<K, TPB> &sh, int sys, 
             }
             if (!isfinite(rel)) { finish(MF_E_NONFINITE, iter, repl, rel); return true; }
             if (done) { finish(MF_OK, iter, repl, rel); return true; }
+            // lossy tiles at their accuracy floor: a pass that follows a drop
+            // of the recurrence by rr_factor but shows a column's true residual
+            // down by less than 2x (or a breakdown) moves the system to the FP64
+            // layout for the rest of its solve (x, r and the direction are kept)
+            if (tid == 0) {
+                bool stall = false;
+                for (int k = 0; k < KA; k++) {
+                    stall = stall || (sh.bc[k] > tol2 * sh_b2[k] && sh.bc[k] > 0.25 * sh.tprev[k]);
+                    sh.tprev[k] = sh.bc[k];
+                }
+                if (stall) sh.use64 = 1;
+            }
             if (tid < K) {
                 const bool restart = force || sh.bc[tid] > 4.0 * sh_g[tid];
                 sh_mode[tid] = (restart ? 1 : 0) | 2;
@@ -441,7 +457,7 @@ __device__ bool cg_system(const BatchArgs &A, BatchShared<K, TPB> &sh, int sys, 
         for (int tr = tr0 + grp; tr < tr1; tr += GPB) {
             const size_t row = (size_t)tr * B + lane;
             double w[K];
-            row_spmvK<B, IW, K, K>(A.L, tr, lane, lanebase, 0, A.r, A.r32, w);
+            row_spmvK<B, IW, K, K>(sh.use64 ? A.L64 : A.L, tr, lane, lanebase, 0, A.r, A.r32, w);
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 const size_t e = row * K + k;
@@ -1095,7 +1111,7 @@ static int batch_stage(mf_batch_s *bt, const double *const *b)
 }
 
 struct BatchOutcome {
-    int status = MF_OK, itmax = 0, repl = 0;
+    int status = MF_OK, itmax = 0, repl = 0, fell = 0;
     double worst = 0.0;
     std::vector<int> iters;
     std::vector<double> relres;
@@ -1126,13 +1142,14 @@ static int batch_outcome(mf_batch_s *bt, BatchOutcome &oc)
     MF_HIP(hipMemcpy(oc.relres.data(), bt->d_relres, sizeof(double) * bt->nsys, hipMemcpyDeviceToHost));
     MF_HIP(hipMemcpy(&ab, bt->d_ctl, sizeof(int), hipMemcpyDeviceToHost));
     oc.status = ab ? MF_E_TIMEOUT : MF_OK;
-    oc.itmax = oc.repl = 0;
+    oc.itmax = oc.repl = oc.fell = 0;
     oc.worst = 0.0;
     for (int s = 0; s < bt->nsys; s++) {
         oc.iters[s] = so[4 * s + 1];
         oc.us[s] = so[4 * s + 3];
         oc.itmax = std::max(oc.itmax, so[4 * s + 1]);
-        oc.repl += so[4 * s + 2];
+        oc.repl += so[4 * s + 2] & 0x3fffffff;
+        oc.fell += so[4 * s + 2] >> 30;
         oc.worst = std::max(oc.worst, oc.relres[s]);
         if (oc.status == MF_OK && so[4 * s] != MF_OK) oc.status = so[4 * s];
     }
@@ -1147,6 +1164,7 @@ static void batch_report(mf_batch_s *bt, const BatchOutcome &oc, float ms, mf_re
     rep->iterations = oc.itmax;
     rep->outer = 1;
     rep->replacements = oc.repl;
+    rep->fallback = oc.fell;  // systems that moved to the FP64 layout
     rep->relres_true = oc.worst;
     rep->ms_solve = ms;
     rep->ms_setup = bt->ms_setup;

@@ -94,7 +94,7 @@ static void free_dev(mf_matrix m)
                   m->d_Tile_csr_Col, m->d_rowtmp, m->d_Blockcsr_Val, m->d_Blockcsr_Val_Low};
     for (void *p : ps)
         if (p) (void)hipFree(p);
-    void *qs[] = {m->d_perm, m->d_pb, m->d_px};
+    void *qs[] = {m->d_perm, m->d_pb, m->d_px, m->d_split};
     for (void *p : qs)
         if (p) (void)hipFree(p);
     if (m->child) mf_matrix_destroy(m->child);

@@ -255,6 +255,7 @@ const char *mf_status_str(int s)
     case MF_E_MAXIT: return "iteration limit reached";
     case MF_E_TIMEOUT: return "grid barrier timeout";
     case MF_E_NONFINITE: return "non-finite reduction";
+    case MF_E_DIVERGED: return "lossy tiles diverged";
     default: return "unknown status";
     }
 }

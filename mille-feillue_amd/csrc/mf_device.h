@@ -33,7 +33,23 @@ struct DevL {
     const int2 *trh;
     const int *stc;
     const uint8_t *blob;
+    // Rows split over lanes (the row-reordered copy of a matrix with long
+    // rows): split[tr] = S lanes per row in tile row tr (S = 1, 2, 4, 8 or 16,
+    // at most B; entries beyond tilem read 0 = no split).  Lanes j S .. j S +
+    // S - 1 hold consecutive segments of one row: the sum of their products
+    // goes to lane j S, the others (rows of their own without entries) get 0.
+    // nullptr: no split tile rows.
+    const uint8_t *split = nullptr;
 };
+
+// Sum over the S-lane segment of this lane (S a power of two within the
+// lane group, every lane of the segment active); the segment's first lane
+// keeps it, the other lanes return 0.
+__device__ __forceinline__ double split_sum(double v, int S, int lane)
+{
+    for (int off = 1; off < S; off <<= 1) v += __shfl_xor(v, off, 64);
+    return (lane & (S - 1)) ? 0.0 : v;
+}
 
 __host__ __device__ constexpr int cls_width(int cls) { return 8 >> cls; }
 
@@ -64,6 +80,7 @@ struct RowCtx {
     int t0;      // first tile of the tile row
     int tc_reg;  // index base: tile column of slot r (IW 1), tr * B (IW 2), 0 (IW 4)
     int4 h01, h23;  // slab headers of classes 0,1 and 2,3
+    int split;   // lanes per row (DevL::split), 1 when none
 };
 
 template <int B, int IW>
@@ -76,6 +93,7 @@ __device__ __forceinline__ RowCtx row_ctx(const DevL &L, int tr, int r)
     const int4 *h = reinterpret_cast<const int4 *>(L.trh + (size_t)tr * 4);
     c.h01 = h[0];
     c.h23 = h[1];
+    c.split = L.split ? (int)L.split[tr] : 1;
     return c;
 }
 
@@ -170,6 +188,7 @@ __device__ __forceinline__ RowCtx row_ctx_from(const DevL &L, int tr, int t0, in
     c.tc_reg = IW == 1 ? (r < nt ? L.stc[t0 + r] : 0) : IW == 2 ? tr * B : 0;
     c.h01 = h01;
     c.h23 = h23;
+    c.split = L.split ? (int)L.split[tr] : 1;
     return c;
 }
 
@@ -307,6 +326,13 @@ __device__ __forceinline__ void tile_rows_spmv(const DevL &L, const int4 *hd, co
     slab_spmv_rows<B, IW, 3, NR, QS, UQL4, UQ64>(L, hd, t0s, q0, tcr, r, lanebase, g32, a32);
 #pragma unroll
     for (int k = 0; k < NR; k++) y[k] = a64[k] + (double)a32[k];
+    if (L.split) {
+#pragma unroll
+        for (int k = 0; k < NR; k++) {
+            const int S = L.split[trq + k * QS];
+            if (S > 1) y[k] = split_sum(y[k], S, r);
+        }
+    }
 }
 
 // FP64-accumulated product of NV gathered vectors at once (true residuals and
@@ -342,6 +368,10 @@ __device__ __forceinline__ void tile_row_spmv_f64_c(const DevL &L, const RowCtx 
     if (c.h01.w) slab_spmv_f64<B, IW, 1, NV>(L, c, make_int2(c.h01.z, c.h01.w), r, lanebase, g, acc);
     if (c.h23.y) slab_spmv_f64<B, IW, 2, NV>(L, c, make_int2(c.h23.x, c.h23.y), r, lanebase, g, acc);
     if (c.h23.w) slab_spmv_f64<B, IW, 3, NV>(L, c, make_int2(c.h23.z, c.h23.w), r, lanebase, g, acc);
+    if (c.split > 1) {
+#pragma unroll
+        for (int q = 0; q < NV; q++) acc[q] = split_sum(acc[q], c.split, r);
+    }
 }
 
 template <int B, int IW, int NV>

@@ -35,7 +35,8 @@ struct Layout {
     bool has64 = false, haslow = false;
     double bytes_alg = 0.0;     // algorithmic bytes per SpMV
     double bytes_actual = 0.0;  // bytes the layout occupies (+ gather + y)
-    DevL view(const int *tile_ptr) const { return DevL{tile_ptr, trh, stc, blob}; }
+    const uint8_t *split = nullptr;  // the matrix's split tile rows (not owned)
+    DevL view(const int *tile_ptr) const { return DevL{tile_ptr, trh, stc, blob, split}; }
     void release();
 };
 
@@ -57,6 +58,9 @@ struct Workspace {
     int dep_G = 0, dep_trw = 0, dep_max = -1;
     int *dep_ptr = nullptr, *dep_idx = nullptr;
     unsigned *flags = nullptr;
+    // vectors of the streamed solver kernels (vec_n doubles)
+    double *vec = nullptr;
+    size_t vec_n = 0;
     void release();
 };
 
@@ -89,9 +93,13 @@ struct mf_matrix_s {
     // to the persistent workgroups' chunks): child = P A P^T, perm[new] = old
     mf_matrix_s *child = nullptr;
     mf::VendorCtx *vendor = nullptr;  // vendor-library baseline state (vendor.hip)
-    int *d_perm = nullptr;
+    int *d_perm = nullptr;  // child row -> row of this matrix, -1 for split segments and padding
     double *d_pb = nullptr, *d_px = nullptr;
     int child_grid = 0;
+    int child_split_rows = 0;  // rows of this matrix split over lanes in the child
+    // rows split over lanes (set on a reordered child): lanes per row of each
+    // tile row, tilem + SPLIT_PAD entries (see DevL::split)
+    uint8_t *d_split = nullptr;
 };
 
 namespace mf {

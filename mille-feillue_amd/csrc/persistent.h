@@ -15,6 +15,11 @@ namespace mf {
 // --------------------------------------------------------- solver kernels
 constexpr int SYNC_K = 4;  // values per grid allreduce, at most
 constexpr int LDS_GATHER = 2048;  // rows of a one-workgroup solve gathered from LDS
+// A true-residual pass on a lossy layout whose squared norm exceeds the best
+// earlier one by this factor (the norm grew 4x while the recurrence claimed
+// progress) ends the launch with MF_E_DIVERGED; the host then solves on the
+// FP64 layout.
+constexpr double DIVERGE_GROWTH = 16.0;
 
 struct SolveArgs {
     DevL L;
@@ -41,6 +46,10 @@ struct SolveArgs {
     const int *dep_ptr, *dep_idx;
     unsigned *flags;
     int nbr;
+    // Krylov vectors of the streamed kernels (systems beyond register/LDS
+    // residency): vector k of the padded rows at vec + k * vstride
+    double *vec;
+    long long vstride;
 };
 
 // Phase clock of block 0 (diagnostic; enabled when A.phase is set).
@@ -373,7 +382,7 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
         if (blockIdx.x == 0 && tid == 0) A.out[0] = sqrt(bnorm2);
         const double tol2 = A.tol * A.tol * bnorm2;
         if (bnorm2 == 0.0) { finish(MF_OK, 0, 0, 0.0, 0.0); return; }
-        double gamma = bnorm2, ref2 = bnorm2, gprev = 1.0;
+        double gamma = bnorm2, ref2 = bnorm2, gprev = 1.0, tbest = bnorm2;
         bool restart = false;   // next iteration starts a new direction (beta = 0)
         bool force = false;     // a breakdown asks for a true-residual pass + restart
         int iter = 0, repl = 0;
@@ -419,6 +428,10 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
                 ++repl;
                 if (!isfinite(t1[0])) { finish(MF_E_NONFINITE, iter, repl, t1[0], gamma); return; }
                 if (t1[0] <= tol2) { finish(MF_OK, iter, repl, t1[0], gamma); return; }
+            // a lossy layout whose rounded operator lost definiteness (or whose
+            // recurrences ran away) shows a true residual above the best one
+            if (A.use64 && t1[0] > DIVERGE_GROWTH * tbest) { finish(MF_E_DIVERGED, iter, repl, t1[0], gamma); return; }
+            tbest = fmin(tbest, t1[0]);
                 // reliable update: keep the direction p (s = A p was refreshed with
                 // it); when the recurrence had drifted by more than a factor 2 in
                 // norm, restart from the true residual
@@ -597,7 +610,7 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
         if (blockIdx.x == 0 && tid == 0) A.out[0] = sqrt(bnorm2);
         const double tol2 = A.tol * A.tol * bnorm2;
         if (bnorm2 == 0.0) { finish(MF_OK, 0, 0, 0.0, 0.0); return; }
-        double rho = bnorm2, ref2 = bnorm2;
+        double rho = bnorm2, ref2 = bnorm2, tbest = bnorm2;
         bool restart = false;
         int iter = 0, repl = 0;
         PhaseClock clk(A.phase);
@@ -686,6 +699,8 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
                 ++repl;
                 if (!isfinite(t2[0])) { finish(MF_E_NONFINITE, iter, repl, t2[0], g2); return; }
                 if (t2[0] <= tol2) { finish(MF_OK, iter, repl, t2[0], g2); return; }
+            if (A.use64 && t2[0] > DIVERGE_GROWTH * tbest) { finish(MF_E_DIVERGED, iter, repl, t2[0], g2); return; }
+            tbest = fmin(tbest, t2[0]);
                 restart = t2[0] > 4.0 * g2;  // recurrence drifted: new direction p = r
                 ref2 = t2[0];
                 rho_new = t2[1];
@@ -715,22 +730,388 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
     solve(A.g64, A.g32, std::false_type{});
 }
 
+// ------------------------------------------------- streamed (HBM) kernels
+// Systems whose rows per workgroup exceed what registers and LDS hold run the
+// same iterations with the Krylov vectors of a workgroup's rows in HBM.  Each
+// thread owns the rows tr0 * B + j * TPB + tid of its chunk (j ascending), the
+// same rows in the same order as a resident kernel on the same grid, so the
+// partial sums -- and with them every iterate -- are bit-identical to it.
+// Only the owning thread touches a row of these vectors; the gathered vector
+// is published as in the resident kernels.  The tile-row headers of the
+// chunk sit in LDS, a window of ST_HDR_BYTES at a time (one window for
+// chunks up to ~2700 tile rows, i.e. beyond 10M rows at B = 16 on 256
+// workgroups).
+constexpr int ST_HDR_BYTES = 96 * 1024;
+
+template <int B, int IW, int TPB>
+struct StreamGeom {
+    static constexpr int GPB = TPB / B;
+    static constexpr int NR = IW < 4 ? 4 : 2;             // tile rows per lane group per SpMV step
+    static constexpr int STEP = NR * GPB;                 // tile rows per SpMV step
+    static constexpr int HW = (ST_HDR_BYTES / 36) / STEP * STEP;  // tile rows of headers in LDS
+    static_assert(HW >= STEP, "header window");
+};
+
+// Stages the headers of tile rows [w0, w0 + HW) of chunk [.., tr1) in LDS
+// (zero headers beyond the chunk); every thread of the workgroup calls it.
+template <int B, int IW, int TPB>
+__device__ __forceinline__ void stage_headers(const SolveArgs &A, int4 *s_h, int *s_t0, int w0, int tr1)
+{
+    constexpr int HW = StreamGeom<B, IW, TPB>::HW;
+    __syncthreads();  // the previous window's readers are done
+    for (int q = threadIdx.x; q <= HW; q += TPB) {
+        const int tr = w0 + q;
+        if (q < HW && tr < tr1) {
+            const int4 *h = reinterpret_cast<const int4 *>(A.L.trh + (size_t)tr * 4);
+            s_h[2 * q] = h[0];
+            s_h[2 * q + 1] = h[1];
+        } else if (q < HW) {
+            s_h[2 * q] = s_h[2 * q + 1] = make_int4(0, 0, 0, 0);
+        }
+        s_t0[q] = A.L.tile_ptr[min(tr, A.tilem)];
+    }
+    __syncthreads();
+}
+
+// One SpMV pass of a streamed kernel over the chunk [tr0, tr1): y of the
+// thread's rows in ascending order, handed to body(row, y).  hw0 tracks the
+// window held in LDS (a chunk within one window is staged once per launch).
+template <int B, int IW, int TPB, typename F>
+__device__ __forceinline__ void stream_spmv(const SolveArgs &A, int4 *s_h, int *s_t0, int &hw0, int tr0, int tr1,
+                                            const double *g64, const float *g32, F &&body)
+{
+    using Geo = StreamGeom<B, IW, TPB>;
+    constexpr int GPB = Geo::GPB, NR = Geo::NR, STEP = Geo::STEP, HW = Geo::HW;
+    const int tid = threadIdx.x, grp = tid / B, lane = tid % B, lanebase = (tid & 63) & ~(B - 1);
+    for (int w0 = tr0; w0 < tr1; w0 += HW) {
+        if (hw0 != w0) {
+            stage_headers<B, IW, TPB>(A, s_h, s_t0, w0, tr1);
+            hw0 = w0;
+        }
+        const int w1 = min(tr1, w0 + HW);
+#pragma unroll 1
+        for (int s0 = w0; s0 < w1; s0 += STEP) {
+            double w[NR];
+            tile_rows_spmv<B, IW, NR, GPB, 4, 4>(A.L, s_h, s_t0, s0 - w0 + grp, s0 + grp, lane, lanebase, g64, g32, w);
+#pragma unroll
+            for (int k = 0; k < NR; k++) {
+                const int tr = s0 + k * GPB + grp;
+                if (tr < tr1) body((size_t)tr * B + lane, w[k]);
+            }
+        }
+    }
+}
+
+// FP64-accumulated product of NV published vectors over the chunk's tile
+// rows (true-residual passes), handed to body(row, acc) in the thread's row
+// order; headers are read from the layout (use64: the FP64 layout).
+template <int B, int IW, int TPB, int NV, typename F>
+__device__ __forceinline__ void stream_spmv_f64(const SolveArgs &A, int tr0, int tr1, const double *const (&g)[NV],
+                                                F &&body)
+{
+    const int tid = threadIdx.x, grp = tid / B, lane = tid % B, lanebase = (tid & 63) & ~(B - 1);
+#pragma unroll 1
+    for (int tr = tr0 + grp; tr < tr1; tr += TPB / B) {
+        double acc[NV];
+#pragma unroll
+        for (int q = 0; q < NV; q++) acc[q] = 0.0;
+        if (A.use64)
+            tile_row_spmv_f64_c<B, IW, NV>(A.L64, row_ctx<B, IW>(A.L64, tr, lane), lane, lanebase, g, acc);
+        else
+            tile_row_spmv_f64_c<B, IW, NV>(A.L, row_ctx<B, IW>(A.L, tr, lane), lane, lanebase, g, acc);
+        body((size_t)tr * B + lane, acc);
+    }
+}
+
+// CG with the vectors in HBM: the iteration of k_cg (same recurrences, same
+// syncs, same order of every partial sum).  Vectors: r, p, s at vec, x in
+// the caller's buffer.
+template <int B, int IW, int TPB>
+__global__ __launch_bounds__(TPB, 1) void k_cg_st(SolveArgs A)
+{
+    using Geo = StreamGeom<B, IW, TPB>;
+    __shared__ int4 s_h[2 * Geo::HW];
+    __shared__ int s_t0[Geo::HW + 1];
+    __shared__ double s_red[(TPB / 64) * 4];
+    __shared__ double s_bc[4];
+    __shared__ int s_flag;
+    Reducer<TPB, 4> R{s_red, s_bc, &s_flag};
+    const int tid = threadIdx.x;
+    const int chunk = xcd_chunk(blockIdx.x, gridDim.x);
+    const int tr0 = chunk * A.trw, tr1 = min(A.tilem, tr0 + A.trw);
+    const size_t row0 = (size_t)tr0 * B + tid, row1 = (size_t)tr1 * B;
+    double *__restrict__ vr = A.vec, *__restrict__ vp = A.vec + A.vstride, *__restrict__ vs = A.vec + 2 * A.vstride;
+    double *__restrict__ x = A.x;
+    int hw0 = -1;
+    auto publish = [&](size_t row, double v) {
+        if (A.has64) st_agent(A.g64 + row, v);
+        if (A.haslow) st_agent(A.g32 + row, (float)v);
+    };
+    auto finish = [&](int status, int iter, int repl, double rt2, double g2) {
+        if (blockIdx.x == 0 && tid == 0) {
+            A.ctl[1] = status;
+            A.ctl[2] = iter;
+            A.ctl[3] = repl;
+            A.out[1] = rt2;
+            A.out[2] = g2;
+        }
+    };
+    double v1[1] = {0.0};
+    for (size_t row = row0; row < row1; row += TPB) {
+        const double bv = A.b[row];
+        x[row] = 0.0;
+        vp[row] = 0.0;
+        vs[row] = 0.0;
+        vr[row] = bv;
+        publish(row, bv);
+        v1[0] += bv * bv;
+    }
+    if (!R.allreduce(A, v1, true)) return;
+    const double bnorm2 = v1[0];
+    if (blockIdx.x == 0 && tid == 0) A.out[0] = sqrt(bnorm2);
+    const double tol2 = A.tol * A.tol * bnorm2;
+    if (bnorm2 == 0.0) { finish(MF_OK, 0, 0, 0.0, 0.0); return; }
+    double gamma = bnorm2, ref2 = bnorm2, gprev = 1.0, tbest = bnorm2;
+    bool restart = false, force = false;
+    int iter = 0, repl = 0;
+    PhaseClock clk(A.phase);
+    for (;;) {
+        if (!isfinite(gamma)) { finish(MF_E_NONFINITE, iter, repl, -1.0, gamma); return; }
+        const bool conv = gamma <= tol2;
+        const bool due = force || (A.rr_factor > 0.0 && gamma <= A.rr_factor * A.rr_factor * ref2);
+        if (conv || due) {
+            // true residual b - A x and s = A p in one FP64-accumulated pass
+            for (size_t row = row0; row < row1; row += TPB) {
+                st_agent(A.h64 + row, x[row]);
+                st_agent(A.q64 + row, vp[row]);
+            }
+            if (!R.barrier(A)) return;
+            double t1[1] = {0.0};
+            const double *g[2] = {A.h64, A.q64};
+            stream_spmv_f64<B, IW, TPB, 2>(A, tr0, tr1, g, [&](size_t row, const double(&acc)[2]) {
+                const double rv = A.b[row] - acc[0];
+                vr[row] = rv;
+                vs[row] = acc[1];
+                publish(row, rv);
+                t1[0] += rv * rv;
+            });
+            if (!R.allreduce(A, t1, true)) return;
+            clk.lap(4);
+            ++repl;
+            if (!isfinite(t1[0])) { finish(MF_E_NONFINITE, iter, repl, t1[0], gamma); return; }
+            if (t1[0] <= tol2) { finish(MF_OK, iter, repl, t1[0], gamma); return; }
+            // a lossy layout whose rounded operator lost definiteness (or whose
+            // recurrences ran away) shows a true residual above the best one
+            if (A.use64 && t1[0] > DIVERGE_GROWTH * tbest) { finish(MF_E_DIVERGED, iter, repl, t1[0], gamma); return; }
+            tbest = fmin(tbest, t1[0]);
+            restart = force || t1[0] > 4.0 * gamma;
+            force = false;
+            gamma = ref2 = t1[0];
+            continue;
+        }
+        if (iter >= A.maxit) { finish(MF_E_MAXIT, iter, repl, -1.0, gamma); return; }
+        const bool fresh = iter == 0 || restart;
+        restart = false;
+        const double beta = fresh ? 0.0 : gamma / gprev;
+        // w = A r; s = w + beta s; p = r + beta p; (p, s)
+        double red[1] = {0.0};
+        stream_spmv<B, IW, TPB>(A, s_h, s_t0, hw0, tr0, tr1, A.g64, A.g32, [&](size_t row, double w) {
+            const double s = w + beta * vs[row];
+            vs[row] = s;
+            const double p = vr[row] + beta * vp[row];
+            vp[row] = p;
+            red[0] += p * s;
+        });
+        clk.lap(0);
+        if (!R.allreduce(A, red, false)) return;
+        clk.lap(1);
+        const double denom = red[0];
+        if (!isfinite(denom)) { finish(MF_E_NONFINITE, iter, repl, -1.0, gamma); return; }
+        if (!(denom > 0.0)) {
+            if (fresh) { finish(MF_E_BREAKDOWN, iter, repl, -1.0, gamma); return; }
+            force = true;
+            continue;
+        }
+        const double alpha = gamma / denom;
+        double g1[1] = {0.0};
+        for (size_t row = row0; row < row1; row += TPB) {
+            const double p = vp[row];
+            x[row] = x[row] + alpha * p;
+            const double rv = vr[row] - alpha * vs[row];
+            vr[row] = rv;
+            publish(row, rv);
+            g1[0] += rv * rv;
+        }
+        gprev = gamma;
+        ++iter;
+        clk.lap(2);
+        if (!R.allreduce(A, g1, true)) return;
+        clk.lap(3);
+        gamma = g1[0];
+    }
+}
+
+// BiCGSTAB with the vectors in HBM: the iteration of k_bicgstab.  Vectors:
+// r, r^, p, s, v = A p, t = A s at vec, x in the caller's buffer.
+template <int B, int IW, int TPB>
+__global__ __launch_bounds__(TPB, 1) void k_bicgstab_st(SolveArgs A)
+{
+    using Geo = StreamGeom<B, IW, TPB>;
+    __shared__ int4 s_h[2 * Geo::HW];
+    __shared__ int s_t0[Geo::HW + 1];
+    __shared__ double s_red[(TPB / 64) * 4];
+    __shared__ double s_bc[4];
+    __shared__ int s_flag;
+    Reducer<TPB, 4> R{s_red, s_bc, &s_flag};
+    const int tid = threadIdx.x;
+    const int chunk = xcd_chunk(blockIdx.x, gridDim.x);
+    const int tr0 = chunk * A.trw, tr1 = min(A.tilem, tr0 + A.trw);
+    const size_t row0 = (size_t)tr0 * B + tid, row1 = (size_t)tr1 * B;
+    const long long S = A.vstride;
+    double *__restrict__ vr = A.vec, *__restrict__ vrh = A.vec + S, *__restrict__ vp = A.vec + 2 * S;
+    double *__restrict__ vs = A.vec + 3 * S, *__restrict__ vv = A.vec + 4 * S, *__restrict__ vt = A.vec + 5 * S;
+    double *__restrict__ x = A.x;
+    int hw0 = -1;
+    auto publish = [&](size_t row, double v) {
+        if (A.has64) st_agent(A.g64 + row, v);
+        if (A.haslow) st_agent(A.g32 + row, (float)v);
+    };
+    auto finish = [&](int status, int iter, int repl, double rt2, double g2) {
+        if (blockIdx.x == 0 && tid == 0) {
+            A.ctl[1] = status;
+            A.ctl[2] = iter;
+            A.ctl[3] = repl;
+            A.out[1] = rt2;
+            A.out[2] = g2;
+        }
+    };
+    double v1[1] = {0.0};
+    for (size_t row = row0; row < row1; row += TPB) {
+        const double bv = A.b[row];
+        x[row] = 0.0;
+        vr[row] = vrh[row] = vp[row] = bv;
+        vv[row] = 0.0;
+        publish(row, bv);
+        v1[0] += bv * bv;
+    }
+    if (!R.allreduce(A, v1, true)) return;
+    const double bnorm2 = v1[0];
+    if (blockIdx.x == 0 && tid == 0) A.out[0] = sqrt(bnorm2);
+    const double tol2 = A.tol * A.tol * bnorm2;
+    if (bnorm2 == 0.0) { finish(MF_OK, 0, 0, 0.0, 0.0); return; }
+    double rho = bnorm2, ref2 = bnorm2, tbest = bnorm2;
+    bool restart = false;
+    int iter = 0, repl = 0;
+    PhaseClock clk(A.phase);
+    for (;;) {
+        // v = A p; (r^, v)
+        double a1[1] = {0.0};
+        stream_spmv<B, IW, TPB>(A, s_h, s_t0, hw0, tr0, tr1, A.g64, A.g32, [&](size_t row, double w) {
+            vv[row] = w;
+            a1[0] += vrh[row] * w;
+        });
+        clk.lap(0);
+        if (!R.allreduce(A, a1, false)) return;
+        clk.lap(1);
+        if (!(a1[0] != 0.0) || !isfinite(a1[0])) { finish(MF_E_BREAKDOWN, iter, repl, -1.0, -1.0); return; }
+        const double alpha = rho / a1[0];
+        for (size_t row = row0; row < row1; row += TPB) {
+            const double s = vr[row] - alpha * vv[row];
+            vs[row] = s;
+            publish(row, s);
+        }
+        clk.lap(2);
+        if (!R.published(A, chunk)) return;
+        clk.lap(3);
+        // t = A s; (t, s), (t, t)
+        double a2[2] = {0.0, 0.0};
+        stream_spmv<B, IW, TPB>(A, s_h, s_t0, hw0, tr0, tr1, A.g64, A.g32, [&](size_t row, double t) {
+            vt[row] = t;
+            a2[0] += t * vs[row];
+            a2[1] += t * t;
+        });
+        clk.lap(0);
+        if (!R.allreduce(A, a2, false)) return;
+        clk.lap(1);
+        const double omega = a2[1] > 0.0 ? a2[0] / a2[1] : 0.0;
+        double a3[2] = {0.0, 0.0};
+        for (size_t row = row0; row < row1; row += TPB) {
+            const double s = vs[row];
+            x[row] = x[row] + alpha * vp[row] + omega * s;
+            const double r = s - omega * vt[row];
+            vr[row] = r;
+            a3[0] += r * vrh[row];
+            a3[1] += r * r;
+        }
+        clk.lap(2);
+        if (!R.allreduce(A, a3, false)) return;
+        clk.lap(3);
+        double rho_new = a3[0], g2 = a3[1];
+        ++iter;
+        if (!finite_all(rho_new, g2)) { finish(MF_E_NONFINITE, iter, repl, -1.0, g2); return; }
+        const bool conv = g2 <= tol2;
+        const bool due = A.rr_factor > 0.0 && g2 <= A.rr_factor * A.rr_factor * ref2;
+        if (conv || due) {
+            for (size_t row = row0; row < row1; row += TPB) st_agent(A.h64 + row, x[row]);
+            if (!R.barrier(A)) return;
+            double t2[2] = {0.0, 0.0};
+            const double *g[1] = {A.h64};
+            stream_spmv_f64<B, IW, TPB, 1>(A, tr0, tr1, g, [&](size_t row, const double(&acc)[1]) {
+                const double r = A.b[row] - acc[0];
+                vr[row] = r;
+                t2[0] += r * r;
+                t2[1] += r * vrh[row];
+            });
+            if (!R.allreduce(A, t2, false)) return;
+            clk.lap(4);
+            ++repl;
+            if (!isfinite(t2[0])) { finish(MF_E_NONFINITE, iter, repl, t2[0], g2); return; }
+            if (t2[0] <= tol2) { finish(MF_OK, iter, repl, t2[0], g2); return; }
+            if (A.use64 && t2[0] > DIVERGE_GROWTH * tbest) { finish(MF_E_DIVERGED, iter, repl, t2[0], g2); return; }
+            tbest = fmin(tbest, t2[0]);
+            restart = t2[0] > 4.0 * g2;
+            ref2 = t2[0];
+            rho_new = t2[1];
+        }
+        if (iter >= A.maxit) { finish(MF_E_MAXIT, iter, repl, -1.0, g2); return; }
+        if (omega == 0.0 || rho == 0.0) { finish(MF_E_BREAKDOWN, iter, repl, -1.0, g2); return; }
+        const double beta = restart ? 0.0 : (rho_new / rho) * (alpha / omega);
+        restart = false;
+        for (size_t row = row0; row < row1; row += TPB) {
+            const double p = vr[row] + beta * (vp[row] - omega * vv[row]);
+            vp[row] = p;
+            publish(row, p);
+        }
+        rho = rho_new;
+        clk.lap(2);
+        if (!R.published(A, chunk)) return;
+        clk.lap(3);
+    }
+}
+
 // ------------------------------------------------------------ dispatch
 using KernelFn = void (*)(SolveArgs);
 
 struct KernelChoice {
     KernelFn fn = nullptr;
     int tpb = 0, rpt = 0;
+    int streamed = 0;  // 1: vectors in HBM (k_cg_st / k_bicgstab_st), rpt = 0
 };
 
+constexpr int ST_TPB = 512;
+// vectors of the streamed kernels: CG r, p, s; BiCGSTAB r, r^, p, s, v, t
+__host__ __device__ constexpr int stream_vectors(int method) { return method == MF_CG ? 3 : 6; }
+
 // 512 threads (8 waves per CU, 256 VGPRs per wave) hold the register-resident
-// vectors of up to 16 rows per thread without spilling.
+// vectors of up to 16 rows per thread without spilling.  Beyond that (or when
+// asked, stream = true) the streamed kernels keep the vectors in HBM.
 template <int B, int IW>
-KernelChoice pick_kernel(int method, long long rows_per_wg)
+KernelChoice pick_kernel(int method, long long rows_per_wg, bool stream)
 {
     KernelChoice k;
     const int q = (int)((rows_per_wg + 511) / 512);
-    if (method == MF_CG) {
+    if (stream) {
+    } else if (method == MF_CG) {
         if (q <= 1) k = {k_cg<B, IW, 512, 1>, 512, 1};
         else if (q <= 2) k = {k_cg<B, IW, 512, 2>, 512, 2};
         else if (q <= 4) k = {k_cg<B, IW, 512, 4>, 512, 4};
@@ -746,18 +1127,20 @@ KernelChoice pick_kernel(int method, long long rows_per_wg)
         else if (q <= 4) k = {k_bicgstab<B, IW, 512, 4>, 512, 4};
         else if (q <= 8) k = {k_bicgstab<B, IW, 512, 8>, 512, 8};
     }
+    if (!k.fn)
+        k = {method == MF_CG ? k_cg_st<B, IW, ST_TPB> : k_bicgstab_st<B, IW, ST_TPB>, ST_TPB, 0, 1};
     return k;
 }
 
 // tile sizes instantiated in their own translation units
-KernelChoice pick_b8(int iw, int method, long long rows_per_wg);   // solver_b8.hip
-KernelChoice pick_b32(int iw, int method, long long rows_per_wg);  // solver_b32.hip
+KernelChoice pick_b8(int iw, int method, long long rows_per_wg, bool stream);   // solver_b8.hip
+KernelChoice pick_b32(int iw, int method, long long rows_per_wg, bool stream);  // solver_b32.hip
 
-template <int B> KernelChoice pick_tile(int iw, int method, long long rows_per_wg)
+template <int B> KernelChoice pick_tile(int iw, int method, long long rows_per_wg, bool stream)
 {
-    return iw == 1   ? pick_kernel<B, 1>(method, rows_per_wg)
-           : iw == 2 ? pick_kernel<B, 2>(method, rows_per_wg)
-                     : pick_kernel<B, 4>(method, rows_per_wg);
+    return iw == 1   ? pick_kernel<B, 1>(method, rows_per_wg, stream)
+           : iw == 2 ? pick_kernel<B, 2>(method, rows_per_wg, stream)
+                     : pick_kernel<B, 4>(method, rows_per_wg, stream);
 }
 
 }  // namespace mf

@@ -110,19 +110,19 @@ __global__ void k_axpy(double *__restrict__ x, const double *__restrict__ d, lon
     if (i < n) x[i] += d[i];
 }
 
-static KernelChoice pick(int B, int iw, int method, long long rows_per_wg)
+static KernelChoice pick(int B, int iw, int method, long long rows_per_wg, bool stream)
 {
     switch (B) {
-    case 16: return pick_tile<16>(iw, method, rows_per_wg);
-    case 8: return pick_b8(iw, method, rows_per_wg);
-    case 32: return pick_b32(iw, method, rows_per_wg);
+    case 16: return pick_tile<16>(iw, method, rows_per_wg, stream);
+    case 8: return pick_b8(iw, method, rows_per_wg, stream);
+    case 32: return pick_b32(iw, method, rows_per_wg, stream);
     default: return KernelChoice{};
     }
 }
 
 void Workspace::release()
 {
-    void *ps[] = {g64, h64, q64, g32, slots, ctl, out, r_outer, d, tmp, red, dep_ptr, dep_idx, flags};
+    void *ps[] = {g64, h64, q64, g32, slots, ctl, out, r_outer, d, tmp, red, dep_ptr, dep_idx, flags, vec};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     *this = Workspace();
@@ -155,11 +155,27 @@ static int ensure_ws(mf_matrix m, int G)
     return MF_OK;
 }
 
+// Krylov vectors of the streamed kernels (nvec vectors of ncol rows), kept
+// with the workspace once allocated.
+static int ensure_vec(mf_matrix m, int nvec)
+{
+    Workspace &w = m->ws;
+    const size_t need = (size_t)nvec * w.ncol;
+    if (w.vec && w.vec_n >= need) return MF_OK;
+    if (w.vec) MF_HIP(hipFree(w.vec));
+    w.vec = nullptr;
+    w.vec_n = 0;
+    MF_HIP(hipMalloc((void **)&w.vec, sizeof(double) * need));
+    w.vec_n = need;
+    return MF_OK;
+}
+
 struct InnerResult {
     int status = MF_OK, iterations = 0, replacements = 0;
     double bnorm = 0, rt2 = 0, g2 = 0;
     double phase_us[5] = {0, 0, 0, 0, 0};  // CG, block 0: per iteration x4, passes total
     int sync_deps = -1;
+    int streamed = 0;
 };
 
 // Empties the sync words, clears the chunk epoch words (epoch 0 never occurs
@@ -258,22 +274,32 @@ void release_phase_buffer()
     d_phase = nullptr;
 }
 
-static int launch_inner(mf_matrix m, const Layout &L, const Layout *L64, int method, const double *d_b,
-                        double *d_x, double tol, const mf_opts &o, bool sync, InnerResult *res, int *grid_out,
-                        int *block_out, int *rpt_out)
+// Device buffers of one persistent launch: the published vectors, the sync
+// words and control words, the neighbour lists (dep_max < 0: none, every
+// publish is a grid barrier) and the streamed kernels' vectors.
+struct WsView {
+    double *g64 = nullptr, *h64 = nullptr, *q64 = nullptr;
+    float *g32 = nullptr;
+    unsigned long long *slots = nullptr;
+    int *ctl = nullptr;
+    double *out = nullptr;
+    const int *dep_ptr = nullptr, *dep_idx = nullptr;
+    unsigned *flags = nullptr;
+    int dep_max = -1;
+    double *vec = nullptr;
+    long long vstride = 0;
+    bool allow_profile = true;  // the phase clock allocates its buffer on first use
+};
+
+// One persistent launch on the grid (G workgroups of trw tile rows) with the
+// kernel kc over the buffers w; allocates nothing unless w.allow_profile and
+// a phase profile is asked for.
+static int launch_persistent(mf_matrix m, const Layout &L, const Layout *L64, int method, const double *d_b,
+                             double *d_x, double tol, const mf_opts &o, int G, int trw, const KernelChoice &kc,
+                             const WsView &w, bool sync, InnerResult *res, int *grid_out, int *block_out,
+                             int *rpt_out)
 {
     hipStream_t st = stream();
-    int G = 1, trw = 1;
-    solver_grid(m->tilem, m->B, o.grid, &G, &trw);
-    KernelChoice kc = pick(m->B, L.iw, method, (long long)trw * m->B);
-    if (!kc.fn)
-        return fail(MF_E_UNSUPPORTED,
-                    "system too large for register-resident vectors at this grid size");
-    int rc = ensure_ws(m, G);
-    if (rc) return rc;
-    rc = ensure_deps(m, G, trw);
-    if (rc) return rc;
-    Workspace &w = m->ws;
     SolveArgs A;
     A.L = L.view(m->d_tile_ptr);
     A.use64 = L64 && L64 != &L ? 1 : 0;
@@ -302,15 +328,21 @@ static int launch_inner(mf_matrix m, const Layout &L, const Layout *L64, int met
     A.flags = w.flags;
     // BiCGSTAB's syncs after publishing p and s carry no reduction: they can
     // be neighbour waits (CG's publish sync also reduces (r, r))
-    A.nbr = method == MF_BICGSTAB && o.sync != MF_SYNC_GLOBAL && w.dep_max <= 64 ? 1 : 0;
-    if (res) res->sync_deps = A.nbr ? w.dep_max : -1;
+    A.nbr = method == MF_BICGSTAB && o.sync != MF_SYNC_GLOBAL && w.flags && w.dep_max >= 0 && w.dep_max <= 64 ? 1 : 0;
+    A.vec = kc.streamed ? w.vec : nullptr;
+    A.vstride = w.vstride;
+    if (kc.streamed && !w.vec) return fail(MF_E_ARG, "streamed kernel without its vectors");
+    if (res) {
+        res->sync_deps = A.nbr ? w.dep_max : -1;
+        res->streamed = kc.streamed;
+    }
     static const bool phase_profile = std::getenv("MF_PHASE_PROFILE") != nullptr;
     const size_t phase_words = 16 + 8 * 1024;
-    const bool prof = phase_profile || o.profile;
+    const bool prof = w.allow_profile && (phase_profile || o.profile);
     if (prof && !d_phase) MF_HIP(hipMalloc((void **)&d_phase, phase_words * sizeof(unsigned long long)));
     A.phase = prof ? d_phase : nullptr;
     if (A.phase) MF_HIP(hipMemsetAsync(d_phase, 0, phase_words * sizeof(unsigned long long), st));
-    k_reset_ctl<<<1, 1024, 0, st>>>(w.slots, SLOT_BUFS * SLOT_WORDS * G, w.flags, 16 * G, w.ctl);
+    k_reset_ctl<<<1, 1024, 0, st>>>(w.slots, SLOT_BUFS * SLOT_WORDS * G, w.flags, w.flags ? 16 * G : 0, w.ctl);
     MF_HIP(hipGetLastError());
     void *args[] = {&A};
     // Cooperative launch guarantees that all G workgroups are resident.  Some
@@ -394,6 +426,43 @@ static int launch_inner(mf_matrix m, const Layout &L, const Layout *L64, int met
         }
     }
     return MF_OK;
+}
+
+// One persistent launch with the matrix's own workspace (allocated, or
+// grown, on first use for the grid and kernel).
+static int launch_inner(mf_matrix m, const Layout &L, const Layout *L64, int method, const double *d_b,
+                        double *d_x, double tol, const mf_opts &o, bool sync, InnerResult *res, int *grid_out,
+                        int *block_out, int *rpt_out)
+{
+    int G = 1, trw = 1;
+    solver_grid(m->tilem, m->B, o.grid, &G, &trw);
+    KernelChoice kc = pick(m->B, L.iw, method, (long long)trw * m->B, o.vectors == MF_VECTORS_HBM);
+    if (!kc.fn) return fail(MF_E_UNSUPPORTED, "no persistent kernel for this tile size and index form");
+    int rc = ensure_ws(m, G);
+    if (rc) return rc;
+    if (kc.streamed) {
+        rc = ensure_vec(m, stream_vectors(method));
+        if (rc) return rc;
+    }
+    rc = ensure_deps(m, G, trw);
+    if (rc) return rc;
+    const Workspace &ws = m->ws;
+    WsView w;
+    w.g64 = ws.g64;
+    w.h64 = ws.h64;
+    w.q64 = ws.q64;
+    w.g32 = ws.g32;
+    w.slots = ws.slots;
+    w.ctl = ws.ctl;
+    w.out = ws.out;
+    w.dep_ptr = ws.dep_ptr;
+    w.dep_idx = ws.dep_idx;
+    w.flags = ws.flags;
+    w.dep_max = ws.dep_max;
+    w.vec = ws.vec;
+    w.vstride = (long long)ws.ncol;
+    return launch_persistent(m, L, L64, method, d_b, d_x, tol, o, G, trw, kc, w, sync, res, grid_out, block_out,
+                             rpt_out);
 }
 
 // r = b - A x over layout L; returns ||r|| (FP64) via host sum of partials.
@@ -492,13 +561,28 @@ static int solve_direct(mf_matrix m, const double *d_b, double *d_x, const mf_op
             if (rc) return rc;
             iters += ir.iterations;
             repl += ir.replacements;
-            if (ir.status == MF_E_TIMEOUT || ir.status == MF_E_NONFINITE) { status = ir.status; break; }
+            if (ir.status == MF_E_TIMEOUT || ir.status == MF_E_NONFINITE || ir.status == MF_E_DIVERGED) {
+                status = ir.status;
+                break;
+            }
             k_axpy<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(d_x, w.d, (long long)n);
             rc = residual(m, m->L64, d_b, d_x, w.r_outer, &rnorm);
             if (rc) return rc;
         }
         if (rnorm <= o.tol * bnorm) status = MF_OK;
         relres = bnorm > 0 ? rnorm / bnorm : 0.0;
+    }
+    // lossy tiles that diverged, broke down, produced non-finite sums or ran
+    // out of iterations: the same solve on the FP64 layout
+    int fallback = 0;
+    if (!exact && (status == MF_E_DIVERGED || status == MF_E_BREAKDOWN || status == MF_E_NONFINITE ||
+                   status == MF_E_MAXIT)) {
+        rc = launch_inner(m, m->L64, nullptr, o.method, d_b, d_x, o.tol, o, true, &ir, &G, &TPB, &RPT);
+        if (rc) return rc;
+        status = ir.status;
+        iters += ir.iterations;
+        repl += ir.replacements;
+        fallback = 1;
     }
     MF_HIP(hipEventRecord(e1, st));
     MF_HIP(hipEventSynchronize(e1));
@@ -525,6 +609,8 @@ static int solve_direct(mf_matrix m, const double *d_b, double *d_x, const mf_op
     rep->rows_per_thread = RPT;
     for (int k = 0; k < 5; k++) rep->phase_us[k] = ir.phase_us[k];
     rep->sync_deps = ir.sync_deps;
+    rep->streamed = ir.streamed;
+    rep->fallback = fallback;
     return status == MF_E_TIMEOUT ? fail(MF_E_TIMEOUT, "grid barrier timeout") : MF_OK;
 }
 
@@ -557,31 +643,49 @@ int spmv_device(mf_matrix m, const Layout &L, const double *d_x, double *d_y, bo
 // ------------------------------------------------------- row reordering
 // Rows of very different lengths pad every slab to the tile row's longest
 // row, and equal tile-row chunks give the persistent workgroups unequal work.
-// The solvers then run on P A P^T: rows sorted by length (longest first) form
-// the tile rows, so a tile row's rows have similar lengths, and the tile rows
-// are dealt round robin to the chunks of the persistent grid, so every
-// workgroup gets a share of long and short rows.  The last tile row (the
-// shortest rows, possibly partial) keeps the last position.  CG and BiCGSTAB
-// on P A P^T with P b give P x; their iterates equal those on A in exact
+// The solvers then run on a reordered copy of A: rows sorted by length
+// (longest first) form the tile rows, so a tile row's rows have similar
+// lengths, and the tile rows are dealt round robin to the chunks of the
+// persistent grid, so every workgroup gets a share of long and short rows.
+// A row longer than SPLIT_ENTRIES is split over S = 2, 4, 8 or 16 (<= B)
+// lanes of its tile row, each holding a consecutive segment of it, so that
+// no lane walks more than about SPLIT_ENTRIES entries (random SPD with
+// Zipf row lengths: rows of ~2000 entries would otherwise be a chain of
+// ~500 dependent quads on one lane); the segments' products are summed
+// across the lanes (DevL::split).  The copy has n' = tilem' B rows: every
+// row of A once (at the first lane of its segment), plus rows without
+// entries (the other lanes of a split row, and the unfilled lanes of a tile
+// row), with b = 0 and x = 0 on them throughout.  CG and BiCGSTAB on the
+// copy with P b give P x; their iterates equal those on A in exact
 // arithmetic.
+constexpr int SPLIT_ENTRIES = 64;
+constexpr int SPLIT_PAD = 2048;  // zero split entries after tilem' (tile rows a kernel may look past)
+
 __global__ void k_permute_in(const int *__restrict__ perm, int n, const double *__restrict__ src,
                              double *__restrict__ dst)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = src[perm[i]];
+    if (i < n) dst[i] = perm[i] >= 0 ? src[perm[i]] : 0.0;
 }
 
 __global__ void k_permute_out(const int *__restrict__ perm, int n, const double *__restrict__ src,
                               double *__restrict__ dst)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[perm[i]] = src[i];
+    if (i < n && perm[i] >= 0) dst[perm[i]] = src[i];
 }
 
+// lanes per row for a row of len entries (a power of two, at most B)
+static int split_lanes(int len, int B)
+{
+    int S = 1;
+    while (S < B && (len + S - 1) / S > SPLIT_ENTRIES) S <<= 1;
+    return S;
+}
 
 static int build_reordered(mf_matrix m, int grid)
 {
-    const int n = m->rowA, B = m->B, tilem = m->tilem, nnz = m->nnz;
+    const int n = m->rowA, B = m->B, nnz = m->nnz;
     std::vector<int> rp(n + 1), ci(std::max(nnz, 1));
     std::vector<double> va(std::max(nnz, 1));
     MF_HIP(hipMemcpy(rp.data(), m->d_rowptr, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
@@ -589,59 +693,106 @@ static int build_reordered(mf_matrix m, int grid)
         MF_HIP(hipMemcpy(ci.data(), m->d_col, sizeof(int) * nnz, hipMemcpyDeviceToHost));
         MF_HIP(hipMemcpy(va.data(), m->d_val, sizeof(double) * nnz, hipMemcpyDeviceToHost));
     }
+    auto len = [&](int r) { return rp[r + 1] - rp[r]; };
     std::vector<int> order(n);
     for (int i = 0; i < n; i++) order[i] = i;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int a, int b) { return rp[a + 1] - rp[a] > rp[b + 1] - rp[b]; });
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return len(a) > len(b); });
+    // tile rows of the copy: consecutive rows (in length order) with the same
+    // lanes per row S, B / S rows each
+    struct TR { int first, count, S; };
+    std::vector<TR> trs;
+    for (int i = 0; i < n;) {
+        const int S = split_lanes(len(order[i]), B), per = B / S;
+        int c = 0;
+        while (c < per && i + c < n && split_lanes(len(order[i + c]), B) == S) c++;
+        trs.push_back({i, c, S});
+        i += c;
+    }
+    const int tilem2 = std::max(1, (int)trs.size());
+    if (trs.empty()) trs.push_back({0, 0, 1});
+    const long long n2l = (long long)tilem2 * B;
+    if (n2l > INT32_MAX / 2) return fail(MF_E_UNSUPPORTED, "reordered copy too large");
+    const int n2 = (int)n2l;
+    // tile rows dealt round robin to the chunks of the persistent grid
     int G = 1, trw = 1;
-    solver_grid(tilem, B, grid, &G, &trw);
-    std::vector<int> start(G), cap(G), fill(G, 0), pos(tilem);
+    solver_grid(tilem2, B, grid, &G, &trw);
+    std::vector<int> start(G), cap(G), fill(G, 0), pos(tilem2);
     for (int g = 0; g < G; g++) {
         start[g] = g * trw;
-        cap[g] = std::min(trw, tilem - g * trw);
+        cap[g] = std::max(0, std::min(trw, tilem2 - g * trw));
     }
-    cap[G - 1] -= 1;  // position tilem - 1 is the last tile row's
-    pos[tilem - 1] = tilem - 1;
-    for (int j = 0, g = 0; j < tilem - 1; j++) {
+    for (int j = 0, g = 0; j < tilem2; j++) {
         while (fill[g] >= cap[g]) g = (g + 1) % G;
         pos[j] = start[g] + fill[g]++;
         g = (g + 1) % G;
     }
-    std::vector<int> perm(n), inv(n);
-    for (int r = 0; r < n; r++) {
-        const int nw = pos[r / B] * B + r % B;
-        perm[nw] = order[r];
-        inv[order[r]] = nw;
+    // new row of every row of A; source row (or -1) of every row of the copy
+    std::vector<int> perm(n2, -1), inv(n);
+    std::vector<uint8_t> split(tilem2 + SPLIT_PAD, 0);
+    bool any_split = false;
+    int split_rows = 0;
+    for (int j = 0; j < tilem2; j++) {
+        const TR &t = trs[j];
+        split[pos[j]] = (uint8_t)t.S;
+        any_split = any_split || t.S > 1;
+        if (t.S > 1) split_rows += t.count;
+        for (int q = 0; q < t.count; q++) {
+            const int nw = pos[j] * B + q * t.S;
+            perm[nw] = order[t.first + q];
+            inv[order[t.first + q]] = nw;
+        }
     }
-    std::vector<int> rp2(n + 1, 0);
-    for (int i = 0; i < n; i++) rp2[i + 1] = rp2[i] + (rp[perm[i] + 1] - rp[perm[i]]);
+    // CSR of the copy: a split row's entries go in S consecutive segments to
+    // its S lanes (columns renumbered)
+    std::vector<int> rp2(n2 + 1, 0);
+    for (int j = 0; j < tilem2; j++) {
+        const TR &t = trs[j];
+        for (int q = 0; q < t.count; q++) {
+            const int o = order[t.first + q], L = len(o), seg = (L + t.S - 1) / t.S;
+            for (int k = 0; k < t.S; k++)
+                rp2[pos[j] * B + q * t.S + k + 1] = std::max(0, std::min(seg, L - k * seg));
+        }
+    }
+    for (int i = 0; i < n2; i++) rp2[i + 1] += rp2[i];
     std::vector<int> ci2(std::max(nnz, 1));
     std::vector<double> va2(std::max(nnz, 1));
-#pragma omp parallel for schedule(dynamic, 1024)
-    for (int i = 0; i < n; i++) {
-        const int o = perm[i];
-        for (int k = rp[o], d = rp2[i]; k < rp[o + 1]; k++, d++) {
-            ci2[d] = inv[ci[k]];
-            va2[d] = va[k];
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int j = 0; j < tilem2; j++) {
+        const TR &t = trs[j];
+        for (int q = 0; q < t.count; q++) {
+            const int o = order[t.first + q];
+            // the S segments of row o are consecutive in the copy's CSR
+            for (int k = rp[o], d = rp2[pos[j] * B + q * t.S]; k < rp[o + 1]; k++, d++) {
+                ci2[d] = inv[ci[k]];
+                va2[d] = va[k];
+            }
         }
     }
     if (m->child) mf_matrix_destroy(m->child);
     m->child = nullptr;
     mf_matrix c = nullptr;
-    int rc = mf_matrix_create(n, n, nnz, rp2.data(), ci2.data(), va2.data(), nullptr, B, &c);
+    int rc = mf_matrix_create(n2, n2, nnz, rp2.data(), ci2.data(), va2.data(), nullptr, B, &c);
     if (rc) return rc;
     c->want_iw = m->want_iw;
     m->child = c;
     m->child_grid = grid;
-    const size_t npad = (size_t)tilem * B;
-    if (!m->d_perm) MF_HIP(hipMalloc((void **)&m->d_perm, sizeof(int) * (size_t)n));
-    if (!m->d_pb) MF_HIP(hipMalloc((void **)&m->d_pb, sizeof(double) * npad));
-    if (!m->d_px) MF_HIP(hipMalloc((void **)&m->d_px, sizeof(double) * npad));
-    // on the library stream, ordered before the solves that read them
+    m->child_split_rows = split_rows;
     hipStream_t st = stream();
-    MF_HIP(hipMemcpyAsync(m->d_perm, perm.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice, st));
-    MF_HIP(hipMemsetAsync(m->d_pb, 0, sizeof(double) * npad, st));
-    MF_HIP(hipMemsetAsync(m->d_px, 0, sizeof(double) * npad, st));
+    if (any_split) {
+        MF_HIP(hipMalloc((void **)&c->d_split, split.size()));
+        MF_HIP(hipMemcpyAsync(c->d_split, split.data(), split.size(), hipMemcpyHostToDevice, st));
+    }
+    for (void *p : {(void *)m->d_perm, (void *)m->d_pb, (void *)m->d_px})
+        if (p) MF_HIP(hipFree(p));
+    m->d_perm = nullptr;
+    m->d_pb = m->d_px = nullptr;
+    MF_HIP(hipMalloc((void **)&m->d_perm, sizeof(int) * (size_t)n2));
+    MF_HIP(hipMalloc((void **)&m->d_pb, sizeof(double) * (size_t)n2));
+    MF_HIP(hipMalloc((void **)&m->d_px, sizeof(double) * (size_t)n2));
+    // on the library stream, ordered before the solves that read them
+    MF_HIP(hipMemcpyAsync(m->d_perm, perm.data(), sizeof(int) * (size_t)n2, hipMemcpyHostToDevice, st));
+    MF_HIP(hipMemsetAsync(m->d_pb, 0, sizeof(double) * (size_t)n2, st));
+    MF_HIP(hipMemsetAsync(m->d_px, 0, sizeof(double) * (size_t)n2, st));
     MF_HIP(hipStreamSynchronize(st));
     return MF_OK;
 }
@@ -675,7 +826,7 @@ int solve_device(mf_matrix m, const double *d_b, double *d_x, const mf_opts &o, 
     }
     const double ms_reorder = now_ms() - t0;
     hipStream_t st = stream();
-    const int n = m->rowA;
+    const int n = m->child->rowA;  // rows of the copy (every row of A, split segments, padding)
     const unsigned nb = (unsigned)((n + 255) / 256);
     hipEvent_t e[4];
     for (hipEvent_t &ev : e) MF_HIP(hipEventCreate(&ev));
@@ -700,6 +851,7 @@ int solve_device(mf_matrix m, const double *d_b, double *d_x, const mf_opts &o, 
     rep->ms_setup += ms_reorder;
     rep->reordered = 1;
     rep->pad_factor = pad;
+    rep->split_rows = m->child_split_rows;
     return rc;
 }
 

@@ -4,9 +4,9 @@
 
 namespace mf {
 
-KernelChoice pick_b32(int iw, int method, long long rows_per_wg)
+KernelChoice pick_b32(int iw, int method, long long rows_per_wg, bool stream)
 {
-    return pick_tile<32>(iw, method, rows_per_wg);
+    return pick_tile<32>(iw, method, rows_per_wg, stream);
 }
 
 }  // namespace mf

@@ -826,6 +826,7 @@ int build_layout(mf_matrix m, Layout &L, double tau, int allow_mask, int want_iw
     case 32: rc = build_layout_B<32>(m, L, tau, allow_mask, want_iw); break;
     default: return fail(MF_E_ARG, "tile size");
     }
+    L.split = m->d_split;
     m->ms_layout += now_ms() - t0;
     return rc;
 }

@@ -143,8 +143,10 @@ def _cg_case(gpu, orc, rp, ci, va, b, B=16, **opts):
     return rep
 
 
-def _cg_case_loose(gpu, orc, rp, ci, va, b, B=16, **opts):
+def _cg_case_loose(gpu, orc, rp, ci, va, b, B=16, xtol=1e-6, **opts):
     # lossy tiles change the iterations; the answer must still be the FP64 one
+    # (within xtol: two solutions at relres 1e-10 differ by up to about
+    # cond(A) 1e-10)
     A = gpu.Matrix(rp, ci, va, B=B)
     x, rep = A.solve(b, method=gpu.MF_CG, tol=TOL, **opts)
     xo, orep = orc.cg(rp, ci, va, b, TOL, 20000)
@@ -152,7 +154,7 @@ def _cg_case_loose(gpu, orc, rp, ci, va, b, B=16, **opts):
           f"relres={rep['relres_true']:.2e}", flush=True)
     assert rep["status"] == 0
     assert rep["relres_true"] <= TOL and _true_relres(rp, ci, va, x, b) <= 1.5 * TOL
-    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-6
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < xtol
     return rep
 
 
@@ -315,6 +317,8 @@ def test_gpu_cg_reordered_rows(gpu, orc):
           f"({rep['ms_solve']:.2f} ms) natural it={rep0['iterations']} ({rep0['ms_solve']:.2f} ms) "
           f"oracle it={orep.iterations}", flush=True)
     assert rep["reordered"] == 1 and rep0["reordered"] == 0 and rep["pad_factor"] > 2.0
+    # rows above 64 entries run split over several lanes of their tile row
+    assert rep["split_rows"] == int(np.sum(np.diff(rp) > 64)) > 0
     for xx, rr in ((x, rep), (x0, rep0)):
         assert rr["status"] == 0 and rr["relres_true"] <= TOL and _true_relres(rp, ci, va, xx, b) <= 1.5 * TOL
         assert _iters_close(rr["iterations"], orep.iterations)
@@ -322,6 +326,25 @@ def test_gpu_cg_reordered_rows(gpu, orc):
     # mixed tiles on the reordered copy
     xm, repm = A.solve(b, method=gpu.MF_CG, tol=TOL, precision=gpu.MF_MIXED, tau=2.0 ** -11, rr_factor=1e-2)
     assert repm["reordered"] == 1 and repm["status"] == 0 and _true_relres(rp, ci, va, xm, b) <= 1.5 * TOL
+    A.close()
+
+
+def test_gpu_bicgstab_reordered_split_rows(gpu, orc):
+    """BiCGSTAB on the reordered copy with long rows split over lanes (both
+    of its SpMVs per iteration and the true-residual pass use the split
+    sums), against the oracle's BiCGSTAB on the original matrix."""
+    rp, ci, va = gpu.gen_random_spd(20000, seed=5)
+    n = len(rp) - 1
+    b = orc.csr_spmv(rp, ci, va, gpu.gen_uniform(13, n, -1, 1))
+    A = gpu.Matrix(rp, ci, va)
+    x, rep = A.solve(b, method=gpu.MF_BICGSTAB, tol=TOL)
+    xo, orep = orc.bicgstab(rp, ci, va, b, TOL, 20000)
+    print(f"bicgstab reordered={rep['reordered']} split_rows={rep['split_rows']} it={rep['iterations']} "
+          f"oracle it={orep.iterations}", flush=True)
+    assert rep["reordered"] == 1 and rep["split_rows"] > 0
+    assert rep["status"] == 0 and rep["relres_true"] <= TOL and _true_relres(rp, ci, va, x, b) <= 1.5 * TOL
+    assert _iters_close(rep["iterations"], orep.iterations)
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-6
     A.close()
 
 
@@ -499,10 +522,16 @@ def test_gpu_one_workgroup_mixed(gpu, orc, method, tau):
     """One-workgroup solves with per-tile classes: the FP32 copy of the
     gathered vector sits in LDS beside the FP64 one, and with lossy tiles the
     true-residual passes read the FP64 layout.  The default grid puts these
-    systems on one workgroup; both solves reach 1e-10 and the oracle's x."""
+    systems on one workgroup; both solves reach 1e-10 and the oracle's x.
+    1138_bus (condition number 8.6e6) changes its iteration count under FP32
+    accumulation, so only the answer is compared; its FP16-rounded operator
+    has a negative eigenvalue (-0.30), so the lossy solve must detect the
+    divergence and finish on the FP64 layout (fallback)."""
     if method == "cg":
         g = load_golden("1138_bus")
         rp, ci, va = g["rowptr"], g["col"], g["val"]
+    elif tau == 0.0:  # a nonsymmetric-solver case with lossless low-precision tiles
+        rp, ci, va = gpu.gen_poisson3d(12, 12, 12)
     else:
         rp, ci, va = gpu.gen_convdiff3d(10, 10.0, 1.0, 0)
     n = len(rp) - 1
@@ -512,11 +541,13 @@ def test_gpu_one_workgroup_mixed(gpu, orc, method, tau):
     print(f"case {method} n={n} tau={tau} codes={np.bincount(codes, minlength=5).tolist()}", flush=True)
     assert np.bincount(codes, minlength=5)[1:].sum() > 0  # low-precision tiles present
     opts = dict(precision=gpu.MF_MIXED, tau=tau, rr_factor=1e-2, maxit=40000)
-    case = (_cg_case_loose if tau else _cg_case) if method == "cg" else _bicg_case
-    extra = {} if method == "cg" else {"check_iters": tau == 0.0}
+    case = _cg_case_loose if method == "cg" else _bicg_case
+    # 1138_bus: cond 8.6e6, so solutions at relres 1e-10 agree to ~1e-4 only
+    extra = {"xtol": 1e-4} if method == "cg" else {"check_iters": tau == 0.0}
     for grid in (0, 1):
         rep = case(gpu, orc, rp, ci, va, b, grid=grid, **opts, **extra)
         assert rep["grid"] == 1
+        assert rep["fallback"] == (1 if method == "cg" and tau > 0 else 0)
 
 
 @pytest.mark.parametrize("B", [8, 32])

@@ -1,0 +1,130 @@
+"""Streamed persistent solvers (Krylov vectors in HBM) on an MI355X.
+
+Systems whose rows per workgroup exceed what registers and LDS hold (CG:
+more than 16 rows per thread of 512 threads on every CU, i.e. beyond
+2,097,152 rows at B = 16 on 256 CUs; BiCGSTAB: beyond 8 rows per thread)
+run the streamed kernels.  On the same grid they own the same rows in the
+same order as the resident kernels, so opts.vectors = MF_VECTORS_HBM must give
+bit-identical iterates; beyond the resident limits the solves are checked
+against the oracle as every other solve: true relative residual <= 1e-10 on
+the GPU and on the CPU, iterations within 5 % (+2) of the oracle's FP64
+method, x within 1e-6 of the oracle's."""
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from fixtures import load_golden
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+THREADS = max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def _true_relres(rp, ci, va, x, b):
+    A = sp.csr_matrix((va, ci, rp), shape=(len(rp) - 1, len(x)))
+    return np.linalg.norm(b - A @ x) / np.linalg.norm(b)
+
+
+def _cases(gpu):
+    g = load_golden("1138_bus")
+    return {
+        "poisson3d_32": (gpu.MF_CG, gpu.gen_poisson3d(32, 32, 32)),
+        "laplace2d_256": (gpu.MF_CG, gpu.gen_laplace2d(256, 256)),
+        "1138_bus": (gpu.MF_CG, (g["rowptr"], g["col"], g["val"])),
+        "convdiff27_24": (gpu.MF_BICGSTAB, gpu.gen_convdiff3d(24, 10.0, 1.0, 0)),
+        "convdiff27_10": (gpu.MF_BICGSTAB, gpu.gen_convdiff3d(10, 10.0, 1.0, 0)),
+    }
+
+
+@pytest.mark.parametrize("name", ["poisson3d_32", "laplace2d_256", "1138_bus", "convdiff27_24", "convdiff27_10"])
+@pytest.mark.parametrize("prec", ["fp64", "mixed_lossless", "mixed_lossy"])
+def test_gpu_streamed_matches_resident(gpu, orc, name, prec):
+    """Streamed and resident kernels on the same grid: same iterations, same
+    replacements, bit-identical x (one-workgroup systems included, whose
+    resident kernel gathers from LDS)."""
+    method, (rp, ci, va) = _cases(gpu)[name]
+    n = len(rp) - 1
+    b = orc.csr_spmv(rp, ci, va, gpu.gen_uniform(11, n, -1.0, 1.0))
+    opts = {"fp64": dict(precision=gpu.MF_FP64),
+            "mixed_lossless": dict(precision=gpu.MF_MIXED, tau=0.0, rr_factor=1e-2),
+            "mixed_lossy": dict(precision=gpu.MF_MIXED, tau=2.0 ** -11, rr_factor=1e-2)}[prec]
+    A = gpu.Matrix(rp, ci, va)
+    out = {}
+    for vec in (gpu.MF_VECTORS_AUTO, gpu.MF_VECTORS_HBM):
+        print(f"case {name} {prec} vectors={vec} n={n}", flush=True)
+        x, rep = A.solve(b, method=method, tol=TOL, maxit=40000, vectors=vec, **opts)
+        print(f"  it={rep['iterations']} repl={rep['replacements']} grid={rep['grid']} streamed={rep['streamed']} "
+              f"relres={rep['relres_true']:.2e}", flush=True)
+        assert rep["status"] == 0 and rep["relres_true"] <= TOL
+        assert rep["streamed"] == (1 if vec == gpu.MF_VECTORS_HBM else 0)
+        out[vec] = (x, rep)
+    (xa, ra), (xh, rh) = out[gpu.MF_VECTORS_AUTO], out[gpu.MF_VECTORS_HBM]
+    assert ra["grid"] == rh["grid"]
+    assert ra["iterations"] == rh["iterations"] and ra["replacements"] == rh["replacements"]
+    assert ra["fallback"] == rh["fallback"]
+    assert np.array_equal(xa, xh)
+    assert _true_relres(rp, ci, va, xh, b) <= 1.5 * TOL
+    A.close()
+
+
+@pytest.mark.parametrize("B", [8, 32])
+def test_gpu_streamed_tile_sizes(gpu, orc, B):
+    """8- and 32-row tiles: the streamed kernels equal the resident ones."""
+    for method, (rp, ci, va) in ((gpu.MF_CG, gpu.gen_poisson3d(24, 24, 24)),
+                                 (gpu.MF_BICGSTAB, gpu.gen_convdiff3d(20, 10.0, 1.0, 0))):
+        n = len(rp) - 1
+        b = orc.csr_spmv(rp, ci, va, gpu.gen_uniform(12, n, -1.0, 1.0))
+        A = gpu.Matrix(rp, ci, va, B=B)
+        print(f"case B={B} method={method} n={n}", flush=True)
+        xa, ra = A.solve(b, method=method, tol=TOL, precision=gpu.MF_MIXED, rr_factor=1e-2)
+        xh, rh = A.solve(b, method=method, tol=TOL, precision=gpu.MF_MIXED, rr_factor=1e-2,
+                         vectors=gpu.MF_VECTORS_HBM)
+        assert ra["status"] == 0 and rh["status"] == 0 and rh["streamed"] == 1
+        assert ra["iterations"] == rh["iterations"] and np.array_equal(xa, xh)
+        A.close()
+
+
+def test_gpu_cg_beyond_resident_rows(gpu, orc):
+    """3D 7-point Poisson 130^3 (2,197,000 rows, above the 2,097,152 rows the
+    resident CG holds on 256 CUs at B = 16): the solver streams the vectors,
+    FP64 and lossless mixed, and meets the oracle."""
+    rp, ci, va = gpu.gen_poisson3d(130, 130, 130)
+    n = len(rp) - 1
+    xs = gpu.gen_uniform(0, n, -1.0, 1.0)
+    b = orc.csr_spmv(rp, ci, va, xs)
+    xo, orep = orc.cg(rp, ci, va, b, TOL, 20000, nthreads=THREADS)
+    A = gpu.Matrix(rp, ci, va)
+    for prec in (gpu.MF_FP64, gpu.MF_MIXED):
+        print(f"case poisson3d_130 cg precision={prec} n={n}", flush=True)
+        x, rep = A.solve(b, method=gpu.MF_CG, tol=TOL, precision=prec, tau=0.0, rr_factor=1e-4)
+        print(f"  it={rep['iterations']} oracle={orep.iterations} streamed={rep['streamed']} grid={rep['grid']} "
+              f"ms={rep['ms_solve']:.2f} relres={rep['relres_true']:.2e}", flush=True)
+        assert rep["status"] == 0 and rep["streamed"] == 1
+        assert rep["relres_true"] <= TOL and _true_relres(rp, ci, va, x, b) <= 1.5 * TOL
+        assert abs(rep["iterations"] - orep.iterations) <= 0.05 * orep.iterations + 2
+        assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-6
+    A.close()
+
+
+def test_gpu_bicgstab_beyond_resident_rows(gpu, orc):
+    """27-point convection-diffusion 102^3 (1,061,208 rows, above the
+    1,048,576 rows the resident BiCGSTAB holds on 256 CUs at B = 16),
+    FP64: streamed vectors, against the oracle's BiCGSTAB."""
+    rp, ci, va = gpu.gen_convdiff3d(102, 10.0, 1.0, 0)
+    n = len(rp) - 1
+    xs = gpu.gen_uniform(0, n, -1.0, 1.0)
+    b = orc.csr_spmv(rp, ci, va, xs)
+    xo, orep = orc.bicgstab(rp, ci, va, b, TOL, 20000, nthreads=THREADS)
+    A = gpu.Matrix(rp, ci, va)
+    print(f"case convdiff27_102 bicgstab n={n}", flush=True)
+    x, rep = A.solve(b, method=gpu.MF_BICGSTAB, tol=TOL)
+    print(f"  it={rep['iterations']} oracle={orep.iterations} streamed={rep['streamed']} grid={rep['grid']} "
+          f"ms={rep['ms_solve']:.2f} relres={rep['relres_true']:.2e}", flush=True)
+    assert rep["status"] == 0 and rep["streamed"] == 1
+    assert rep["relres_true"] <= TOL and _true_relres(rp, ci, va, x, b) <= 1.5 * TOL
+    assert abs(rep["iterations"] - orep.iterations) <= 0.05 * orep.iterations + 2
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-6
+    A.close()

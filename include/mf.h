@@ -274,6 +274,26 @@ int mf_solve(mf_matrix m, const double *b, double *x, const mf_opts *o, mf_repor
 int mf_solve_device(mf_matrix m, const double *d_b, double *d_x, const mf_opts *o,
                     mf_report *rep);
 
+/* mf_solve_device without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate (code under graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The launch's
+ * buffers (sync and control words, published vector, true-residual slots)
+ * sit in one static device array per GPU of about 56.4 MiB, held until the
+ * process ends, looked up by mf_init.  Limits: at most 2,097,152 padded
+ * rows (tilem*B) in the system solved, at most 1024 workgroups, Krylov
+ * vectors resident in registers/LDS (CG up to 2,097,152 rows and BiCGSTAB up
+ * to 1,048,576 on 256 CUs at B = 16; MF_E_UNSUPPORTED beyond), no outer
+ * refinement (ir_max_outer = 0), BiCGSTAB publishes through grid barriers
+ * (the same iterates as the neighbour waits), no phase profile.  What the
+ * solve reads must exist (else MF_E_ARG): the FP64 layout and, for
+ * MF_MIXED, the mixed layout of opts->tau/allow_mask (mf_matrix_layout, or
+ * an earlier mf_solve*), and, where opts->reorder picks the row-reordered
+ * copy as mf_solve_device does, that copy (an earlier mf_solve_device with
+ * the same opts->grid builds it).  relres_true is the kernel's final FP64
+ * true-residual pass.  Calls that overlap on one GPU race on its array. */
+int mf_solve_device_noalloc(mf_matrix m, const double *d_b, double *d_x, const mf_opts *o,
+                            mf_report *rep);
+
 /* Vendor-library baseline on the same matrix: CG or BiCGSTAB (opts->method)
  * in FP64 as the reference's hipSPARSE drivers run it
  * (Performance-Comparison/hipSPARSE_CG.cu:182, hipSPARSE_BiCGSTAB.cu:282):

@@ -127,7 +127,7 @@ EXPORTS = (
     "Tile_create", "Tile_destroy", "cg_solve_inc", "bicgstab", "mf_status_str", "mf_last_error",
     "mf_init", "mf_finalize", "mf_matrix_create", "mf_matrix_destroy", "mf_matrix_export", "mf_ref_tiles_free",
     "mf_matrix_encode_cols", "mf_matrix_classes", "mf_matrix_get_info", "mf_opts_default",
-    "mf_solve", "mf_solve_device", "mf_vendor_solve", "mf_vendor_solve_device", "mf_matrix_layout", "mf_matrix_set_index_width", "mf_spmv", "mf_spmv_bench",
+    "mf_solve", "mf_solve_device", "mf_solve_device_noalloc", "mf_vendor_solve", "mf_vendor_solve_device", "mf_matrix_layout", "mf_matrix_set_index_width", "mf_spmv", "mf_spmv_bench",
     "mf_solve_batched", "mf_mc_classify", "mf_memory_report", "mf_compact_rows", "mf_partition_build", "mf_free",
     "mf_gen_laplace2d", "mf_gen_poisson3d", "mf_gen_convdiff3d", "mf_gen_random_spd",
     "mf_gen_stencil_lognormal", "mf_gen_uniform", "mf_gen_normal",
@@ -173,6 +173,7 @@ def lib():
     L.mf_opts_default.argtypes = [C.POINTER(Opts)]
     L.mf_solve.argtypes = [C.c_void_p, _d, _d, C.POINTER(Opts), C.POINTER(Report)]
     L.mf_solve_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(Opts), C.POINTER(Report)]
+    L.mf_solve_device_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(Opts), C.POINTER(Report)]
     L.mf_mc_classify.argtypes = [C.c_void_p, C.POINTER(C.c_longlong)]
     L.mf_mc_classify.restype = C.c_double
     L.mf_compact_rows.argtypes = [C.POINTER(RefTiles), C.POINTER(Compact)]

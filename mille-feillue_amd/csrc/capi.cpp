@@ -18,6 +18,9 @@
 namespace mf {
 
 int solve_device(mf_matrix m, const double *d_b, double *d_x, const mf_opts &o, mf_report *rep);
+int noalloc_setup(int dev);  // solver.hip
+void noalloc_release();      // solver.hip
+int solve_noalloc(mf_matrix m, int dev, const double *d_b, double *d_x, const mf_opts &o, mf_report *rep);
 int spmv_device(mf_matrix m, const Layout &L, const double *d_x, double *d_y, bool convert);
 
 int hip_fail(hipError_t e, const char *what)
@@ -69,6 +72,9 @@ static int init_device(int dev)
     g_cus = p.multiProcessorCount;
     if (g_stream) (void)hipStreamDestroy(g_stream);
     MF_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    // the static buffers of the allocation-free solve on this device
+    const int rc = noalloc_setup(dev);
+    if (rc) return rc;
     g_device = dev;
     return MF_OK;
 }
@@ -222,6 +228,7 @@ void mf_finalize(void)
     if (g_device >= 0) {
         (void)hipSetDevice(g_device);
         mf::release_phase_buffer();
+        mf::noalloc_release();
         if (g_stream) (void)hipStreamDestroy(g_stream);
         g_stream = nullptr;
         g_device = -1;
@@ -444,6 +451,15 @@ int mf_solve_device(mf_matrix m, const double *d_b, double *d_x, const mf_opts *
     clear_error();
     if (!m || !d_b || !d_x || !o || !rep) return fail(MF_E_ARG, "null");
     return solve_device(m, d_b, d_x, *o, rep);
+}
+
+int mf_solve_device_noalloc(mf_matrix m, const double *d_b, double *d_x, const mf_opts *o, mf_report *rep)
+{
+    clear_error();
+    if (!m || !d_b || !d_x || !o || !rep) return fail(MF_E_ARG, "null");
+    const int rc = ensure_device();
+    if (rc) return rc;
+    return mf::solve_noalloc(m, g_device, d_b, d_x, *o, rep);
 }
 
 int mf_vendor_solve_device(mf_matrix m, const double *d_b, double *d_x, const mf_opts *o, mf_report *rep)

@@ -614,6 +614,146 @@ static int solve_direct(mf_matrix m, const double *d_b, double *d_x, const mf_op
     return status == MF_E_TIMEOUT ? fail(MF_E_TIMEOUT, "grid barrier timeout") : MF_OK;
 }
 
+// ------------------------------------------------ allocation-free solves
+// For callers that cannot allocate device memory (code under graph capture,
+// serving loops, harnesses that forbid it): the buffers of one persistent
+// launch -- sync words, control words, the published vector (FP64 and FP32)
+// and the true-residual pass's two publishing slots -- live in one static
+// device array per GPU, held until the process ends.  It is sized for
+// NA_MAX_ROWS padded rows and NA_MAX_G workgroups:
+//   slots  SLOT_BUFS x SLOT_WORDS x NA_MAX_G words of 8 bytes (196,608 B)
+//   ctl    8 ints, out 4 doubles (256 B each)
+//   g64, h64, q64  8 B per row; g32  4 B per row
+// = 58,917,376 + 197,120 bytes (about 56.4 MiB).
+// Every call on a GPU uses that GPU's array and leaves its contents; the
+// sync and control words are cleared by a fill kernel at the start of each
+// call, and every entry of the vector buffers is written in a launch before
+// it is read.  Calls that overlap on one GPU race on the array.
+constexpr size_t NA_MAX_ROWS = size_t(1) << 21;
+constexpr int NA_MAX_G = 1024;
+constexpr size_t NA_OFF_CTL = (size_t)SLOT_BUFS * SLOT_WORDS * NA_MAX_G * 8;
+constexpr size_t NA_OFF_OUT = NA_OFF_CTL + 256;
+constexpr size_t NA_OFF_G64 = NA_OFF_OUT + 256;
+constexpr size_t NA_OFF_H64 = NA_OFF_G64 + 8 * NA_MAX_ROWS;
+constexpr size_t NA_OFF_Q64 = NA_OFF_H64 + 8 * NA_MAX_ROWS;
+constexpr size_t NA_OFF_G32 = NA_OFF_Q64 + 8 * NA_MAX_ROWS;
+constexpr size_t NA_BYTES = NA_OFF_G32 + 4 * NA_MAX_ROWS;
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+constexpr int NA_MAX_DEV = 64;
+static void *g_scratch[NA_MAX_DEV];          // device address of scratch, per GPU
+static hipEvent_t g_na_ev[NA_MAX_DEV][2];    // timing events of the allocation-free solves
+
+// Once per device, with that device current (the library's device set-up):
+// the device address of that GPU's copy of scratch, and two timing events.
+int noalloc_setup(int dev)
+{
+    if (dev < 0 || dev >= NA_MAX_DEV) return fail(MF_E_NODEV, "device index beyond the allocation-free table");
+    if (g_scratch[dev]) return MF_OK;
+    void *p = nullptr;
+    MF_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(MF_E_HIP, "no device address for the static solve buffers");
+    for (hipEvent_t &e : g_na_ev[dev]) MF_HIP(hipEventCreate(&e));
+    g_scratch[dev] = p;
+    return MF_OK;
+}
+
+// At library finalisation: the events go, the array stays (the next set-up
+// looks its address up again).
+void noalloc_release()
+{
+    for (int d = 0; d < NA_MAX_DEV; d++)
+        if (g_scratch[d]) {
+            for (hipEvent_t &e : g_na_ev[d]) (void)hipEventDestroy(e);
+            g_scratch[d] = nullptr;
+        }
+}
+
+// The allocation-free solve of m in its own row order (solve_noalloc below
+// picks m or its reordered copy); rep gets everything but the timing.
+static int noalloc_core(mf_matrix m, int dev, const double *d_b, double *d_x, const mf_opts &o, mf_report *rep)
+{
+    hipStream_t st = stream();
+    if (m->rowA != m->colA) return fail(MF_E_ARG, "solver needs a square matrix");
+    const size_t ncol = (size_t)std::max(m->tilem, m->tilen) * m->B;
+    if (ncol > NA_MAX_ROWS)
+        return fail(MF_E_ARG, "allocation-free solve: " + std::to_string(ncol) + " padded rows exceed the limit of " +
+                                  std::to_string(NA_MAX_ROWS));
+    if (o.method != MF_CG && o.method != MF_BICGSTAB) return fail(MF_E_ARG, "method");
+    if (o.ir_max_outer > 0) return fail(MF_E_UNSUPPORTED, "allocation-free solve: no outer refinement loop");
+    const bool mixed = o.precision == MF_MIXED;
+    const int want = o.index_width ? o.index_width : m->want_iw;
+    if (!m->L64.built || m->L64.want_iw != want)
+        return fail(MF_E_ARG, "allocation-free solve: FP64 layout not built (mf_matrix_layout or a solve first)");
+    if (mixed && (!m->Lmix.built || m->Lmix.tau != o.tau || m->Lmix.allow_mask != o.allow_mask ||
+                  m->Lmix.want_iw != want))
+        return fail(MF_E_ARG, "allocation-free solve: mixed layout for these tau/allow_mask not built "
+                              "(mf_matrix_layout first)");
+    const Layout &L = mixed ? m->Lmix : m->L64;
+    const bool exact = !mixed || o.tau == 0.0;
+    int G = 1, trw = 1;
+    solver_grid(m->tilem, m->B, o.grid, &G, &trw);
+    if (G > NA_MAX_G) return fail(MF_E_ARG, "allocation-free solve: more than 1024 workgroups");
+    const KernelChoice kc = pick(m->B, L.iw, o.method, (long long)trw * m->B, false);
+    if (!kc.fn || kc.streamed)
+        return fail(MF_E_UNSUPPORTED, "allocation-free solve: system beyond register/LDS residency at this grid");
+    unsigned char *base = static_cast<unsigned char *>(g_scratch[dev]);
+    WsView w;
+    w.slots = reinterpret_cast<unsigned long long *>(base);
+    w.ctl = reinterpret_cast<int *>(base + NA_OFF_CTL);
+    w.out = reinterpret_cast<double *>(base + NA_OFF_OUT);
+    w.g64 = reinterpret_cast<double *>(base + NA_OFF_G64);
+    w.h64 = reinterpret_cast<double *>(base + NA_OFF_H64);
+    w.q64 = reinterpret_cast<double *>(base + NA_OFF_Q64);
+    w.g32 = reinterpret_cast<float *>(base + NA_OFF_G32);
+    w.allow_profile = false;  // no neighbour lists (grid barriers), no phase clock
+    InnerResult ir;
+    int TPB = 0, RPT = 0, Gl = 0;
+    int ctl[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double out[4] = {0.0, 0.0, 0.0, 0.0};
+    MF_HIP(hipEventRecord(g_na_ev[dev][0], st));
+    // one launch on Lr (true-residual passes on L64r when given); control
+    // words back to the host
+    auto run = [&](const Layout &Lr, const Layout *L64r, const KernelChoice &k) -> int {
+        int rc = launch_persistent(m, Lr, L64r, o.method, d_b, d_x, o.tol, o, G, trw, k, w, false, &ir, &Gl, &TPB,
+                                   &RPT);
+        if (rc) return rc;
+        MF_HIP(hipMemcpyAsync(ctl, w.ctl, sizeof(ctl), hipMemcpyDeviceToHost, st));
+        MF_HIP(hipMemcpyAsync(out, w.out, sizeof(out), hipMemcpyDeviceToHost, st));
+        MF_HIP(hipStreamSynchronize(st));
+        return MF_OK;
+    };
+    int rc = run(L, exact ? nullptr : &m->L64, kc);
+    if (rc) return rc;
+    int status = ctl[0] ? MF_E_TIMEOUT : ctl[1], iters = ctl[2], repl = ctl[3];
+    // lossy tiles that failed: the same solve on the FP64 layout, as mf_solve_device
+    if (!exact && (status == MF_E_DIVERGED || status == MF_E_BREAKDOWN || status == MF_E_NONFINITE ||
+                   status == MF_E_MAXIT)) {
+        const KernelChoice k64 = pick(m->B, m->L64.iw, o.method, (long long)trw * m->B, false);
+        if (!k64.fn || k64.streamed) return fail(MF_E_UNSUPPORTED, "allocation-free solve: no FP64 kernel");
+        rc = run(m->L64, nullptr, k64);
+        if (rc) return rc;
+        status = ctl[0] ? MF_E_TIMEOUT : ctl[1];
+        iters += ctl[2];
+        repl += ctl[3];
+        rep->fallback = 1;
+    }
+    MF_HIP(hipEventRecord(g_na_ev[dev][1], st));
+    MF_HIP(hipEventSynchronize(g_na_ev[dev][1]));
+    float ms = 0.0f;
+    MF_HIP(hipEventElapsedTime(&ms, g_na_ev[dev][0], g_na_ev[dev][1]));
+    rep->status = status;
+    rep->iterations = iters;
+    rep->replacements = repl;
+    rep->outer = 1;
+    rep->relres_true = out[0] > 0 && out[1] >= 0 ? std::sqrt(out[1]) / out[0] : (out[0] > 0 ? -1.0 : 0.0);
+    rep->grid = Gl;
+    rep->block = TPB;
+    rep->rows_per_thread = RPT;
+    rep->sync_deps = -1;
+    return rep->status == MF_E_TIMEOUT ? fail(MF_E_TIMEOUT, "grid barrier timeout") : MF_OK;
+}
+
 // SpMV y = A x (device pointers); x has tilen*B entries.
 int spmv_device(mf_matrix m, const Layout &L, const double *d_x, double *d_y, bool convert)
 {
@@ -852,6 +992,70 @@ int solve_device(mf_matrix m, const double *d_b, double *d_x, const mf_opts &o, 
     rep->reordered = 1;
     rep->pad_factor = pad;
     rep->split_rows = m->child_split_rows;
+    return rc;
+}
+
+__global__ void k_zero(double *__restrict__ x, long long n)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = 0.0;
+}
+
+// mf_solve_device_noalloc: the matrix itself, or (opts.reorder as in
+// solve_device) its row-reordered copy, which an earlier mf_solve_device
+// must have built -- the copy's buffers exist then, and only kernels run here
+// (fills are kernels, so the call can sit in a captured graph).
+int solve_noalloc(mf_matrix m, int dev, const double *d_b, double *d_x, const mf_opts &o, mf_report *rep)
+{
+    hipStream_t st = stream();
+    std::memset(rep, 0, sizeof(*rep));
+    if (dev < 0 || dev >= NA_MAX_DEV || !g_scratch[dev])
+        return fail(MF_E_NODEV, "allocation-free solve: device not set up (mf_init)");
+    if (m->rowA != m->colA) return fail(MF_E_ARG, "solver needs a square matrix");
+    if (o.reorder < MF_REORDER_AUTO || o.reorder > MF_REORDER_ON) return fail(MF_E_ARG, "reorder mode");
+    const size_t ncol = (size_t)std::max(m->tilem, m->tilen) * m->B;
+    if (ncol > NA_MAX_ROWS)
+        return fail(MF_E_ARG, "allocation-free solve: " + std::to_string(ncol) + " padded rows exceed the limit of " +
+                                  std::to_string(NA_MAX_ROWS));
+    const int want = o.index_width ? o.index_width : m->want_iw;
+    if (!m->L64.built || m->L64.want_iw != want)
+        return fail(MF_E_ARG, "allocation-free solve: FP64 layout not built (mf_matrix_layout or a solve first)");
+    double pad = 0.0;
+    bool use = false;
+    if (o.reorder != MF_REORDER_OFF) {
+        pad = (double)m->L64.blob_bytes / (8.0 + m->L64.iw) / std::max(1, m->nnz);
+        use = o.reorder == MF_REORDER_ON || pad > 2.0;
+    }
+    if (use && (!m->child || m->child_grid != o.grid))
+        return fail(MF_E_ARG, "allocation-free solve: the row-reordered copy is not built (an earlier "
+                              "mf_solve_device with these options builds it)");
+    mf_opts oc = o;
+    oc.reorder = MF_REORDER_OFF;
+    MF_HIP(hipEventRecord(g_na_ev[dev][0], st));
+    int rc;
+    if (!use) {
+        rc = noalloc_core(m, dev, d_b, d_x, oc, rep);
+    } else {
+        const int n2 = m->child->rowA;
+        const unsigned nb = (unsigned)((n2 + 255) / 256);
+        k_permute_in<<<nb, 256, 0, st>>>(m->d_perm, n2, d_b, m->d_pb);
+        MF_HIP(hipGetLastError());
+        rc = noalloc_core(m->child, dev, m->d_pb, m->d_px, oc, rep);
+        if (rc && rc != MF_E_TIMEOUT) return rc;
+        const long long nx = (long long)m->tilem * m->B;
+        k_zero<<<(unsigned)((nx + 255) / 256), 256, 0, st>>>(d_x, nx);
+        k_permute_out<<<nb, 256, 0, st>>>(m->d_perm, n2, m->d_px, d_x);
+        MF_HIP(hipGetLastError());
+        rep->reordered = 1;
+        rep->split_rows = m->child_split_rows;
+    }
+    if (rc && rc != MF_E_TIMEOUT) return rc;
+    MF_HIP(hipEventRecord(g_na_ev[dev][1], st));
+    MF_HIP(hipEventSynchronize(g_na_ev[dev][1]));
+    float ms = 0.0f;
+    MF_HIP(hipEventElapsedTime(&ms, g_na_ev[dev][0], g_na_ev[dev][1]));
+    rep->ms_solve = ms;
+    rep->pad_factor = pad;
     return rc;
 }
 

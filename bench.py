@@ -241,6 +241,8 @@ def config_run(mf, torch, name, steps, warmup, cpu=True):
                "err_vs_xstar": float(np.linalg.norm(x - xs) / np.linalg.norm(xs)),
                "grid": reps[0]["grid"], "reordered": reps[0]["reordered"], "sync_deps": reps[0]["sync_deps"],
                "nnz_per_code": A.info()["nnz_per_code"] if opts.get("precision") == mf.MF_MIXED else None,
+               "streamed": reps[0]["streamed"], "fallback": reps[0]["fallback"],
+               "split_rows": reps[0]["split_rows"],
                "roofline": _roofline(it, spmvs, balg, ms)}
         out[label] = rec
         log(f"{name} {label}: {ms:.3f} ms, {it} it, converged {ok}, roofline {rec['roofline']['frac']}")
@@ -495,6 +497,7 @@ def main():
             "replacements_mixed": repsmx[0]["replacements"],
             "grid": repsmx[0]["grid"], "block": repsmx[0]["block"],
             "rows_per_thread": repsmx[0]["rows_per_thread"],
+            "index_width": iw,
         },
         "fp64_ms": round(ms64, 4),
         "fp64_ms_per_step_wall": round(wall64, 4),

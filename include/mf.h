@@ -199,7 +199,7 @@ enum { MF_CG = 0, MF_BICGSTAB = 1 };
 enum { MF_FP64 = 0, MF_MIXED = 1 };
 enum { MF_REORDER_AUTO = 0, MF_REORDER_OFF = 1, MF_REORDER_ON = 2 };
 enum { MF_SYNC_AUTO = 0, MF_SYNC_GLOBAL = 1 };
-enum { MF_VECTORS_AUTO = 0, MF_VECTORS_HBM = 1 };
+enum { MF_VECTORS_AUTO = 0, MF_VECTORS_HBM = 1, MF_VECTORS_ONCHIP = 2 };
 typedef struct {
     int method;         /* MF_CG or MF_BICGSTAB */
     int precision;      /* MF_FP64 or MF_MIXED */
@@ -234,7 +234,17 @@ typedef struct {
                            workgroup's rows stay in registers and LDS where they
                            fit (CG up to 16 rows per thread at B >= 16, else 8;
                            BiCGSTAB 8) and are streamed from HBM beyond that;
-                           MF_VECTORS_HBM: always streamed (same iterates) */
+                           MF_VECTORS_HBM: single systems: MF_E_UNSUPPORTED where
+                           the vectors fit on chip (forcing the streamed kernels
+                           there is switched off);
+                           MF_VECTORS_ONCHIP: single systems: MF_E_UNSUPPORTED
+                           where they do not fit on chip.
+                           Batches: AUTO and HBM keep the vectors in HBM (one
+                           workgroup per system where its share of the work
+                           allows); ONCHIP runs every system on a gang of
+                           workgroups that keeps x, r, p and s of 8 columns in
+                           LDS (2 to 8 right-hand sides; MF_E_UNSUPPORTED for
+                           one) */
 } mf_opts;
 void mf_opts_default(mf_opts *o);
 
@@ -315,7 +325,9 @@ int mf_spmv_bench(mf_matrix m, int mixed, int reps, double *ms);
 
 /* Batched solver object: nsys independent square SPD systems (CSR), each
  * with nrhs (1..8) right-hand sides, solved by CG in one persistent launch
- * captured in a hipGraph.  Tiles are built on the GPU per system and joined
+ * captured in a hipGraph (the vectors in HBM, or, with opts->vectors =
+ * MF_VECTORS_ONCHIP and 2 to 8 right-hand sides, in the LDS of gangs of
+ * workgroups).  Tiles are built on the GPU per system and joined
  * into one block-diagonal layout; opts->block picks the tile size (8, 16 or
  * 32; 0 = 16), opts->grid the target number of workgroups (0 = 2 per CU),
  * opts->precision/tau/allow_mask the per-tile classes (every system reaches
@@ -331,6 +343,10 @@ typedef struct {
     double tau;                    /* their rounding threshold */
     char tuned[512];               /* the autotuner's trials and choice */
     int index_width;               /* slab index bytes shared by the systems (1, 2, 4) */
+    int on_chip;                   /* 1: every system runs on a gang of workgroups that keeps
+                                      its vectors in registers and LDS (8 columns, systems up
+                                      to ~1024 rows per CU); 0: vectors in HBM */
+    int fell_back;                 /* systems of the last solve that left lossy tiles */
 } mf_batch_info_t;
 int mf_batch_create(int nsys, const int *n, const int *const *rowptr, const int *const *colidx,
                     const double *const *val, int nrhs, const mf_opts *o, mf_batch *out);

@@ -18,7 +18,7 @@ MF_CG, MF_BICGSTAB = 0, 1
 MF_FP64, MF_MIXED = 0, 1
 MF_REORDER_AUTO, MF_REORDER_OFF, MF_REORDER_ON = 0, 1, 2
 MF_SYNC_AUTO, MF_SYNC_GLOBAL = 0, 1
-MF_VECTORS_AUTO, MF_VECTORS_HBM = 0, 1
+MF_VECTORS_AUTO, MF_VECTORS_HBM, MF_VECTORS_ONCHIP = 0, 1, 2
 CODE_FP64, CODE_FP8, CODE_FP16, CODE_FP32 = 0, 2, 3, 4
 ALLOW_ALL_LOW = (1 << 2) | (1 << 3) | (1 << 4)
 
@@ -257,7 +257,8 @@ class BatchInfo(C.Structure):
     _fields_ = [("nsys", C.c_int), ("nrhs", C.c_int), ("B", C.c_int), ("G", C.c_int),
                 ("rows", C.c_longlong), ("tilenum", C.c_longlong), ("bytes_alg", C.c_double),
                 ("ms_setup", C.c_double), ("nnz_per_code", C.c_longlong * 5), ("mixed", C.c_int),
-                ("tau", C.c_double), ("tuned", C.c_char * 512), ("index_width", C.c_int)]
+                ("tau", C.c_double), ("tuned", C.c_char * 512), ("index_width", C.c_int),
+                ("on_chip", C.c_int), ("fell_back", C.c_int)]
 
 
 class Batch:

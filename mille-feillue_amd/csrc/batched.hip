@@ -35,8 +35,11 @@ struct BatchArgs {
     DevL L;    // inner layout (mixed classes, or FP64)
     DevL L64;  // FP64 layout: true residuals
     int G, nrhs;
+    int nlines;                           // sync lines per parity buffer (G, or the on-chip tasks' members)
     const int *wg_task;                   // [G + 1]: tasks wg_task[g] .. wg_task[g + 1] - 1 of workgroup g
-    const int *task;                      // [5 t]: system, tile rows tr0, tr1, group's first workgroup, group size
+    const int *task;                      // [5 t]: system, tile rows tr0, tr1, group's first workgroup, group size;
+                                          // on-chip kernel [6 t]: system, tr0, tr1, first sync line, gang size,
+                                          // index in the gang
     const double *b;                      // [rows][K]
     double *x, *r, *p, *s;                // [rows][K]
     float *r32;                           // FP32 copy of r for low-precision tiles
@@ -64,10 +67,17 @@ __device__ __forceinline__ void st_ag(float *p, float v)
 // whose high halves carry the sync epoch and whose low halves carry one half
 // of the value each, so a word showing the current epoch holds the current
 // half whatever order the words arrive in; every workgroup of the group polls
-// the group's words until all show the epoch and sums them in workgroup order
-// (identical bits everywhere).  Two buffers alternate by epoch parity: a
-// buffer is rewritten only after every member has passed the next sync, which
-// it can do only after reading the buffer.
+// the group's words until all show the epoch, lane q reading member q's words
+// (members q + 64, ... too), so one poll costs one memory latency whatever the
+// group size, and sums them in a fixed order (per lane, then a butterfly over
+// the wave: identical bits everywhere).  Two buffers alternate by epoch
+// parity: a buffer is rewritten only after every member has passed the next
+// sync, which it can do only after reading the buffer.
+// The epoch restarts at 0 for every task, and the sync words are cleared once
+// per launch, so a group's lines (gw0 .. gw0 + gn - 1) must not carry words of
+// an earlier task of the same launch: batch_assign gives a workgroup's
+// multi-workgroup task the first place in its list and no workgroup two such
+// tasks, and batch_assign_oc gives every gang lines of its own.
 template <int TPB>
 struct GroupSync {
     double *s_red;  // [TPB/64][BK_MAXV]
@@ -99,7 +109,7 @@ struct GroupSync {
             if (gn == 1) {
                 if (lane < N) s_bc[lane] = part;
             } else {
-                unsigned long long *W = A.slots + (size_t)(ep & 1u) * A.G * 2 * BK_MAXV;
+                unsigned long long *W = A.slots + (size_t)(ep & 1u) * A.nlines * 2 * BK_MAXV;
                 const unsigned long long tag = (unsigned long long)ep << 32;
                 if (lane < N) {
                     const unsigned long long bits = (unsigned long long)__double_as_longlong(part);
@@ -108,18 +118,23 @@ struct GroupSync {
                     __hip_atomic_store(mine + 1, tag | (bits >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
                 long long spins = 0;
-                double acc = 0.0;
+                double acc[N];
                 for (;;) {
                     bool ok = true;
-                    acc = 0.0;
-                    if (lane < N)
-                        for (int q = 0; q < gn; q++) {
-                            const unsigned long long *w = W + (size_t)(gw0 + q) * 2 * BK_MAXV + 2 * lane;
-                            const unsigned long long lo = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                            const unsigned long long hi = __hip_atomic_load(w + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+                    for (int k = 0; k < N; k++) acc[k] = 0.0;
+                    for (int q = lane; q < gn; q += 64) {
+                        const unsigned long long *w = W + (size_t)(gw0 + q) * 2 * BK_MAXV;
+#pragma unroll
+                        for (int k = 0; k < N; k++) {
+                            const unsigned long long lo =
+                                __hip_atomic_load(w + 2 * k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            const unsigned long long hi =
+                                __hip_atomic_load(w + 2 * k + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                             ok = ok && (lo >> 32) == ep && (hi >> 32) == ep;
-                            acc += __longlong_as_double((long long)((hi << 32) | (lo & 0xffffffffull)));
+                            acc[k] += __longlong_as_double((long long)((hi << 32) | (lo & 0xffffffffull)));
                         }
+                    }
                     if (__all(ok)) break;
                     if (__builtin_amdgcn_readfirstlane(ld_agent(A.ctl))) { ab = 1; break; }
                     if (++spins > A.spin_limit) {
@@ -129,7 +144,11 @@ struct GroupSync {
                     }
                     __builtin_amdgcn_s_sleep(1);
                 }
-                if (lane < N) s_bc[lane] = acc;
+#pragma unroll
+                for (int k = 0; k < N; k++) {
+                    const double t = wave_sum(acc[k]);
+                    if (lane == 0) s_bc[k] = t;
+                }
                 if (!ab && publish) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             }
             if (lane == 0) *s_flag = ab;
@@ -555,6 +574,272 @@ __global__ __launch_bounds__(TPB) void k_cg_batched(BatchArgs A)
     }
 }
 
+// ------------------------------------------------ on-chip batched solver
+// Each system runs on a gang of workgroups holding at most OC_RPT * TPB rows
+// each, with the Krylov vectors of its rows on chip for the whole solve: x
+// and p of the thread's rows (8 columns each) in registers, r and s in LDS;
+// only r (and its FP32 copy for low-precision tiles) goes to HBM, published
+// for the gang's gathers, so an iteration moves the matrix and one vector
+// instead of four vectors read and written.  Hestenes-Stiefel CG per column
+// with s = A p by recurrence and (p, A p) summed directly, two gang syncs per
+// iteration, FP64 true-residual passes on the FP64 layout (x and p published
+// for them), converged columns frozen, lossy systems moved to the FP64
+// layout when their true residual stalls -- the same algorithm as cg_system.
+// A workgroup runs its tasks in the order of their scheduled start; a gang's
+// members share one start, so no cycle of waits can form.  Each task has its
+// own sync lines (first line, gang size), so the epochs of one task never
+// meet words of another.
+constexpr int OC_TPB = 512, OC_RPT = 1, OC_K = 8;
+
+template <int TPB, int RPT>
+struct OcShared {
+    double x[RPT * OC_K * TPB], r[RPT * OC_K * TPB], s[RPT * OC_K * TPB], p[RPT * OC_K * TPB];  // [i][k][thread]
+    double red[(TPB / 64) * BK_MAXV];
+    double bc[BK_MAXV];
+    int flag;
+    double g[OC_K], gp[OC_K], ref[OC_K], b2[OC_K], al[OC_K], be[OC_K], tprev[OC_K];
+    int mode[OC_K];
+    int bd, use64;
+};
+
+template <int B, int IW, int TPB, int RPT>
+__device__ bool cg_system_oc(const BatchArgs &A, OcShared<TPB, RPT> &sh, int sys, int tr0, int tr1, int line0,
+                             int gn, int gi)
+{
+    constexpr int K = OC_K, GPB = TPB / B;
+    const int tid = threadIdx.x, grp = tid / B, lane = tid % B, lanebase = (tid & 63) & ~(B - 1);
+    GroupSync<TPB> S{sh.red, sh.bc, &sh.flag, gi, gn, line0};
+    const int KA = A.nrhs;
+    const double tol2 = A.tol * A.tol, rr2 = A.rr_factor * A.rr_factor;
+    const unsigned long long t_start = wall_clock64();
+    auto tile_row = [&](int i) { return tr0 + i * GPB + grp; };
+    auto row_of = [&](int i) { return (size_t)tile_row(i) * B + lane; };
+    auto R = [&](int i, int k) -> double & { return sh.r[(i * K + k) * TPB + tid]; };
+    auto Sv = [&](int i, int k) -> double & { return sh.s[(i * K + k) * TPB + tid]; };
+    auto P = [&](int i, int k) -> double & { return sh.p[(i * K + k) * TPB + tid]; };
+    auto X = [&](int i, int k) -> double & { return sh.x[(i * K + k) * TPB + tid]; };
+    auto publish_r = [&](int i) {
+        const size_t e = row_of(i) * K;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            st_ag(A.r + e + k, R(i, k));
+            if (A.haslow) st_ag(A.r32 + e + k, (float)R(i, k));
+        }
+    };
+    auto finish = [&](int status, int iter, int repl, double relres) {
+#pragma unroll
+        for (int i = 0; i < RPT; i++)
+            if (tile_row(i) < tr1) {
+                const size_t e = row_of(i) * K;
+#pragma unroll
+                for (int k = 0; k < K; k++) A.x[e + k] = X(i, k);
+            }
+        if (gi == 0 && tid == 0) {
+            A.sys_out[4 * sys] = status;
+            A.sys_out[4 * sys + 1] = iter;
+            A.sys_out[4 * sys + 2] = repl | (sh.use64 && A.haslow ? 1 << 30 : 0);
+            A.sys_out[4 * sys + 3] = (int)min((wall_clock64() - t_start) / 100ull, 0x7fffffffull);
+            A.sys_relres[sys] = relres;
+        }
+    };
+
+    double v[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) v[k] = 0.0;
+#pragma unroll
+    for (int i = 0; i < RPT; i++) {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            X(i, k) = R(i, k) = Sv(i, k) = P(i, k) = 0.0;
+        }
+        if (tile_row(i) < tr1) {
+            const size_t e = row_of(i) * K;
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const double bv = A.b[e + k];
+                R(i, k) = bv;
+                v[k] += bv * bv;
+            }
+            publish_r(i);
+        }
+    }
+    if (!S.allreduce(A, v, true)) return false;
+    if (tid < K) {
+        sh.b2[tid] = sh.g[tid] = sh.ref[tid] = sh.tprev[tid] = sh.bc[tid];
+        sh.gp[tid] = 1.0;
+        sh.mode[tid] = 1;
+    }
+    if (tid == 0) sh.use64 = A.haslow ? 0 : 1;
+    __syncthreads();
+    int iter = 0, repl = 0;
+    bool force = false;
+    for (;;) {
+        bool conv = true, due = A.rr_factor > 0.0, finite = true;
+        for (int k = 0; k < KA; k++) {
+            const double gk = sh.g[k];
+            finite = finite && isfinite(gk);
+            conv = conv && gk <= tol2 * sh.b2[k];
+            due = due && gk <= rr2 * sh.ref[k];
+        }
+        if (!finite) { finish(MF_E_NONFINITE, iter, repl, -1.0); return true; }
+        if (conv || due || force) {
+            // r = b - A x and s = A p over the FP64 layout, from x and p
+            // published for the gang
+#pragma unroll
+            for (int i = 0; i < RPT; i++)
+                if (tile_row(i) < tr1) {
+                    const size_t e = row_of(i) * K;
+#pragma unroll
+                    for (int k = 0; k < K; k++) {
+                        st_ag(A.x + e + k, X(i, k));
+                        st_ag(A.p + e + k, P(i, k));
+                    }
+                }
+            if (!S.allreduce(A, v, true)) return false;
+            double t0[K];
+#pragma unroll
+            for (int k = 0; k < K; k++) t0[k] = 0.0;
+#pragma unroll
+            for (int i = 0; i < RPT; i++)
+                if (tile_row(i) < tr1) {
+                    const size_t e = row_of(i) * K;
+                    // two halves of the columns (fewer live registers)
+#pragma unroll
+                    for (int h = 0; h < 2; h++) {
+                        double ax[K / 2], ap[K / 2];
+                        row_spmv64_xp<B, IW, K, K / 2>(A.L64, tile_row(i), lane, lanebase, h * (K / 2), A.x, A.p,
+                                                       ax, ap);
+#pragma unroll
+                        for (int j = 0; j < K / 2; j++) {
+                            const int k = h * (K / 2) + j;
+                            const double rv = A.b[e + k] - ax[j];
+                            R(i, k) = rv;
+                            Sv(i, k) = ap[j];
+                            t0[k] += rv * rv;
+                        }
+                    }
+                    publish_r(i);
+                }
+            if (!S.allreduce(A, t0, true)) return false;
+            ++repl;
+            bool done = true;
+            double rel = 0.0;
+            for (int k = 0; k < KA; k++) {
+                done = done && sh.bc[k] <= tol2 * sh.b2[k];
+                if (sh.b2[k] > 0.0) rel = fmax(rel, sqrt(sh.bc[k] / sh.b2[k]));
+            }
+            if (!isfinite(rel)) { finish(MF_E_NONFINITE, iter, repl, rel); return true; }
+            if (done) { finish(MF_OK, iter, repl, rel); return true; }
+            if (tid == 0) {
+                bool stall = false;
+                for (int k = 0; k < KA; k++) {
+                    stall = stall || (sh.bc[k] > tol2 * sh.b2[k] && sh.bc[k] > 0.25 * sh.tprev[k]);
+                    sh.tprev[k] = sh.bc[k];
+                }
+                if (stall) sh.use64 = 1;
+            }
+            if (tid < K) {
+                const bool restart = force || sh.bc[tid] > 4.0 * sh.g[tid];
+                sh.mode[tid] = restart ? 1 : 0;
+                sh.g[tid] = sh.ref[tid] = sh.bc[tid];
+            }
+            force = false;
+            __syncthreads();
+            continue;
+        }
+        if (iter >= A.maxit) {
+            double rel = 0.0;
+            for (int k = 0; k < KA; k++)
+                if (sh.b2[k] > 0.0) rel = fmax(rel, sqrt(sh.g[k] / sh.b2[k]));
+            finish(MF_E_MAXIT, iter, repl, rel);
+            return true;
+        }
+        if (tid < K) {
+            const int m = sh.mode[tid];
+            const bool fresh = iter == 0 || (m & 1);
+            const bool frozen = tid >= KA || sh.g[tid] <= tol2 * sh.b2[tid];
+            sh.be[tid] = (fresh || frozen) ? 0.0 : sh.g[tid] / sh.gp[tid];
+            sh.mode[tid] = (fresh ? 4 : 0) | (frozen ? 16 : 0);
+        }
+        if (tid == 0) sh.bd = 0;
+        __syncthreads();
+        // w = A r; s = w + beta s; p = r + beta p; (p, s)
+        double d[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) d[k] = 0.0;
+        const bool u64 = sh.use64;
+#pragma unroll
+        for (int i = 0; i < RPT; i++)
+            if (tile_row(i) < tr1) {
+                double w[K];
+                row_spmvK<B, IW, K, K>(u64 ? A.L64 : A.L, tile_row(i), lane, lanebase, 0, A.r, A.r32, w);
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    const double bk = sh.be[k];
+                    const double sv = w[k] + bk * Sv(i, k);
+                    Sv(i, k) = sv;
+                    const double pv = R(i, k) + bk * P(i, k);
+                    P(i, k) = pv;
+                    d[k] += pv * sv;
+                }
+            }
+        if (!S.allreduce(A, d, false)) return false;
+        if (tid < K) {
+            const int m = sh.mode[tid];
+            const double denom = sh.bc[tid];
+            if (m & 16) {
+                sh.al[tid] = 0.0;
+            } else if (!(denom > 0.0) || !isfinite(denom)) {
+                sh.al[tid] = 0.0;
+                atomicOr(&sh.bd, (m & 4) ? 1 : 2);
+            } else {
+                sh.al[tid] = sh.g[tid] / denom;
+            }
+        }
+        __syncthreads();
+        if (sh.bd) {
+            if (sh.bd & 1) { finish(MF_E_BREAKDOWN, iter, repl, -1.0); return true; }
+            force = true;
+            continue;
+        }
+        double gg[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) gg[k] = 0.0;
+#pragma unroll
+        for (int i = 0; i < RPT; i++)
+            if (tile_row(i) < tr1) {
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    const double ak = sh.al[k];
+                    X(i, k) = X(i, k) + ak * P(i, k);
+                    const double rv = R(i, k) - ak * Sv(i, k);
+                    R(i, k) = rv;
+                    gg[k] += rv * rv;
+                }
+                publish_r(i);
+            }
+        if (!S.allreduce(A, gg, true)) return false;
+        if (tid < K) {
+            if (!(sh.mode[tid] & 16)) sh.gp[tid] = sh.g[tid];
+            sh.g[tid] = sh.bc[tid];
+            sh.mode[tid] = 0;
+        }
+        __syncthreads();
+        ++iter;
+    }
+}
+
+template <int B, int IW, int TPB, int RPT>
+__global__ __launch_bounds__(TPB, 1) void k_cg_batched_oc(BatchArgs A)
+{
+    __shared__ OcShared<TPB, RPT> sh;
+    for (int t = A.wg_task[blockIdx.x]; t < A.wg_task[blockIdx.x + 1]; t++) {
+        const int *q = A.task + 6 * (size_t)t;
+        if (!cg_system_oc<B, IW, TPB, RPT>(A, sh, q[0], q[1], q[2], q[3], q[4], q[5])) return;
+        __syncthreads();  // LDS is reused by the next system
+    }
+}
+
 __global__ void k_batch_reset(unsigned long long *slots, long long nslots, int *ctl, int *sys_out, int nsys)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -633,6 +918,9 @@ struct mf_batch_s {
     BatchLayout L64, Lmix;
     int G = 0;
     std::vector<int> sys_nwg;
+    bool oc = false;       // on-chip kernel (vectors in registers/LDS, gangs of workgroups)
+    int fell_last = 0;     // systems of the last solve that left lossy tiles
+    int nlines = 0;        // sync lines per parity buffer
     int *d_wg = nullptr;  // wg_task [G + 1], then task records [5][tasks]
     double *d_vec = nullptr;  // b, x, r, p, s: 5 x rows x K
     float *d_r32 = nullptr;
@@ -673,6 +961,22 @@ static KernelFn batched_fn(int K)
 template <int B> static KernelFn batched_fn_iw(int iw, int K)
 {
     return iw == 1 ? batched_fn<B, 1>(K) : iw == 2 ? batched_fn<B, 2>(K) : batched_fn<B, 4>(K);
+}
+
+template <int B> static KernelFn batched_oc_fn(int iw)
+{
+    return iw == 1   ? (KernelFn)k_cg_batched_oc<B, 1, OC_TPB, OC_RPT>
+           : iw == 2 ? (KernelFn)k_cg_batched_oc<B, 2, OC_TPB, OC_RPT>
+                     : (KernelFn)k_cg_batched_oc<B, 4, OC_TPB, OC_RPT>;
+}
+
+static KernelFn pick_batched_oc(int B, int iw)
+{
+    switch (B) {
+    case 8: return batched_oc_fn<8>(iw);
+    case 16: return batched_oc_fn<16>(iw);
+    default: return batched_oc_fn<32>(iw);
+    }
 }
 
 static KernelFn pick_batched(int B, int iw, int K)
@@ -737,8 +1041,11 @@ static int concat_layouts(mf_batch_s *bt, const std::vector<mf_matrix> &mats, bo
 // system's workgroup); bisection finds the smallest T that fits g_target
 // workgroups, and that assignment is used.
 // iters: per-system iteration counts of an earlier solve (or estimates).
+static int batch_assign_oc(mf_batch_s *bt, const std::vector<int> &iters);
+
 static int batch_assign(mf_batch_s *bt, int g_target, const std::vector<int> &iters)
 {
+    if (bt->oc) return batch_assign_oc(bt, iters);
     // work in microseconds of one workgroup: measured (wtime) when a solve
     // has timed every system, else the byte model at about 2.5 KB per us plus
     // ~3 us per iteration of scalar work and block reductions; a split
@@ -822,6 +1129,11 @@ static int batch_assign(mf_batch_s *bt, int g_target, const std::vector<int> &it
         const int wg = pack_of[s];
         tasks[wg].insert(tasks[wg].end(), {s, bt->trbase[s], bt->trbase[s] + bt->tilem[s], wg, 1});
     }
+    // GroupSync restarts its epoch per task on lines cleared once per launch:
+    // a multi-workgroup task must be its workgroups' first and only one
+    for (int q = 0; q < G; q++)
+        for (size_t t = 5; t < tasks[q].size(); t += 5)
+            if (tasks[q][t + 4] > 1) return fail(MF_E_UNSUPPORTED, "batch plan: a split system after another task");
     std::vector<int> h(G + 1, 0);
     for (int q = 0; q < G; q++) h[q + 1] = h[q] + (int)tasks[q].size() / 5;
     for (int q = 0; q < G; q++) h.insert(h.end(), tasks[q].begin(), tasks[q].end());
@@ -833,6 +1145,73 @@ static int batch_assign(mf_batch_s *bt, int g_target, const std::vector<int> &it
     MF_HIP(hipMemcpy(bt->d_wg, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice));
     MF_HIP(hipMalloc((void **)&bt->d_slots, sizeof(unsigned long long) * 2 * 2 * BK_MAXV * (size_t)G));
     bt->G = G;
+    bt->nlines = G;
+    bt->TPB = 256;
+    return MF_OK;
+}
+
+// On-chip assignment: system s needs W_s = ceil(tile rows / (OC_RPT x
+// OC_TPB / B)) workgroups (its vectors then fit their registers and LDS),
+// all running it together for t_s.  List scheduling on the resident grid (one
+// workgroup per CU): systems largest gang first (then longest), each on the
+// W_s workgroups free earliest, starting when the last of them is free.  A
+// workgroup's tasks are in start order, and every member of a gang has the
+// same start, so the earliest unfinished task can always run (no cycle of
+// waits).  t_s: measured on an earlier solve (wtime), else iterations x
+// (bytes of the matrix and the published vector per workgroup at ~10 GB/s
+// per CU, plus ~8 us of syncs and scalar work).
+static int batch_assign_oc(mf_batch_s *bt, const std::vector<int> &iters)
+{
+    const int n_sys = bt->nsys, G = device_cu_count();
+    const int trw = OC_RPT * (OC_TPB / bt->B);
+    const bool measured = (int)bt->wtime.size() == n_sys;
+    std::vector<int> W(n_sys);
+    std::vector<double> t(n_sys);
+    for (int s = 0; s < n_sys; s++) {
+        W[s] = (bt->tilem[s] + trw - 1) / trw;
+        if (W[s] > G) return fail(MF_E_UNSUPPORTED, "system larger than the device holds on chip");
+        const double its = iters.empty() ? 1.0 : std::max(1, iters[s]);
+        const double bytes = (12.0 * bt->nnz[s] + 2.0 * 64.0 * bt->tilem[s] * bt->B) / W[s];
+        t[s] = measured ? std::max(bt->wtime[s], 1.0) : its * (bytes / 1.0e4 + 8.0);
+    }
+    std::vector<int> order(n_sys);
+    for (int s = 0; s < n_sys; s++) order[s] = s;
+    std::sort(order.begin(), order.end(), [&](int a, int c) {
+        return W[a] != W[c] ? W[a] > W[c] : (t[a] != t[c] ? t[a] > t[c] : a < c);
+    });
+    std::vector<double> avail(G, 0.0);
+    std::vector<std::vector<int>> tasks(G);  // per workgroup: 6-int records
+    std::vector<int> idx(G);
+    int line = 0;
+    for (int s : order) {
+        for (int g = 0; g < G; g++) idx[g] = g;
+        std::partial_sort(idx.begin(), idx.begin() + W[s], idx.end(),
+                          [&](int a, int c) { return avail[a] < avail[c] || (avail[a] == avail[c] && a < c); });
+        double start = 0.0;
+        for (int q = 0; q < W[s]; q++) start = std::max(start, avail[idx[q]]);
+        for (int q = 0; q < W[s]; q++) {
+            const int g = idx[q];
+            const int a = bt->trbase[s] + (int)((long long)bt->tilem[s] * q / W[s]);
+            const int b = bt->trbase[s] + (int)((long long)bt->tilem[s] * (q + 1) / W[s]);
+            tasks[g].insert(tasks[g].end(), {s, a, b, line, W[s], q});
+            avail[g] = start + t[s];
+        }
+        line += W[s];
+    }
+    bt->sys_nwg.assign(W.begin(), W.end());
+    std::vector<int> h(G + 1, 0);
+    for (int q = 0; q < G; q++) h[q + 1] = h[q] + (int)tasks[q].size() / 6;
+    for (int q = 0; q < G; q++) h.insert(h.end(), tasks[q].begin(), tasks[q].end());
+    if (bt->d_wg) MF_HIP(hipFree(bt->d_wg));
+    if (bt->d_slots) MF_HIP(hipFree(bt->d_slots));
+    bt->d_wg = nullptr;
+    bt->d_slots = nullptr;
+    MF_HIP(hipMalloc((void **)&bt->d_wg, sizeof(int) * h.size()));
+    MF_HIP(hipMemcpy(bt->d_wg, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice));
+    MF_HIP(hipMalloc((void **)&bt->d_slots, sizeof(unsigned long long) * 2 * 2 * BK_MAXV * (size_t)line));
+    bt->G = G;
+    bt->nlines = line;
+    bt->TPB = OC_TPB;
     return MF_OK;
 }
 
@@ -846,6 +1225,7 @@ static int batch_capture(mf_batch_s *bt)
     A.L = DevL{bt->d_tile_ptr, Li.trh, Li.stc, Li.blob};
     A.L64 = DevL{bt->d_tile_ptr, bt->L64.trh, bt->L64.stc, bt->L64.blob};
     A.G = bt->G;
+    A.nlines = bt->nlines;
     A.nrhs = bt->nrhs;
     A.wg_task = bt->d_wg;
     A.task = bt->d_wg + bt->G + 1;
@@ -875,7 +1255,7 @@ static int batch_capture(mf_batch_s *bt)
         MF_HIP(hipMemset(bt->d_prof, 0, sizeof(unsigned long long) * 4 * bt->G));
         A.prof = bt->d_prof;
     }
-    bt->fn = pick_batched(bt->B, bt->iw, bt->K);
+    bt->fn = bt->oc ? pick_batched_oc(bt->B, bt->iw) : pick_batched(bt->B, bt->iw, bt->K);
     int per_cu = 0;
     MF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bt->fn, bt->TPB, 0));
     if ((long long)per_cu * device_cu_count() < bt->G)
@@ -884,7 +1264,7 @@ static int batch_capture(mf_batch_s *bt)
     if (bt->graph) (void)hipGraphDestroy(bt->graph);
     bt->gexec = nullptr;
     bt->graph = nullptr;
-    const long long nslots = 2ll * 2 * BK_MAXV * bt->G;
+    const long long nslots = 2ll * 2 * BK_MAXV * bt->nlines;
     const long long nr = std::max<long long>(nslots, 4ll * bt->nsys);
     MF_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     k_batch_reset<<<(unsigned)((nr + 255) / 256), 256, 0, st>>>(bt->d_slots, nslots, bt->d_ctl, bt->d_sys_out, bt->nsys);
@@ -1036,6 +1416,8 @@ int mf_batch_create(int nsys, const int *n, const int *const *rowptr, const int 
     if (nrhs < 1 || nrhs > 8) return fail(MF_E_ARG, "nrhs must be 1..8");
     if (opts && (opts->index_width < 0 || opts->index_width == 3 || opts->index_width > 4))
         return fail(MF_E_ARG, "index width must be 0, 1, 2 or 4");
+    if (opts && (opts->vectors < MF_VECTORS_AUTO || opts->vectors > MF_VECTORS_ONCHIP))
+        return fail(MF_E_ARG, "vectors");
     int rc = ensure_device();
     if (rc) return rc;
     mf_batch_s *bt = new mf_batch_s();
@@ -1071,6 +1453,17 @@ int mf_batch_create(int nsys, const int *n, const int *const *rowptr, const int 
         }
     }
     if (!rc) rc = batch_build(bt);
+    if (!rc) {
+        // the on-chip kernel when asked for: the 8-column form (2 to 8
+        // right-hand sides), every system's gang within the resident grid
+        if (bt->o.vectors == MF_VECTORS_ONCHIP) {
+            const int trw = OC_RPT * (OC_TPB / bt->B);
+            bool fit = bt->K == OC_K;
+            for (int s = 0; s < nsys && fit; s++) fit = (bt->tilem[s] + trw - 1) / trw <= device_cu_count();
+            if (!fit) rc = fail(MF_E_UNSUPPORTED, "on-chip batch: needs 2 to 8 right-hand sides and systems that fit the grid");
+            bt->oc = fit;
+        }
+    }
     if (!rc) rc = batch_assign(bt, batch_target(bt), bt->weights);
     if (!rc) rc = batch_capture(bt);
     if (rc) {
@@ -1127,7 +1520,8 @@ static void batch_measure(mf_batch_s *bt, const BatchOutcome &oc)
     for (int s = 0; s < bt->nsys; s++) {
         const int w = s < (int)bt->sys_nwg.size() ? bt->sys_nwg[s] : 1;
         const double t = oc.us[s];
-        bt->wtime[s] = w > 1 ? std::max(t - oc.iters[s] * C_SYNC, 0.5 * t) * w : t;
+        // on chip: a gang's time as it ran
+        bt->wtime[s] = bt->oc ? t : w > 1 ? std::max(t - oc.iters[s] * C_SYNC, 0.5 * t) * w : t;
     }
 }
 
@@ -1196,6 +1590,7 @@ int mf_batch_solve(mf_batch bt, const double *const *b, double *const *x, int *i
         if (relres) relres[s] = oc.relres[s];
     }
     batch_report(bt, oc, ms, rep);
+    bt->fell_last = oc.fell;
     if (bt->args.prof) {
         std::vector<unsigned long long> pf(4 * (size_t)bt->G);
         MF_HIP(hipMemcpy(pf.data(), bt->args.prof, sizeof(unsigned long long) * pf.size(), hipMemcpyDeviceToHost));
@@ -1218,7 +1613,8 @@ int mf_batch_solve(mf_batch bt, const double *const *b, double *const *x, int *i
         if (std::atoi(std::getenv("MF_PHASE_PROFILE")) == 2) {
             std::vector<int> h(bt->G + 1);
             MF_HIP(hipMemcpy(h.data(), bt->d_wg, sizeof(int) * (bt->G + 1), hipMemcpyDeviceToHost));
-            std::vector<int> tk(5 * (size_t)h[bt->G]);
+            const int TS = bt->oc ? 6 : 5;  // ints per task record
+            std::vector<int> tk(TS * (size_t)h[bt->G]);
             MF_HIP(hipMemcpy(tk.data(), bt->d_wg + bt->G + 1, sizeof(int) * tk.size(), hipMemcpyDeviceToHost));
             for (int g = 0; g < bt->G; g++) {
                 double t = 0;
@@ -1226,7 +1622,7 @@ int mf_batch_solve(mf_batch bt, const double *const *b, double *const *x, int *i
                 std::fprintf(stderr, "[mf batch wg] %d total %.0f spmv %.0f update %.0f sync %.0f |", g, t,
                              pf[4 * g] / 100.0, pf[4 * g + 1] / 100.0, pf[4 * g + 2] / 100.0);
                 for (int q = h[g]; q < h[g + 1]; q++) {
-                    const int *r = tk.data() + 5 * (size_t)q;
+                    const int *r = tk.data() + TS * (size_t)q;
                     std::fprintf(stderr, " s%d n=%d it=%d w=%d us=%.0f;", r[0], bt->n[r[0]], oc.iters[r[0]], r[4],
                                  oc.us[r[0]]);
                 }
@@ -1361,6 +1757,8 @@ int mf_batch_info(mf_batch bt, mf_batch_info_t *info)
     info->tau = bt->tau.empty() ? 0.0 : bt->tau[0];
     std::snprintf(info->tuned, sizeof info->tuned, "%s", bt->tuned.c_str());
     info->index_width = bt->iw;
+    info->on_chip = bt->oc ? 1 : 0;
+    info->fell_back = bt->fell_last;
     return MF_OK;
 }
 

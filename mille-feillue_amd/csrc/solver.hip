@@ -436,8 +436,15 @@ static int launch_inner(mf_matrix m, const Layout &L, const Layout *L64, int met
 {
     int G = 1, trw = 1;
     solver_grid(m->tilem, m->B, o.grid, &G, &trw);
-    KernelChoice kc = pick(m->B, L.iw, method, (long long)trw * m->B, o.vectors == MF_VECTORS_HBM);
+    KernelChoice kc = pick(m->B, L.iw, method, (long long)trw * m->B, false);
     if (!kc.fn) return fail(MF_E_UNSUPPORTED, "no persistent kernel for this tile size and index form");
+    // streaming the vectors of a system whose rows fit on chip is switched
+    // off: two runs that forced it on small grids ended with a GPU fault
+    // whose cause is not known
+    if (o.vectors == MF_VECTORS_HBM && !kc.streamed)
+        return fail(MF_E_UNSUPPORTED, "vectors in HBM only for systems beyond register/LDS residency");
+    if (o.vectors == MF_VECTORS_ONCHIP && kc.streamed)
+        return fail(MF_E_UNSUPPORTED, "vectors on chip: the system exceeds register/LDS residency at this grid");
     int rc = ensure_ws(m, G);
     if (rc) return rc;
     if (kc.streamed) {
@@ -497,6 +504,7 @@ static int solve_direct(mf_matrix m, const double *d_b, double *d_x, const mf_op
     if (m->rowA != m->colA) return fail(MF_E_ARG, "solver needs a square matrix");
     if (o.index_width < 0 || o.index_width == 3 || o.index_width > 4)
         return fail(MF_E_ARG, "index width must be 0, 1, 2 or 4");
+    if (o.vectors < MF_VECTORS_AUTO || o.vectors > MF_VECTORS_ONCHIP) return fail(MF_E_ARG, "vectors");
     double t0 = now_ms();
     int rc;
     const int want = o.index_width ? o.index_width : m->want_iw;

@@ -131,3 +131,60 @@ def test_gpu_batched_mixed_lossy_tiles(gpu):
     xs, it, rel, rep = bt.solve(b)
     _check_batch(None, systems, b, xs, it, rel, rep, check_iters=False)
     bt.close()
+
+
+def test_gpu_batched_on_chip_matches_hbm(gpu, orc):
+    """The on-chip kernel (vectors = MF_VECTORS_ONCHIP: each system on a gang
+    of workgroups with its vectors in LDS) and the default HBM-vector kernel
+    on the same batch: every column at 1e-10 in both, iterations within 5%
+    (+2) of each other and of the oracle; gangs of up to 18 workgroups (9000
+    rows at 512 rows each)."""
+    systems = _systems(gpu, [4000, 9000, 600, 1500], seed=3)
+    b = _rhs(systems, 8, 4)
+    out = {}
+    for vec in (gpu.MF_VECTORS_ONCHIP, gpu.MF_VECTORS_AUTO):
+        bt = gpu.Batch(systems, nrhs=8, tol=TOL, vectors=vec)
+        info = bt.info()
+        print(f"vectors={vec} on_chip={info['on_chip']} G={info['G']}", flush=True)
+        assert info["on_chip"] == (1 if vec == gpu.MF_VECTORS_ONCHIP else 0)
+        xs, it, rel, rep = bt.solve(b)
+        _check_batch(orc, systems, b, xs, it, rel, rep, check_iters=(vec == gpu.MF_VECTORS_ONCHIP))
+        out[vec] = it
+        bt.close()
+    a, h = out[gpu.MF_VECTORS_ONCHIP], out[gpu.MF_VECTORS_AUTO]
+    assert np.all(np.abs(a - h) <= 0.05 * h + 2), (a, h)
+
+
+def test_gpu_batched_on_chip_queued_gangs(gpu):
+    """More gang slots than CUs (30 systems of ~9000 rows need ~540
+    workgroup tasks on 256 workgroups): the scheduled order runs every gang
+    without a cycle of waits, twice in a row on the captured graph."""
+    systems = _systems(gpu, [9000 + 37 * i for i in range(30)], seed=21)
+    b = _rhs(systems, 8, 22)
+    bt = gpu.Batch(systems, nrhs=8, tol=TOL, vectors=gpu.MF_VECTORS_ONCHIP)
+    assert bt.info()["on_chip"] == 1
+    xs, it, rel, rep = bt.solve(b)
+    print(f"queued gangs: it max {it.max()} ms {rep['ms_solve']:.1f}", flush=True)
+    _check_batch(None, systems, b, xs, it, rel, rep, check_iters=False)
+    xs2, it2, rel2, rep2 = bt.solve(b)
+    assert rep2["status"] == 0 and np.array_equal(it, it2)
+    assert all(np.array_equal(p, q) for p, q in zip(xs, xs2))
+    bt.close()
+
+
+def test_gpu_batched_on_chip_refusals(gpu):
+    """MF_VECTORS_ONCHIP needs two or more right-hand sides (the kernel's 8
+    columns); a single system that fits on chip solves with it, and an
+    out-of-range vectors value is refused."""
+    systems = _systems(gpu, [900, 1200], seed=5)
+    with pytest.raises(gpu.MFError):
+        gpu.Batch(systems, nrhs=1, tol=TOL, vectors=gpu.MF_VECTORS_ONCHIP)
+    with pytest.raises(gpu.MFError):
+        gpu.Batch(systems, nrhs=8, tol=TOL, vectors=7)
+    rp, ci, va = systems[0]
+    A = gpu.Matrix(rp, ci, va)
+    x, rep = A.solve(np.ones(len(rp) - 1), method=gpu.MF_CG, tol=TOL, vectors=gpu.MF_VECTORS_ONCHIP)
+    assert rep["status"] == 0 and rep["streamed"] == 0
+    with pytest.raises(gpu.MFError):
+        A.solve(np.ones(len(rp) - 1), method=gpu.MF_CG, tol=TOL, vectors=7)
+    A.close()

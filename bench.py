@@ -243,6 +243,8 @@ def config_run(mf, torch, name, steps, warmup, cpu=True):
                "nnz_per_code": A.info()["nnz_per_code"] if opts.get("precision") == mf.MF_MIXED else None,
                "streamed": reps[0]["streamed"], "fallback": reps[0]["fallback"],
                "split_rows": reps[0]["split_rows"],
+               "pattern_slabs": A.info()["pattern_slabs_mixed" if opts.get("precision") == mf.MF_MIXED
+                                         else "pattern_slabs_fp64"],
                "roofline": _roofline(it, spmvs, balg, ms)}
         out[label] = rec
         log(f"{name} {label}: {ms:.3f} ms, {it} it, converged {ok}, roofline {rec['roofline']['frac']}")
@@ -400,6 +402,7 @@ def main():
     if args.calib:
         calibration_copy(torch)
 
+    info = A.info()  # both layouts are built by now
     # standalone SpMV kernel
     spmv64 = A.spmv_bench(50, mixed=False)
     spmvmx = A.spmv_bench(50, mixed=True)
@@ -498,6 +501,8 @@ def main():
             "grid": repsmx[0]["grid"], "block": repsmx[0]["block"],
             "rows_per_thread": repsmx[0]["rows_per_thread"],
             "index_width": iw,
+            "pattern_slabs": {"fp64": info["pattern_slabs_fp64"], "mixed": info["pattern_slabs_mixed"],
+                              "tile_rows": info["tilem"]},
         },
         "fp64_ms": round(ms64, 4),
         "fp64_ms_per_step_wall": round(wall64, 4),

@@ -182,6 +182,10 @@ typedef struct {
     double bytes_alg_mixed;       /* same for the last mixed layout built */
     double bytes_layout_mixed;    /* actual bytes of the mixed device layout */
     int index_width;              /* slab index bytes of that layout (1, 2 or 4) */
+    long long pattern_slabs_fp64; /* (tile row, class) slabs in pattern form: FP64 layout */
+    long long pattern_slabs_mixed;/* and the mixed layout (pattern slabs: one offset table
+                                     per slab instead of an index per entry, where the
+                                     rows of a tile row share their column offsets) */
 } mf_matrix_info;
 int mf_matrix_get_info(mf_matrix m, mf_matrix_info *info);
 
@@ -189,9 +193,13 @@ int mf_matrix_get_info(mf_matrix m, mf_matrix_info *info);
  * = automatic: the two-byte tile-row-relative form where every tile lies
  * within 32768/B - 1 tile columns of the diagonal, else the one-byte slot
  * form where every tile row has few enough tiles (B = 8: 8, 16: 16, 32: 8),
- * else the four-byte column.  1, 2 or 4 = that form where it fits, else the
- * next wider one.  Layouts already built are released.  No reference
- * counterpart (the reference stores four-byte columns only). */
+ * else the four-byte column; with the two- and four-byte forms, a class slab
+ * whose rows share their column offsets (stencils) stores one table of
+ * offsets instead of an index per entry (pattern slab).  1, 2 or 4 = that
+ * form where it fits, else the next wider one, and no pattern slabs.
+ * Layouts already built are released.  Results are bit-identical across
+ * forms.  No reference counterpart (the reference stores four-byte columns
+ * only). */
 int mf_matrix_set_index_width(mf_matrix m, int width);
 
 /* Precision / solver options. */

@@ -50,7 +50,8 @@ class Info(C.Structure):
                 ("tilem", C.c_int), ("tilen", C.c_int), ("tilenum", C.c_int),
                 ("tiles_per_code", C.c_longlong * 5), ("nnz_per_code", C.c_longlong * 5),
                 ("bytes_alg_fp64", C.c_double), ("bytes_alg_mixed", C.c_double),
-                ("bytes_layout_mixed", C.c_double), ("index_width", C.c_int)]
+                ("bytes_layout_mixed", C.c_double), ("index_width", C.c_int),
+                ("pattern_slabs_fp64", C.c_longlong), ("pattern_slabs_mixed", C.c_longlong)]
 
 
 class Opts(C.Structure):

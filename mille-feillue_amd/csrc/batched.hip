@@ -1318,8 +1318,8 @@ static int batch_build(mf_batch_s *bt)
     }
     const int iw = resolve_index_width(bt->o.index_width, fits);
     for (int s = 0; s < bt->nsys && !rc; s++) {
-        rc = build_layout(mats[s], mats[s]->L64, 0.0, 0, iw);
-        if (!rc && bt->mixed) rc = build_layout(mats[s], mats[s]->Lmix, bt->tau[s], bt->o.allow_mask, iw);
+        rc = build_layout(mats[s], mats[s]->L64, 0.0, 0, iw, false);
+        if (!rc && bt->mixed) rc = build_layout(mats[s], mats[s]->Lmix, bt->tau[s], bt->o.allow_mask, iw, false);
         if (!rc && (mats[s]->L64.iw != iw || (bt->mixed && mats[s]->Lmix.iw != iw)))
             rc = fail(MF_E_FORMAT, "slab index width differs between systems");
     }

@@ -380,6 +380,8 @@ int mf_matrix_get_info(mf_matrix m, mf_matrix_info *info)
     info->bytes_alg_mixed = L.bytes_alg;
     info->bytes_layout_mixed = L.bytes_actual;
     info->index_width = L.iw;
+    info->pattern_slabs_fp64 = m->L64.pattern_slabs;
+    info->pattern_slabs_mixed = L.pattern_slabs;
     return MF_OK;
 }
 

@@ -28,6 +28,14 @@ namespace mf {
 // rows are padded with value 0 / index 0 (in every form a valid column).
 // Every slab size is a multiple of 16 bytes.  trh[tr * 4 + c] = {slab offset / 8, L_c}.  Class c:
 // 0 FP64, 1 FP32, 2 FP16, 3 FP8 E4M3 (OCP).
+// Pattern slabs (two- and four-byte forms; bit 0 of the offset word set):
+// when the rows of a tile row have their class-c entries in ascending column
+// order and the union of their offsets column - row is short (stencils), the
+// slab stores one table of those offsets (ascending, L_c of them, in quads of
+// 4 x IW bytes) instead of an index per entry; row r's entry with offset d
+// sits at the table position of d, and positions it lacks hold value 0 at the
+// in-range column row + d.  Entry order per row is unchanged, so the sums
+// are bit-identical to the indexed form.
 struct DevL {
     const int *tile_ptr;
     const int2 *trh;
@@ -81,6 +89,7 @@ struct RowCtx {
     int tc_reg;  // index base: tile column of slot r (IW 1), tr * B (IW 2), 0 (IW 4)
     int4 h01, h23;  // slab headers of classes 0,1 and 2,3
     int split;   // lanes per row (DevL::split), 1 when none
+    int trB;     // first row of the tile row (tr * B)
 };
 
 template <int B, int IW>
@@ -94,19 +103,23 @@ __device__ __forceinline__ RowCtx row_ctx(const DevL &L, int tr, int r)
     c.h01 = h[0];
     c.h23 = h[1];
     c.split = L.split ? (int)L.split[tr] : 1;
+    c.trB = tr * B;
     return c;
 }
 
-// Gather-vector index of one slab entry (as gather_col, from a row context).
+// Gather-vector index of one slab entry (as gather_col): base is the row
+// context's index base, or the lane's row for a pattern slab.
 template <int B, int IW>
-__device__ __forceinline__ int gather_index(uint32_t id, const RowCtx &c, const DevL &L, int lanebase)
+__device__ __forceinline__ int gather_index(uint32_t id, int base, int lanebase)
 {
-    (void)L;
-    if (IW == 4) return (int)id;
-    if (IW == 2) return c.tc_reg + (int)id;
+    if (IW == 4 || IW == 2) return base + (int)id;
     const int slot = (int)(id >> log2_of<B>()), col = (int)(id & (B - 1));
-    return __shfl(c.tc_reg, lanebase + slot, 64) * B + col;
+    return __shfl(base, lanebase + slot, 64) * B + col;
 }
+
+// Slab of a header word: byte offset and pattern flag (bit 0 of offset / 8).
+__device__ __forceinline__ size_t slab_off(int hx) { return (size_t)((unsigned)hx & ~1u) * 8; }
+__device__ __forceinline__ bool slab_pattern(int hx) { return hx & 1; }
 
 // One quad (four consecutive entries of one lane's row) of values and of
 // indices, loaded with one or two vector loads.
@@ -189,6 +202,7 @@ __device__ __forceinline__ RowCtx row_ctx_from(const DevL &L, int tr, int t0, in
     c.h01 = h01;
     c.h23 = h23;
     c.split = L.split ? (int)L.split[tr] : 1;
+    c.trB = tr * B;
     return c;
 }
 
@@ -209,8 +223,7 @@ __device__ __forceinline__ uint32_t gather_col(uint32_t id, int tcb, int t0, con
 {
     (void)t0;
     (void)L;
-    if (IW == 4) return id;
-    if (IW == 2) return (uint32_t)tcb + id;
+    if (IW == 4 || IW == 2) return (uint32_t)tcb + id;
     const uint32_t slot = id >> log2_of<B>(), col = id & (B - 1);
     return (uint32_t)__builtin_amdgcn_ds_bpermute((int)((lanebase + slot) << 2), tcb) + col;
 }
@@ -246,7 +259,7 @@ __device__ __forceinline__ int load_tcr(const DevL &L, const int *t0s, int q, in
 // (random SPD, 1M rows: FP64 CG 315 -> 218 ms).
 template <int B, int IW, int CLS, int NR, int QS, int UQL4, int UQ64, typename AccT, typename GT>
 __device__ __forceinline__ void slab_spmv_rows(const DevL &L, const int4 *hd, const int *t0s, int q0,
-                                               const int (&tcr)[NR], int r, int lanebase,
+                                               const int (&tcr)[NR], int row0, int r, int lanebase,
                                                const GT *__restrict__ g, AccT (&acc)[NR])
 {
     constexpr int NB = CLS == 0 && NR > 1 ? NR / 2 : NR;
@@ -266,13 +279,15 @@ __device__ __forceinline__ void slab_spmv_rows(const DevL &L, const int4 *hd, co
 #pragma unroll
             for (int k = 0; k < NB; k++) {
                 const int nq = (h[k].y + 3) >> 2;
-                const uint8_t *vals = L.blob + (size_t)(unsigned)h[k].x * 8;
+                const uint8_t *vals = L.blob + slab_off(h[k].x);
                 const uint8_t *idx = vals + (size_t)nq * B * 4 * cls_width(CLS);
+                // a pattern slab's offset table is one lane's worth of index quads
+                const bool pat = IW != 1 && slab_pattern(h[k].x);
 #pragma unroll
                 for (int u = 0; u < UQ; u++) {
                     if (j0 + u < nq) {
                         v[k][u].load(vals, (size_t)(j0 + u) * B + r);
-                        id[k][u].load(idx, (size_t)(j0 + u) * B + r);
+                        id[k][u].load(idx, pat ? (size_t)(j0 + u) : (size_t)(j0 + u) * B + r);
                     } else {
                         v[k][u].zero();
                         id[k][u].zero();
@@ -283,11 +298,15 @@ __device__ __forceinline__ void slab_spmv_rows(const DevL &L, const int4 *hd, co
 #pragma unroll
             for (int k = 0; k < NB; k++) {
                 const int t0 = 0;
+                // index base: the lane's row for a pattern slab, else the tile row's
+                const int tcb = IW != 1 && slab_pattern(h[k].x)
+                                    ? (IW == 2 ? tcr[kb + k] + r : row0 + (kb + k) * QS * B)
+                                    : tcr[kb + k];
 #pragma unroll
                 for (int u = 0; u < UQ; u++)
                     static_for<4>([&](auto e) {
                         xg[k][u][e] =
-                            ld_elem(g, gather_col<B, IW>(id[k][u].template get<e>(), tcr[kb + k], t0, L, lanebase));
+                            ld_elem(g, gather_col<B, IW>(id[k][u].template get<e>(), tcb, t0, L, lanebase));
                     });
             }
             // padding entries (value 0) add +0 or -0 to a sum that starts at +0
@@ -320,10 +339,11 @@ __device__ __forceinline__ void tile_rows_spmv(const DevL &L, const int4 *hd, co
         a64[k] = 0.0;
         a32[k] = 0.0f;
     }
-    slab_spmv_rows<B, IW, 0, NR, QS, UQL4, UQ64>(L, hd, t0s, q0, tcr, r, lanebase, g64, a64);
-    slab_spmv_rows<B, IW, 1, NR, QS, UQL4, UQ64>(L, hd, t0s, q0, tcr, r, lanebase, g32, a32);
-    slab_spmv_rows<B, IW, 2, NR, QS, UQL4, UQ64>(L, hd, t0s, q0, tcr, r, lanebase, g32, a32);
-    slab_spmv_rows<B, IW, 3, NR, QS, UQL4, UQ64>(L, hd, t0s, q0, tcr, r, lanebase, g32, a32);
+    const int row0 = trq * B + r;
+    slab_spmv_rows<B, IW, 0, NR, QS, UQL4, UQ64>(L, hd, t0s, q0, tcr, row0, r, lanebase, g64, a64);
+    slab_spmv_rows<B, IW, 1, NR, QS, UQL4, UQ64>(L, hd, t0s, q0, tcr, row0, r, lanebase, g32, a32);
+    slab_spmv_rows<B, IW, 2, NR, QS, UQL4, UQ64>(L, hd, t0s, q0, tcr, row0, r, lanebase, g32, a32);
+    slab_spmv_rows<B, IW, 3, NR, QS, UQL4, UQ64>(L, hd, t0s, q0, tcr, row0, r, lanebase, g32, a32);
 #pragma unroll
     for (int k = 0; k < NR; k++) y[k] = a64[k] + (double)a32[k];
     if (L.split) {
@@ -342,17 +362,19 @@ __device__ __forceinline__ void slab_spmv_f64(const DevL &L, const RowCtx &c, in
                                               const double *const (&g)[NV], double (&acc)[NV])
 {
     const int Lc = h.y, nq = (Lc + 3) >> 2;
-    const uint8_t *vals = L.blob + (size_t)(unsigned)h.x * 8;
+    const uint8_t *vals = L.blob + slab_off(h.x);
     const uint8_t *idx = vals + (size_t)nq * B * 4 * cls_width(CLS);
+    const bool pat = IW != 1 && slab_pattern(h.x);
+    const int base = pat ? c.trB + r : c.tc_reg;
     for (int j = 0; j < nq; j++) {
         Quad<CLS> v;
         IdxQuad<IW> id;
         v.load(vals, (size_t)j * B + r);
-        id.load(idx, (size_t)j * B + r);
+        id.load(idx, pat ? (size_t)j : (size_t)j * B + r);
         static_for<4>([&](auto e) {
             if (j * 4 + e < Lc) {
                 const double a = (double)v.template get<e>();
-                const int gi = gather_index<B, IW>(id.template get<e>(), c, L, lanebase);
+                const int gi = gather_index<B, IW>(id.template get<e>(), base, lanebase);
 #pragma unroll
                 for (int q = 0; q < NV; q++) acc[q] = __builtin_fma(a, g[q][gi], acc[q]);
             }

@@ -35,6 +35,7 @@ struct Layout {
     bool has64 = false, haslow = false;
     double bytes_alg = 0.0;     // algorithmic bytes per SpMV
     double bytes_actual = 0.0;  // bytes the layout occupies (+ gather + y)
+    long long pattern_slabs = 0;  // (tile row, class) slabs in pattern form
     const uint8_t *split = nullptr;  // the matrix's split tile rows (not owned)
     DevL view(const int *tile_ptr) const { return DevL{tile_ptr, trh, stc, blob, split}; }
     void release();
@@ -104,8 +105,10 @@ struct mf_matrix_s {
 
 namespace mf {
 int build_reference_tiles(mf_matrix m);              // tiles.hip
-// want_iw: slab index width asked for (0: the matrix's own setting)
-int build_layout(mf_matrix m, Layout &L, double tau, int allow_mask, int want_iw = 0);  // tiles.hip
+// want_iw: slab index width asked for (0: the matrix's own setting);
+// pattern: pattern slabs where they fit (automatic index form only)
+int build_layout(mf_matrix m, Layout &L, double tau, int allow_mask, int want_iw = 0,
+                 bool pattern = true);  // tiles.hip
 // fits bit 0: the one-byte slot form fits, bit 1: the two-byte form fits
 int index_fits(mf_matrix m, int *fits);  // tiles.hip
 // the width used for a request (0 automatic: two bytes, else one, else four;

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <vector>
 
@@ -459,19 +460,107 @@ __global__ __launch_bounds__(256) void k_max_tiles(const int *__restrict__ tile_
     atomicMax(out + 1, d);
 }
 
+// ----------------------------------------------------------- pattern slabs
+// A class slab of a tile row becomes a pattern slab (one table of offsets
+// column - row shared by the B rows instead of an index per entry) when
+// every row's entries of the class are in strictly ascending column order,
+// the union of the offsets has at most PAT_MAX entries, every row + offset
+// of the union (and row + 0 for the table's padding) is a valid column, and
+// the slab gets smaller.
+constexpr int PAT_MAX = 64;
+
+// Union of the offsets of tile row tr's class-c entries, ascending, into U;
+// returns its length, or 0 when some row is not strictly ascending or the
+// union exceeds PAT_MAX.  lmax gets the longest row's entry count.
+template <int B>
+__device__ int pattern_union(int tr, int c, const int *__restrict__ tile_ptr, const int *__restrict__ tile_start,
+                             const int *__restrict__ tile_columnidx, const uint8_t *__restrict__ rowtmp,
+                             const uint8_t *__restrict__ bcol, const uint8_t *__restrict__ codes, int *U,
+                             int *lmax)
+{
+    int prev[B], len[B];
+    for (int r = 0; r < B; r++) {
+        prev[r] = INT_MIN;
+        len[r] = 0;
+    }
+    int n = 0;
+    bool ok = true;
+    for (int t = tile_ptr[tr]; t < tile_ptr[tr + 1]; t++) {
+        if (code_to_cls(codes ? codes[t] : 0) != c) continue;
+        const int cb = (tile_columnidx[t] - tr) * B;
+        for (int k = tile_start[t]; k < tile_start[t + 1]; k++) {
+            const int r = rowtmp[k], d = cb + bcol[k] - r;
+            len[r]++;
+            if (!ok) continue;
+            if (d <= prev[r]) { ok = false; continue; }
+            prev[r] = d;
+            int lo = 0, hi = n;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (U[mid] < d) lo = mid + 1; else hi = mid;
+            }
+            if (lo < n && U[lo] == d) continue;
+            if (n == PAT_MAX) { ok = false; continue; }
+            for (int q = n; q > lo; q--) U[q] = U[q - 1];
+            U[lo] = d;
+            n++;
+        }
+    }
+    int m = 0;
+    for (int r = 0; r < B; r++) m = max(m, len[r]);
+    *lmax = m;
+    return ok ? n : 0;
+}
+
+__host__ __device__ inline unsigned long long slab_bytes(int L, int w, int iw, int B)
+{
+    return (unsigned long long)((L + 3) >> 2) * B * 4 * (w + iw);
+}
+__host__ __device__ inline unsigned long long pattern_bytes(int L, int w, int iw, int B)
+{
+    const unsigned long long nq = (unsigned long long)((L + 3) >> 2);
+    return nq * B * 4 * w + ((nq * 4 * iw + 15) & ~15ull);
+}
+
+// pat[tr * 4 + c] = union length of a pattern slab, 0 for an indexed slab.
+template <int B>
+__global__ __launch_bounds__(256) void k_slab_pattern(const int *__restrict__ tile_ptr, int tilem, int ncolv,
+                                                      const int *__restrict__ tile_start,
+                                                      const int *__restrict__ tile_columnidx,
+                                                      const uint8_t *__restrict__ rowtmp,
+                                                      const uint8_t *__restrict__ bcol,
+                                                      const uint8_t *__restrict__ codes, int iw, int *pat)
+{
+    const int tr = blockIdx.x * 256 + threadIdx.x;
+    if (tr >= tilem) return;
+    int U[PAT_MAX];
+    for (int c = 0; c < 4; c++) {
+        int lmax = 0;
+        const int n = pattern_union<B>(tr, c, tile_ptr, tile_start, tile_columnidx, rowtmp, bcol, codes, U, &lmax);
+        bool ok = n > 0;
+        if (ok) {
+            const long long lo = (long long)tr * B + min(0, U[0]), hi = (long long)tr * B + (B - 1) + max(0, U[n - 1]);
+            ok = lo >= 0 && hi < ncolv;
+        }
+        ok = ok && pattern_bytes(n, cls_width(c), iw, B) < slab_bytes(lmax, cls_width(c), iw, B);
+        pat[(size_t)tr * 4 + c] = ok ? n : 0;
+    }
+}
+
 // Per tile row: row lengths per class -> slab sizes (8-byte units) and the
-// tile / nonzero counts per code.
+// tile / nonzero counts per code (stats[10]: pattern slabs).
 template <int B>
 __global__ __launch_bounds__(256) void k_slab_sizes(const int *__restrict__ tile_ptr, int tilem,
                                                     const int *__restrict__ tile_start,
                                                     const uint8_t *__restrict__ rowtmp,
                                                     const uint8_t *__restrict__ codes, int iw,
+                                                    const int *__restrict__ pat,
                                                     unsigned long long *units, unsigned long long *stats)
 {
     // tiles and nonzeros per code: per-thread counts, summed in LDS, then one
     // global add per block and code
-    __shared__ unsigned long long s_stats[10];
-    if (threadIdx.x < 10) s_stats[threadIdx.x] = 0ull;
+    __shared__ unsigned long long s_stats[11];
+    if (threadIdx.x < 11) s_stats[threadIdx.x] = 0ull;
     __syncthreads();
     const int tr = blockIdx.x * 256 + threadIdx.x;
     unsigned long long u = 0;
@@ -489,15 +578,19 @@ __global__ __launch_bounds__(256) void k_slab_sizes(const int *__restrict__ tile
         }
         for (int k = 0; k < 10; k++)
             if (cnt[k]) atomicAdd(&s_stats[k], cnt[k]);
+        unsigned long long npat = 0;
         for (int c = 0; c < 4; c++) {
             int Lc = 0;
             for (int r = 0; r < B; r++) Lc = max(Lc, len[c][r]);
-            u += (unsigned long long)((Lc + 3) >> 2) * B * 4 * (cls_width(c) + iw) / 8;
+            const int Lp = pat ? pat[(size_t)tr * 4 + c] : 0;
+            u += (Lp ? pattern_bytes(Lp, cls_width(c), iw, B) : slab_bytes(Lc, cls_width(c), iw, B)) / 8;
+            npat += Lp ? 1 : 0;
         }
+        if (npat) atomicAdd(&s_stats[10], npat);
         units[tr] = u;
     }
     __syncthreads();
-    if (threadIdx.x < 10 && s_stats[threadIdx.x]) atomicAdd(&stats[threadIdx.x], s_stats[threadIdx.x]);
+    if (threadIdx.x < 11 && s_stats[threadIdx.x]) atomicAdd(&stats[threadIdx.x], s_stats[threadIdx.x]);
 }
 
 
@@ -509,11 +602,13 @@ __global__ __launch_bounds__(256) void k_slab_pack(const int *__restrict__ tile_
                                                    const double *__restrict__ bval,
                                                    const uint8_t *__restrict__ bcol,
                                                    const uint8_t *__restrict__ codes, int iw,
+                                                   const int *__restrict__ pat,
                                                    const unsigned long long *__restrict__ off8,
                                                    int2 *trh, int *stc, uint8_t *blob, int *err)
 {
     const int tr = blockIdx.x * 256 + threadIdx.x;
     if (tr >= tilem) return;
+    int U[PAT_MAX];
     const int t0 = tile_ptr[tr], t1 = tile_ptr[tr + 1];
     int len[4][B];
     for (int c = 0; c < 4; c++)
@@ -528,9 +623,50 @@ __global__ __launch_bounds__(256) void k_slab_pack(const int *__restrict__ tile_
         int Lc = 0;
         for (int r = 0; r < B; r++) { Lc = max(Lc, len[c][r]); len[c][r] = 0; }
         if (o > 0xFFFFFFFFull) { atomicOr(err, 2); return; }
+        const int w = cls_width(c);
+        const int Lp = pat ? pat[(size_t)tr * 4 + c] : 0;
+        if (Lp) {
+            // pattern slab: the offset table, then each entry at its offset's position
+            int lmax = 0;
+            const int n = pattern_union<B>(tr, c, tile_ptr, tile_start, tile_columnidx, rowtmp, bcol, codes, U, &lmax);
+            if (n != Lp) { atomicOr(err, 4); return; }
+            trh[(size_t)tr * 4 + c] = make_int2((int)((unsigned)o | 1u), Lp);
+            const int nq = (Lp + 3) >> 2;
+            uint8_t *vals = blob + o * 8;
+            uint8_t *tab = vals + (size_t)nq * B * 4 * w;
+            for (int q = 0; q < nq * 4; q++) {
+                const int d = q < Lp ? U[q] : 0;
+                if (iw == 2) reinterpret_cast<int16_t *>(tab)[q] = (int16_t)d;
+                else reinterpret_cast<int32_t *>(tab)[q] = d;
+            }
+            for (int t = t0; t < t1; t++) {
+                if (code_to_cls(codes ? codes[t] : 0) != c) continue;
+                stc[t0 + slot] = tile_columnidx[t];
+                const int cb = (tile_columnidx[t] - tr) * B;
+                for (int k = tile_start[t]; k < tile_start[t + 1]; k++) {
+                    const int r = rowtmp[k], d = cb + bcol[k] - r;
+                    int lo = 0, hi = Lp;
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (U[mid] < d) lo = mid + 1; else hi = mid;
+                    }
+                    const size_t e = ((size_t)(lo >> 2) * B + r) * 4 + (lo & 3);
+                    const double v = bval[k];
+                    switch (c) {
+                    case 0: reinterpret_cast<double *>(vals)[e] = v; break;
+                    case 1: reinterpret_cast<float *>(vals)[e] = (float)v; break;
+                    case 2: reinterpret_cast<uint16_t *>(vals)[e] = f64_to_f16(v); break;
+                    default: vals[e] = f64_to_e4m3(v); break;
+                    }
+                }
+                slot++;
+            }
+            o += pattern_bytes(Lp, w, iw, B) / 8;
+            continue;
+        }
         trh[(size_t)tr * 4 + c] = make_int2((int)(unsigned)o, Lc);
         if (!Lc) continue;
-        const int w = cls_width(c), nq = (Lc + 3) >> 2;
+        const int nq = (Lc + 3) >> 2;
         uint8_t *vals = blob + o * 8;
         uint8_t *idx = vals + (size_t)nq * B * 4 * w;
         for (int t = t0; t < t1; t++) {
@@ -732,7 +868,7 @@ void Layout::release()
 }
 
 template <int B>
-static int build_layout_B(mf_matrix m, Layout &L, double tau, int allow_mask, int want)
+static int build_layout_B(mf_matrix m, Layout &L, double tau, int allow_mask, int want, bool pattern)
 {
     hipStream_t st = stream();
     L.release();
@@ -742,9 +878,9 @@ static int build_layout_B(mf_matrix m, Layout &L, double tau, int allow_mask, in
     int *err = nullptr;
     MF_HIP(hipMallocAsync((void **)&codes, (size_t)tilenum + 1, st));
     MF_HIP(hipMallocAsync((void **)&units, sizeof(unsigned long long) * (tilem + 1), st));
-    MF_HIP(hipMallocAsync((void **)&stats, sizeof(unsigned long long) * 10, st));
+    MF_HIP(hipMallocAsync((void **)&stats, sizeof(unsigned long long) * 11, st));
     MF_HIP(hipMallocAsync((void **)&err, sizeof(int), st));
-    MF_HIP(hipMemsetAsync(stats, 0, sizeof(unsigned long long) * 10, st));
+    MF_HIP(hipMemsetAsync(stats, 0, sizeof(unsigned long long) * 11, st));
     MF_HIP(hipMemsetAsync(err, 0, sizeof(int), st));
     MF_HIP(hipMemsetAsync(units, 0, sizeof(unsigned long long) * (tilem + 1), st));
     const unsigned nbt = (unsigned)((tilenum + 255) / 256), nbr = (unsigned)((tilem + 255) / 256);
@@ -757,11 +893,19 @@ static int build_layout_B(mf_matrix m, Layout &L, double tau, int allow_mask, in
     int rc = index_fits(m, &fits);
     if (rc) return rc;
     const int iw = resolve_index_width(want, fits);
-    k_slab_sizes<B><<<nbr, 256, 0, st>>>(m->d_tile_ptr, tilem, m->d_tile_start, m->d_rowtmp, codes, iw,
+    // pattern slabs: automatic index form only, two- or four-byte widths
+    int *pat = nullptr;
+    if (pattern && want == 0 && iw != 1 && tilem > 0) {
+        MF_HIP(hipMallocAsync((void **)&pat, sizeof(int) * 4 * (size_t)tilem, st));
+        k_slab_pattern<B><<<nbr, 256, 0, st>>>(m->d_tile_ptr, tilem, m->tilen * B, m->d_tile_start,
+                                               m->d_tile_columnidx, m->d_rowtmp, m->d_Tile_csr_Col, codes, iw, pat);
+        MF_HIP(hipGetLastError());
+    }
+    k_slab_sizes<B><<<nbr, 256, 0, st>>>(m->d_tile_ptr, tilem, m->d_tile_start, m->d_rowtmp, codes, iw, pat,
                                          units, stats);
     rc = scan_exclusive<unsigned long long>(units, units, tilem + 1, st);
     if (rc) return rc;
-    unsigned long long total8 = 0, hs[10];
+    unsigned long long total8 = 0, hs[11];
     MF_HIP(hipMemcpyAsync(&total8, units + tilem, sizeof(total8), hipMemcpyDeviceToHost, st));
     MF_HIP(hipMemcpyAsync(hs, stats, sizeof(hs), hipMemcpyDeviceToHost, st));
     MF_HIP(hipStreamSynchronize(st));
@@ -773,20 +917,22 @@ static int build_layout_B(mf_matrix m, Layout &L, double tau, int allow_mask, in
     MF_HIP(hipMemsetAsync(L.stc, 0, sizeof(int) * ((size_t)tilenum + 1), st));
     MF_HIP(hipMemsetAsync(L.trh, 0, sizeof(int2) * 4 * ((size_t)tilem + 1), st));
     k_slab_pack<B><<<nbr, 256, 0, st>>>(m->d_tile_ptr, tilem, m->d_tile_start, m->d_tile_columnidx,
-                                        m->d_rowtmp, m->d_Blockcsr_Val, m->d_Tile_csr_Col, codes, iw,
+                                        m->d_rowtmp, m->d_Blockcsr_Val, m->d_Tile_csr_Col, codes, iw, pat,
                                         units, L.trh, L.stc, L.blob, err);
     MF_HIP(hipGetLastError());
     int herr = 0;
     MF_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, st));
     MF_HIP(hipStreamSynchronize(st));
     MF_HIP(hipFree(codes));
+    if (pat) MF_HIP(hipFree(pat));
     MF_HIP(hipFree(units));
     MF_HIP(hipFree(stats));
     MF_HIP(hipFree(err));
     if (herr) {
         L.release();
-        return fail(MF_E_FORMAT, "layout larger than 32 GB");
+        return fail(MF_E_FORMAT, herr & 4 ? "pattern slab rebuilt differently" : "layout larger than 32 GB");
     }
+    L.pattern_slabs = (long long)hs[10];
     for (int c = 0; c < 5; c++) {
         L.tiles_per_code[c] = (long long)hs[c];
         L.nnz_per_code[c] = (long long)hs[5 + c];
@@ -815,15 +961,15 @@ static int build_layout_B(mf_matrix m, Layout &L, double tau, int allow_mask, in
     return MF_OK;
 }
 
-int build_layout(mf_matrix m, Layout &L, double tau, int allow_mask, int want_iw)
+int build_layout(mf_matrix m, Layout &L, double tau, int allow_mask, int want_iw, bool pattern)
 {
     double t0 = now_ms();
     int rc;
     if (!want_iw) want_iw = m->want_iw;
     switch (m->B) {
-    case 8: rc = build_layout_B<8>(m, L, tau, allow_mask, want_iw); break;
-    case 16: rc = build_layout_B<16>(m, L, tau, allow_mask, want_iw); break;
-    case 32: rc = build_layout_B<32>(m, L, tau, allow_mask, want_iw); break;
+    case 8: rc = build_layout_B<8>(m, L, tau, allow_mask, want_iw, pattern); break;
+    case 16: rc = build_layout_B<16>(m, L, tau, allow_mask, want_iw, pattern); break;
+    case 32: rc = build_layout_B<32>(m, L, tau, allow_mask, want_iw, pattern); break;
     default: return fail(MF_E_ARG, "tile size");
     }
     L.split = m->d_split;

@@ -98,9 +98,16 @@ def test_gpu_spmv_index_widths_bit_exact(gpu, orc, name, fits1, fits2):
     for w in (0, 1, 2, 4):
         A.set_index_width(w)
         A.layout(2.0 ** -11)
-        got = A.info()["index_width"]
-        print(f"case {name} width {w} -> {got}", flush=True)
+        info = A.info()
+        got = info["index_width"]
+        pats = (info["pattern_slabs_fp64"], info["pattern_slabs_mixed"])
+        print(f"case {name} width {w} -> {got}, pattern slabs {pats}", flush=True)
         assert got == _expected_width(w, fits1, fits2)
+        # pattern slabs (one offset table per stencil slab) only in the automatic form
+        if w != 0:
+            assert pats == (0, 0)
+        elif not name.startswith("random"):
+            assert pats[0] > 0 and pats[1] > 0
         assert np.array_equal(A.spmv(x, mixed=False), y64)
         assert np.array_equal(A.spmv(x, mixed=True), ymix)
     A.close()
@@ -109,12 +116,13 @@ def test_gpu_spmv_index_widths_bit_exact(gpu, orc, name, fits1, fits2):
 @pytest.mark.parametrize("method", ["cg", "bicgstab"])
 def test_gpu_solvers_index_widths_identical(gpu, orc, method):
     # same sums in the same order: bit-identical iterates for every index form
+    # width 0: the automatic form, with pattern slabs on these stencils
     if method == "cg":
         rp, ci, va = gpu.gen_poisson3d(24, 24, 24)
-        mth, widths = gpu.MF_CG, (1, 2, 4)
+        mth, widths = gpu.MF_CG, (0, 1, 2, 4)
     else:
         rp, ci, va = gpu.gen_convdiff3d(32, 10.0, 1.0, 0)
-        mth, widths = gpu.MF_BICGSTAB, (2, 4)
+        mth, widths = gpu.MF_BICGSTAB, (0, 2, 4)
     b = orc.csr_spmv(rp, ci, va, gpu.gen_uniform(5, len(rp) - 1, -1, 1))
     A = gpu.Matrix(rp, ci, va)
     ref = None

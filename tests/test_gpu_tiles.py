@@ -119,9 +119,10 @@ def test_gpu_layout_info(gpu):
 def test_gpu_memory_report(gpu, orc):
     """Memory-Cost figure of the reference format from the oracle's compact
     rows and magnitude bins, beside the slab layouts' bytes: the FP64 slabs
-    hold at least 8 + index bytes per nonzero, the E4M3 slabs of the Poisson
-    matrix (values 6 and -1 exact) at least 1 + index bytes and less than the
-    FP64 ones."""
+    hold at least 8 bytes per nonzero, the E4M3 slabs of the Poisson matrix
+    (values 6 and -1 exact) at least 1 byte and less than half the FP64 ones
+    (the stencil's slabs are pattern slabs: one offset table per slab, no
+    index per entry)."""
     rp, ci, va = gpu.gen_poisson3d(20, 20, 20)
     m = len(rp) - 1
     A = gpu.Matrix(rp, ci, va)
@@ -137,6 +138,11 @@ def test_gpu_memory_report(gpu, orc):
     iw = r["index_width"]
     print(r, flush=True)
     assert r["nnz_per_code"][2] == nnz  # every value exact in E4M3
-    assert r["layout_fp64_MB"] * MB >= nnz * (8 + iw) and r["layout_mixed_MB"] * MB >= nnz * (1 + iw)
+    assert r["layout_fp64_MB"] * MB >= nnz * 8 and r["layout_mixed_MB"] * MB >= nnz * 1
     assert r["layout_mixed_MB"] < 0.5 * r["layout_fp64_MB"]
+    # the same width asked for explicitly gives indexed slabs, which are larger
+    A.set_index_width(iw)
+    ri = A.memory_report(0.0)
+    assert ri["layout_fp64_MB"] * MB >= nnz * (8 + iw) and ri["layout_mixed_MB"] * MB >= nnz * (1 + iw)
+    assert r["layout_fp64_MB"] < ri["layout_fp64_MB"] and r["layout_mixed_MB"] < ri["layout_mixed_MB"]
     A.close()

@@ -187,6 +187,7 @@ def _configs(mf):
                                 convdiff,
                                 {"fp64": dict(precision=mf.MF_FP64),
                                  "fp32_tiles": dict(precision=mf.MF_MIXED, tau=2.0 ** -24, rr_factor=1e-2),
+                                 "fp16_tiles": dict(precision=mf.MF_MIXED, tau=2.0 ** -11, rr_factor=1e-2),
                                  "e4m3_tiles": dict(precision=mf.MF_MIXED, tau=2.0 ** -4, rr_factor=1e-2)}),
         "random_spd_1m_cg": ("random SPD n = 1,000,000 (Zipf(2) row lengths in [2, 2000], B + B^T, diagonal "
                              "1.05 x row abs-sum), CG to true relres 1e-10, b = A x*, x* ~ U(-1,1) seed 0", mf.MF_CG,

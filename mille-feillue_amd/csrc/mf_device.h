@@ -323,7 +323,9 @@ __device__ __forceinline__ void slab_spmv_rows(const DevL &L, const int4 *hd, co
 }
 
 // y of NR tile rows (local rows q0 + k * QS, global tile rows trq + k * QS,
-// headers in LDS), the same sums as tile_row_spmv_c.
+// headers in LDS).  Per row, slab by slab in class order: the FP64 tiles'
+// products summed in FP64, the FP32, FP16 and E4M3 tiles' in FP32 against
+// the FP32 copy of the vector; y = a64 + (double)a32.
 template <int B, int IW, int NR, int QS, int UQL4 = 2, int UQ64 = 4>
 __device__ __forceinline__ void tile_rows_spmv(const DevL &L, const int4 *hd, const int *t0s, int q0, int trq,
                                                int r, int lanebase, const double *__restrict__ g64,

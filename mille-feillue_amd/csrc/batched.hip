@@ -120,6 +120,8 @@ struct GroupSync {
                 long long spins = 0;
                 double acc[N];
                 for (;;) {
+                    // the abort word loads with the members' words (one latency a round)
+                    const int abw = ld_agent(A.ctl);
                     bool ok = true;
 #pragma unroll
                     for (int k = 0; k < N; k++) acc[k] = 0.0;
@@ -136,7 +138,7 @@ struct GroupSync {
                         }
                     }
                     if (__all(ok)) break;
-                    if (__builtin_amdgcn_readfirstlane(ld_agent(A.ctl))) { ab = 1; break; }
+                    if (__builtin_amdgcn_readfirstlane(abw)) { ab = 1; break; }
                     if (++spins > A.spin_limit) {
                         if (lane == 0) __hip_atomic_store(A.ctl, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         ab = 1;

@@ -167,20 +167,37 @@ struct Reducer {
             long long spins = 0;
             int ab = 0;
             for (;;) {
+                // one round: the abort word and every slot word of this lane
+                // (PQ workgroups per lane at a time, clamped addresses) are
+                // loaded before the first wait, so a round costs one memory
+                // latency rather than one per workgroup the lane reads
+                constexpr int PQ = 4;
+                const int abw = ld_agent(A.ctl);
                 bool ok = true;
 #pragma unroll
                 for (int k = 0; k < N; k++) acc[k] = 0.0;
-                for (unsigned g = lane; g < G; g += 64) {
+                for (unsigned g0 = lane; g0 < G; g0 += 64 * PQ) {
+                    unsigned long long w[PQ][N];
 #pragma unroll
-                    for (int k = 0; k < N; k++) {
-                        const unsigned long long w =
-                            __hip_atomic_load(W + (size_t)g * SLOT_WORDS + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        ok = ok && w != SLOT_EMPTY;
-                        acc[k] += __longlong_as_double((long long)w);
+                    for (int q = 0; q < PQ; q++) {
+                        const unsigned g = min(g0 + 64u * q, G - 1);
+#pragma unroll
+                        for (int k = 0; k < N; k++)
+                            w[q][k] = __hip_atomic_load(W + (size_t)g * SLOT_WORDS + k, __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT);
+                    }
+#pragma unroll
+                    for (int q = 0; q < PQ; q++) {
+                        if (g0 + 64u * q >= G) continue;
+#pragma unroll
+                        for (int k = 0; k < N; k++) {
+                            ok = ok && w[q][k] != SLOT_EMPTY;
+                            acc[k] += __longlong_as_double((long long)w[q][k]);
+                        }
                     }
                 }
                 if (__all(ok)) break;
-                if (__builtin_amdgcn_readfirstlane(ld_agent(A.ctl))) { ab = 1; break; }
+                if (__builtin_amdgcn_readfirstlane(abw)) { ab = 1; break; }
                 if (++spins > A.spin_limit) {
                     if (lane == 0) __hip_atomic_store(A.ctl, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     ab = 1;
@@ -239,9 +256,12 @@ struct Reducer {
             long long spins = 0;
             int ab = 0;
             for (;;) {
-                const bool ok = !f || ld_agent(f) >= nep;
+                // the flag and the abort word in one round
+                const unsigned fv = f ? ld_agent(f) : nep;
+                const int abw = ld_agent(A.ctl);
+                const bool ok = fv >= nep;
                 if (__all(ok)) break;
-                if (__builtin_amdgcn_readfirstlane(ld_agent(A.ctl))) { ab = 1; break; }
+                if (__builtin_amdgcn_readfirstlane(abw)) { ab = 1; break; }
                 if (++spins > A.spin_limit) {
                     if (lane == 0) __hip_atomic_store(A.ctl, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     ab = 1;

@@ -448,10 +448,10 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
                 ++repl;
                 if (!isfinite(t1[0])) { finish(MF_E_NONFINITE, iter, repl, t1[0], gamma); return; }
                 if (t1[0] <= tol2) { finish(MF_OK, iter, repl, t1[0], gamma); return; }
-            // a lossy layout whose rounded operator lost definiteness (or whose
-            // recurrences ran away) shows a true residual above the best one
-            if (A.use64 && t1[0] > DIVERGE_GROWTH * tbest) { finish(MF_E_DIVERGED, iter, repl, t1[0], gamma); return; }
-            tbest = fmin(tbest, t1[0]);
+                // a lossy layout whose rounded operator lost definiteness (or whose
+                // recurrences ran away) shows a true residual above the best one
+                if (A.use64 && t1[0] > DIVERGE_GROWTH * tbest) { finish(MF_E_DIVERGED, iter, repl, t1[0], gamma); return; }
+                tbest = fmin(tbest, t1[0]);
                 // reliable update: keep the direction p (s = A p was refreshed with
                 // it); when the recurrence had drifted by more than a factor 2 in
                 // norm, restart from the true residual
@@ -719,8 +719,8 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
                 ++repl;
                 if (!isfinite(t2[0])) { finish(MF_E_NONFINITE, iter, repl, t2[0], g2); return; }
                 if (t2[0] <= tol2) { finish(MF_OK, iter, repl, t2[0], g2); return; }
-            if (A.use64 && t2[0] > DIVERGE_GROWTH * tbest) { finish(MF_E_DIVERGED, iter, repl, t2[0], g2); return; }
-            tbest = fmin(tbest, t2[0]);
+                if (A.use64 && t2[0] > DIVERGE_GROWTH * tbest) { finish(MF_E_DIVERGED, iter, repl, t2[0], g2); return; }
+                tbest = fmin(tbest, t2[0]);
                 restart = t2[0] > 4.0 * g2;  // recurrence drifted: new direction p = r
                 ref2 = t2[0];
                 rho_new = t2[1];

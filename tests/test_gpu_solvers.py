@@ -65,6 +65,30 @@ def test_gpu_spmv_mixed_bit_exact(gpu, orc, tau):
         assert np.array_equal(A.spmv(x, mixed=False), t.spmv(x))
 
 
+@pytest.mark.parametrize("B", [8, 32])
+def test_gpu_spmv_tile_sizes_bit_exact(gpu, orc, B):
+    """8- and 32-row tiles: the FP64 and mixed SpMV equal the oracle's tile
+    SpMV of the same tile size bit for bit, with pattern slabs on the
+    stencils and indexed slabs on the random matrix."""
+    for name, (rp, ci, va) in (("convdiff27_18", gpu.gen_convdiff3d(18, 10.0, 1.0, 0)),
+                               ("poisson3d_20", gpu.gen_poisson3d(20, 20, 20)),
+                               ("random_spd_12k", gpu.gen_random_spd(12000, seed=3))):
+        m = len(rp) - 1
+        A = gpu.Matrix(rp, ci, va, B=B)
+        A.layout(2.0 ** -11)
+        info = A.info()
+        t = orc.tiles(B, m, m, rp, ci, va)
+        codes = t.classes(2.0 ** -11)
+        x = np.random.default_rng(6).standard_normal(m)
+        print(f"case {name} B={B} index width {info['index_width']} pattern slabs "
+              f"{(info['pattern_slabs_fp64'], info['pattern_slabs_mixed'])}", flush=True)
+        if not name.startswith("random"):
+            assert info["pattern_slabs_fp64"] > 0 and info["pattern_slabs_mixed"] > 0
+        assert np.array_equal(A.spmv(x, mixed=False), t.spmv(x))
+        assert np.array_equal(A.spmv(x, mixed=True), t.spmv_mixed(codes, x))
+        A.close()
+
+
 def _index_width_cases(gpu):
     # B = 16: Poisson 24^3 fits the one-byte slot form (<= 16 tiles per tile
     # row) and the two-byte form; the 27-point stencil on 32^3 (18 tiles per

@@ -322,11 +322,17 @@ __device__ __forceinline__ void slab_spmv_rows(const DevL &L, const int4 *hd, co
     }
 }
 
+// Class masks of a layout: bit c set when it holds class-c tiles (0 FP64,
+// 1 FP32, 2 FP16, 3 E4M3).  Kernels built for a mask compile only those
+// classes' slab loops: fewer live registers in the SpMV.
+constexpr int CM_ALL = 0xF, CM_F64 = 0x1, CM_LOW = 0xE;
+
 // y of NR tile rows (local rows q0 + k * QS, global tile rows trq + k * QS,
 // headers in LDS).  Per row, slab by slab in class order: the FP64 tiles'
 // products summed in FP64, the FP32, FP16 and E4M3 tiles' in FP32 against
-// the FP32 copy of the vector; y = a64 + (double)a32.
-template <int B, int IW, int NR, int QS, int UQL4 = 2, int UQ64 = 4>
+// the FP32 copy of the vector; y = a64 + (double)a32.  Only the classes in
+// CM are read (the layout holds no others).
+template <int B, int IW, int NR, int QS, int UQL4 = 2, int UQ64 = 4, int CM = CM_ALL>
 __device__ __forceinline__ void tile_rows_spmv(const DevL &L, const int4 *hd, const int *t0s, int q0, int trq,
                                                int r, int lanebase, const double *__restrict__ g64,
                                                const float *__restrict__ g32, double (&y)[NR])
@@ -342,10 +348,10 @@ __device__ __forceinline__ void tile_rows_spmv(const DevL &L, const int4 *hd, co
         a32[k] = 0.0f;
     }
     const int row0 = trq * B + r;
-    slab_spmv_rows<B, IW, 0, NR, QS, UQL4, UQ64>(L, hd, t0s, q0, tcr, row0, r, lanebase, g64, a64);
-    slab_spmv_rows<B, IW, 1, NR, QS, UQL4, UQ64>(L, hd, t0s, q0, tcr, row0, r, lanebase, g32, a32);
-    slab_spmv_rows<B, IW, 2, NR, QS, UQL4, UQ64>(L, hd, t0s, q0, tcr, row0, r, lanebase, g32, a32);
-    slab_spmv_rows<B, IW, 3, NR, QS, UQL4, UQ64>(L, hd, t0s, q0, tcr, row0, r, lanebase, g32, a32);
+    if constexpr (CM & 1) slab_spmv_rows<B, IW, 0, NR, QS, UQL4, UQ64>(L, hd, t0s, q0, tcr, row0, r, lanebase, g64, a64);
+    if constexpr (CM & 2) slab_spmv_rows<B, IW, 1, NR, QS, UQL4, UQ64>(L, hd, t0s, q0, tcr, row0, r, lanebase, g32, a32);
+    if constexpr (CM & 4) slab_spmv_rows<B, IW, 2, NR, QS, UQL4, UQ64>(L, hd, t0s, q0, tcr, row0, r, lanebase, g32, a32);
+    if constexpr (CM & 8) slab_spmv_rows<B, IW, 3, NR, QS, UQL4, UQ64>(L, hd, t0s, q0, tcr, row0, r, lanebase, g32, a32);
 #pragma unroll
     for (int k = 0; k < NR; k++) y[k] = a64[k] + (double)a32[k];
     if (L.split) {

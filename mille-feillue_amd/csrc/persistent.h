@@ -310,7 +310,7 @@ __device__ __forceinline__ int opaque(int v)
 // rather than recurred (Chronopoulos-Gear's (A r, r) - beta (r, r) / alpha
 // form), which holds after a residual replacement that keeps the direction
 // and, with lossy tiles, halves the iterations on rough coefficients.
-template <int B, int IW, int TPB, int RPT>
+template <int B, int IW, int TPB, int RPT, int CM = CM_ALL>
 __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
 {
     __shared__ double s_r[RPT * TPB];
@@ -471,7 +471,7 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
                 // four-byte indices: four quads of a row in flight up to 8 rows per
                 // thread (at 16 the registers spill; BiCGSTAB's allow four only
                 // for FP64)
-                tile_rows_spmv<B, IW, NR, GPB, RPT <= 8 ? 4 : 2, RPT <= 8 ? 4 : 1>(
+                tile_rows_spmv<B, IW, NR, GPB, RPT <= 8 ? 4 : 2, RPT <= 8 ? 4 : 1, CM>(
                     A.L, s_h, s_t0, i0 * GPB + grp, tile_row(i0), lane, lanebase, g64, g32, w);
     #pragma unroll
                 for (int k = 0; k < NR; k++) {
@@ -530,7 +530,7 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
 }
 
 // BiCGSTAB: registers hold x, r, r^, p, s; LDS holds v = A p and t = A s.
-template <int B, int IW, int TPB, int RPT>
+template <int B, int IW, int TPB, int RPT, int CM = CM_ALL>
 __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
 {
     __shared__ double s_v[RPT * TPB];
@@ -606,7 +606,8 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
     #pragma unroll 1
             for (int i0 = 0; i0 < RPT; i0 += NR) {
                 double w[NR];
-                tile_rows_spmv<B, IW, NR, GPB>(A.L, s_h, s_t0, i0 * GPB + grp, tile_row(i0), lane, lanebase, g64, g32, w);
+                tile_rows_spmv<B, IW, NR, GPB, 2, 4, CM>(A.L, s_h, s_t0, i0 * GPB + grp, tile_row(i0), lane, lanebase,
+                                                         g64, g32, w);
     #pragma unroll
                 for (int k = 0; k < NR; k++)
                     if (tile_row(i0 + k) < tr1) dst[(i0 + k) * TPB + tid] = w[k];
@@ -1152,9 +1153,42 @@ KernelChoice pick_kernel(int method, long long rows_per_wg, bool stream)
     return k;
 }
 
-// tile sizes instantiated in their own translation units
+// Resident kernels built for a layout's class mask (CM_F64: FP64 tiles only;
+// CM_LOW: no FP64 tiles) at 4 or more rows per thread, where the SpMV's live
+// registers bind (16-row tiles, 128^3 Poisson CG at 16 rows per thread: 256
+// VGPRs and 14 spilled with every class, 207 or 252 and none with one of
+// these masks).  An empty choice leaves the pick to pick_kernel.
+template <int B, int IW, int CM>
+KernelChoice pick_kernel_cm(int method, long long rows_per_wg)
+{
+    KernelChoice k;
+    const int q = (int)((rows_per_wg + 511) / 512);
+    if (q <= 2) return k;
+    if (method == MF_CG) {
+        if (q <= 4) k = {k_cg<B, IW, 512, 4, CM>, 512, 4};
+        else if (q <= 8) k = {k_cg<B, IW, 512, 8, CM>, 512, 8};
+        else if constexpr (B >= 16) {
+            if (q <= 16) k = {k_cg<B, IW, 512, 16, CM>, 512, 16};
+        }
+    } else {
+        if (q <= 4) k = {k_bicgstab<B, IW, 512, 4, CM>, 512, 4};
+        else if (q <= 8) k = {k_bicgstab<B, IW, 512, 8, CM>, 512, 8};
+    }
+    return k;
+}
+
+template <int B, int CM> KernelChoice pick_tile_cm(int iw, int method, long long rows_per_wg)
+{
+    return iw == 1   ? pick_kernel_cm<B, 1, CM>(method, rows_per_wg)
+           : iw == 2 ? pick_kernel_cm<B, 2, CM>(method, rows_per_wg)
+                     : pick_kernel_cm<B, 4, CM>(method, rows_per_wg);
+}
+
+// tile sizes and class masks instantiated in their own translation units
 KernelChoice pick_b8(int iw, int method, long long rows_per_wg, bool stream);   // solver_b8.hip
 KernelChoice pick_b32(int iw, int method, long long rows_per_wg, bool stream);  // solver_b32.hip
+KernelChoice pick_b16_f64(int iw, int method, long long rows_per_wg);           // solver_f64.hip
+KernelChoice pick_b16_low(int iw, int method, long long rows_per_wg);           // solver_low.hip
 
 template <int B> KernelChoice pick_tile(int iw, int method, long long rows_per_wg, bool stream)
 {

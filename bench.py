@@ -451,10 +451,15 @@ def main():
         # calibrated FETCH_SIZE + WRITE_SIZE of the mixed launch, per iteration
         traffic = round(pmc["mixed"]["traffic_bytes_per_iteration"] * itmx)
     iw = info["index_width"]
+    # class mask of the kernel the mixed layout selects: 1 FP64 tiles only,
+    # 14 no FP64 tiles, 15 both (codes: 0 FP64, 4 FP32, 3 FP16, 2 E4M3)
+    npc = info["nnz_per_code"]
+    cm = 1 if sum(npc[2:]) == 0 else (14 if npc[0] == 0 else 15)
     roof = _roofline(itmx, 1, bmx, msmx)
     roof.update({
         "traffic": traffic,
-        "kernel": f"k_cg<16,{iw},512,16> (persistent mixed CG, {iw}-byte slab indices, one launch per solve)",
+        "kernel": f"k_cg<16,{iw},512,{repsmx[0]['rows_per_thread']},{cm}> (persistent mixed CG, {iw}-byte slab "
+                  f"indices, class mask {cm}, one launch per solve)",
         "unit_def": "one SpMV at B_alg (SpMV algorithmic bytes of the mixed layout); one per CG iteration",
         "launch_ms": round(msmx, 4),
         "per_step_GBps": round(itmx * bmx / (wallmx * 1e-3) / 1e9, 1),

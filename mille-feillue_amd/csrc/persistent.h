@@ -13,7 +13,7 @@
 namespace mf {
 
 // --------------------------------------------------------- solver kernels
-constexpr int SYNC_K = 4;  // values per grid allreduce, at most
+constexpr int SYNC_K = 5;  // values per grid allreduce, at most
 constexpr int LDS_GATHER = 2048;  // rows of a one-workgroup solve gathered from LDS
 // A true-residual pass on a lossy layout whose squared norm exceeds the best
 // earlier one by this factor (the norm grew 4x while the recurrence claimed
@@ -529,16 +529,31 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
     solve(A.g64, A.g32, std::false_type{});
 }
 
+// rho = (r^, r) and (r, r) of r = s - omega t from the reduced dots
+// d = {(t, s), (t, t), (r^, s), (r^, t), (s, s)}; (r, r) is clamped at 0
+// against rounding.
+__device__ __forceinline__ void next_rho_g2(const double (&d)[5], double omega, double &rho, double &g2)
+{
+    rho = d[2] - omega * d[3];
+    g2 = fmax(0.0, d[4] - omega * (2.0 * d[0] - omega * d[1]));
+}
+
 // BiCGSTAB: registers hold x, r, r^, p, s; LDS holds v = A p and t = A s.
+// Four syncs per iteration: the reduction after t = A s carries (r^, s),
+// (r^, t) and (s, s) besides (t, s) and (t, t), so that rho and (r, r) of the
+// new iterate follow as (r^, s) - omega (r^, t) and (s, s) - 2 omega (t, s) +
+// omega^2 (t, t) -- the values of (r^, r) and (r, r) in exact arithmetic --
+// without a reduction of their own; s and p are published with neighbour
+// waits.
 template <int B, int IW, int TPB, int RPT, int CM = CM_ALL>
 __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
 {
     __shared__ double s_v[RPT * TPB];
     __shared__ double s_t[RPT * TPB];
-    __shared__ double s_red[(TPB / 64) * 4];
-    __shared__ double s_bc[4];
+    __shared__ double s_red[(TPB / 64) * 5];
+    __shared__ double s_bc[5];
     __shared__ int s_flag;
-    Reducer<TPB, 4> R{s_red, s_bc, &s_flag};
+    Reducer<TPB, 5> R{s_red, s_bc, &s_flag};
     constexpr int GPB = TPB / B, NR = spmv_rows(RPT, IW);
     const int tid = threadIdx.x;
     const int chunk = xcd_chunk(blockIdx.x, gridDim.x);
@@ -658,30 +673,28 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
             clk.lap(3);
             spmv_into(s_t);  // t = A s
             clk.lap(0);
-            double a2[2] = {0.0, 0.0};
+            double a2[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     #pragma clang loop unroll(full)
             for (int i = 0; i < RPT; i++)
                 if (tile_row(i) < tr1) {
                     const double t = s_t[i * TPB + tid];
                     a2[0] += t * sr[i];
                     a2[1] += t * t;
+                    a2[2] += rh[i] * sr[i];
+                    a2[3] += rh[i] * t;
+                    a2[4] += sr[i] * sr[i];
                 }
             if (!R.allreduce(A, a2, false)) return;
             clk.lap(1);
             const double omega = a2[1] > 0.0 ? a2[0] / a2[1] : 0.0;
-            double a3[2] = {0.0, 0.0};
     #pragma clang loop unroll(full)
             for (int i = 0; i < RPT; i++)
                 if (tile_row(i) < tr1) {
                     xr[i] = xr[i] + alpha * pr[i] + omega * sr[i];
                     rr[i] = sr[i] - omega * s_t[i * TPB + tid];
-                    a3[0] += rr[i] * rh[i];
-                    a3[1] += rr[i] * rr[i];
                 }
-            clk.lap(2);
-            if (!R.allreduce(A, a3, false)) return;
-            clk.lap(3);
-            double rho_new = a3[0], g2 = a3[1];
+            double rho_new, g2;
+            next_rho_g2(a2, omega, rho_new, g2);
             ++iter;
             if (!finite_all(rho_new, g2)) { finish(MF_E_NONFINITE, iter, repl, -1.0, g2); return; }
             const bool conv = g2 <= tol2;
@@ -980,10 +993,10 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab_st(SolveArgs A)
     using Geo = StreamGeom<B, IW, TPB>;
     __shared__ int4 s_h[2 * Geo::HW];
     __shared__ int s_t0[Geo::HW + 1];
-    __shared__ double s_red[(TPB / 64) * 4];
-    __shared__ double s_bc[4];
+    __shared__ double s_red[(TPB / 64) * 5];
+    __shared__ double s_bc[5];
     __shared__ int s_flag;
-    Reducer<TPB, 4> R{s_red, s_bc, &s_flag};
+    Reducer<TPB, 5> R{s_red, s_bc, &s_flag};
     const int tid = threadIdx.x;
     const int chunk = xcd_chunk(blockIdx.x, gridDim.x);
     const int tr0 = chunk * A.trw, tr1 = min(A.tilem, tr0 + A.trw);
@@ -1044,30 +1057,28 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab_st(SolveArgs A)
         clk.lap(2);
         if (!R.published(A, chunk)) return;
         clk.lap(3);
-        // t = A s; (t, s), (t, t)
-        double a2[2] = {0.0, 0.0};
+        // t = A s; (t, s), (t, t), (r^, s), (r^, t), (s, s)
+        double a2[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         stream_spmv<B, IW, TPB>(A, s_h, s_t0, hw0, tr0, tr1, A.g64, A.g32, [&](size_t row, double t) {
             vt[row] = t;
-            a2[0] += t * vs[row];
+            const double s = vs[row], rh = vrh[row];
+            a2[0] += t * s;
             a2[1] += t * t;
+            a2[2] += rh * s;
+            a2[3] += rh * t;
+            a2[4] += s * s;
         });
         clk.lap(0);
         if (!R.allreduce(A, a2, false)) return;
         clk.lap(1);
         const double omega = a2[1] > 0.0 ? a2[0] / a2[1] : 0.0;
-        double a3[2] = {0.0, 0.0};
         for (size_t row = row0; row < row1; row += TPB) {
             const double s = vs[row];
             x[row] = x[row] + alpha * vp[row] + omega * s;
-            const double r = s - omega * vt[row];
-            vr[row] = r;
-            a3[0] += r * vrh[row];
-            a3[1] += r * r;
+            vr[row] = s - omega * vt[row];
         }
-        clk.lap(2);
-        if (!R.allreduce(A, a3, false)) return;
-        clk.lap(3);
-        double rho_new = a3[0], g2 = a3[1];
+        double rho_new, g2;
+        next_rho_g2(a2, omega, rho_new, g2);
         ++iter;
         if (!finite_all(rho_new, g2)) { finish(MF_E_NONFINITE, iter, repl, -1.0, g2); return; }
         const bool conv = g2 <= tol2;

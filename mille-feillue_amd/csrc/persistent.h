@@ -504,7 +504,7 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
             for (int i = 0; i < RPT; i++)
                 if (tile_row(i) < tr1) xr[i] = xr[i] + alpha * pr[i];
             double g1[1] = {0.0};
-    #pragma unroll 1
+    #pragma clang loop unroll(full)
             for (int i = 0; i < RPT; i++)
                 if (tile_row(i) < tr1) {
                     const double rv = s_r[i * TPB + tid] - alpha * s_s[i * TPB + tid];

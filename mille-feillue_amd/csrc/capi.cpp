@@ -153,6 +153,7 @@ int mf_matrix_create(int rowA, int colA, int nnz, const int *rowptr, const int *
     // host-side bounds check: every index the kernels will derive stays inside
     // the arrays they read
     int bad = 0;
+    omp_quiet();
 #pragma omp parallel for reduction(| : bad)
     for (int i = 0; i < rowA; i++)
         if (rowptr[i + 1] < rowptr[i]) bad |= 1;

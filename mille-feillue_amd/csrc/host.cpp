@@ -35,6 +35,8 @@ double now_ms()
     return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
 }
 
+void omp_quiet() { kmp_set_blocktime(0); }
+
 // ---------------------------------------------------------------- Matrix Market
 // Semantics of PC/mmio_highlevel.h:17-593 with the banner rules of
 // PC/mmio.h:398-590: the data type picks how an entry line is read (real /
@@ -100,6 +102,7 @@ int mm_parse(const char *path, MMData *out)
     }
     if (m < 0 || n < 0 || nz < 0) return fail(MF_E_SIZE, "negative size");
     // split the entry region into chunks at line starts
+    omp_quiet();
     int nth = std::max(1, std::min(omp_get_max_threads(), 64));
     std::vector<const char *> cut(nth + 1);
     cut[0] = p;

@@ -38,4 +38,11 @@ int csv_append_bicg(const char *path, const char *matrix, int itr, double time_s
 
 double now_ms();
 
+// Called before each OpenMP loop of the library (host.cpp, built with
+// -fopenmp): the calling thread's workers sleep as soon as a loop ends
+// instead of spinning for the runtime's default 200 ms.  Spinning workers,
+// with another busy thread pool in the process (numpy's BLAS), were measured
+// delaying the completion of the next GPU solves by ~2.6 ms each.
+void omp_quiet();
+
 }  // namespace mf

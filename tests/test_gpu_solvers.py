@@ -264,6 +264,44 @@ def test_gpu_bicgstab_fp64(gpu, orc):
         _bicg_case(gpu, orc, g["rowptr"], g["col"], g["val"], b)
 
 
+def _diag_csr(d):
+    n = len(d)
+    return np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32), np.asarray(d, np.float64)
+
+
+def test_gpu_bicgstab_finite_termination(gpu, orc):
+    """BiCGSTAB on diagonal systems, where the mathematics fixes the answer.
+    A = 2I: alpha = (b, b) / (b, 2b) = 1/2 exactly, s = b - alpha 2b = 0
+    exactly, so t = 0 and omega = 0 (the zero-denominator branch), the new
+    (r, r) from the reduced dots is 0, and the solve ends after one iteration
+    with x = b / 2 bit for bit.  Two distinct eigenvalues: BiCG's polynomial
+    has degree 2, so the solve ends within two iterations (three with
+    rounding), as the oracle's does."""
+    n = 4096
+    b = gpu.gen_uniform(3, n, -1.0, 1.0)
+    rp, ci, va = _diag_csr(np.full(n, 2.0))
+    A = gpu.Matrix(rp, ci, va)
+    x, rep = A.solve(b, method=gpu.MF_BICGSTAB, tol=TOL)
+    xo, orep = orc.bicgstab(rp, ci, va, b, TOL, 100)
+    print(f"2I: gpu it={rep['iterations']} oracle it={orep.iterations}", flush=True)
+    assert rep["status"] == 0 and rep["iterations"] == 1 and orep.iterations == 1
+    assert np.array_equal(x, b / 2) and np.array_equal(xo, b / 2)
+    A.close()
+    d = np.where(np.arange(n) % 3 == 0, 1.0, 3.0)
+    rp, ci, va = _diag_csr(d)
+    A = gpu.Matrix(rp, ci, va)
+    for prec in (gpu.MF_FP64, gpu.MF_MIXED):
+        x, rep = A.solve(b, method=gpu.MF_BICGSTAB, tol=TOL, precision=prec, rr_factor=1e-2)
+        print(f"two eigenvalues prec {prec}: gpu it={rep['iterations']} relres={rep['relres_true']:.2e}", flush=True)
+        assert rep["status"] == 0 and rep["relres_true"] <= TOL
+        assert np.linalg.norm(x - b / d) / np.linalg.norm(b / d) < 1e-9
+        if prec == gpu.MF_FP64:
+            assert rep["iterations"] <= 3
+    xo, orep = orc.bicgstab(rp, ci, va, b, TOL, 100)
+    assert orep.converged and orep.iterations <= 3
+    A.close()
+
+
 @pytest.mark.parametrize("tau,outer", [(2.0 ** -24, 0), (2.0 ** -11, 0), (2.0 ** -11, 8)])
 def test_gpu_bicgstab_mixed_lossy(gpu, orc, tau, outer):
     # lossy tiles: reliable updates on the FP64 layout inside the launch

@@ -23,6 +23,7 @@ constexpr double DIVERGE_GROWTH = 16.0;
 
 struct SolveArgs {
     DevL L;
+    unsigned long long blob_bytes;  // bytes of L's slab blob
     DevL L64;          // FP64 layout for the true-residual passes of a lossy layout
     int use64;         // 1: passes use L64 (headers read from L64), else L
     int tilem, G, trw;
@@ -302,6 +303,59 @@ __device__ __forceinline__ int opaque(int v)
     return v;
 }
 
+// Slab bytes a resident kernel keeps in LDS per workgroup (0: none).  The
+// LDS left beside the kernels' vectors, headers and gathered-vector copy at
+// 512 threads and 1, 2 or 4 rows per thread.
+constexpr int lds_slab_bytes(int rpt) { return rpt == 1 ? 112 * 1024 : rpt == 2 ? 96 * 1024 : rpt == 4 ? 64 * 1024 : 0; }
+
+// Stages the headers of tile rows tr0 .. tr0 + TRW - 1 of a chunk (zeros
+// from tr1 on) and their first tiles.  When the chunk's slabs fit in LSB
+// bytes they are copied to s_slab and the staged offsets are rebased onto
+// that copy (returns true).  A chunk's slabs are one byte range of the blob:
+// slabs follow tile-row order, and class 0's header of every tile row holds
+// the row's start, empty or not, so the range runs from tile row tr0's start
+// to tile row tr1's (the blob's end after the last tile row).  Offsets are
+// multiples of 16 bytes, which keeps the copy's quads aligned.  Every thread
+// calls it; a __syncthreads must follow.
+template <int TPB>
+__device__ bool stage_chunk(const SolveArgs &A, int tr0, int tr1, int TRW, int4 *s_h, int *s_t0, uint8_t *s_slab,
+                            int LSB)
+{
+    const int tid = threadIdx.x;
+    size_t lo = 0;
+    bool lds = false;
+    if (LSB > 0 && tr0 < tr1) {
+        lo = slab_off(A.L.trh[(size_t)tr0 * 4].x);
+        const size_t hi = tr1 < A.tilem ? slab_off(A.L.trh[(size_t)tr1 * 4].x) : A.blob_bytes;
+        lds = hi - lo <= (size_t)LSB;
+        if (lds) {
+            const uint4 *src = reinterpret_cast<const uint4 *>(A.L.blob + lo);
+            uint4 *dst = reinterpret_cast<uint4 *>(s_slab);
+            for (size_t i = tid; i < (hi - lo) / 16; i += TPB) dst[i] = src[i];
+        }
+    }
+    const int d = (int)(lo / 8);  // in the headers' 8-byte units (even: bit 0 is the pattern flag)
+    for (int q = tid; q <= TRW; q += TPB) {
+        const int tr = tr0 + q;
+        if (q < TRW && tr < tr1) {
+            const int4 *h = reinterpret_cast<const int4 *>(A.L.trh + (size_t)tr * 4);
+            int4 a = h[0], b = h[1];
+            if (lds) {
+                a.x -= d;
+                a.z -= d;
+                b.x -= d;
+                b.z -= d;
+            }
+            s_h[2 * q] = a;
+            s_h[2 * q + 1] = b;
+        } else if (q < TRW) {  // beyond the chunk: no slabs
+            s_h[2 * q] = s_h[2 * q + 1] = make_int4(0, 0, 0, 0);
+        }
+        s_t0[q] = A.L.tile_ptr[min(tr, A.tilem)];
+    }
+    return lds;
+}
+
 // CG (Hestenes-Stiefel with s = A p carried by recurrence): registers hold
 // x and p of the thread's rows, LDS holds r and s.  Per iteration: [r
 // published, (r, r) reduced] -> convergence test -> w = A r, s = w + beta s,
@@ -330,17 +384,10 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
     constexpr int TRW = RPT * GPB;
     __shared__ int4 s_h[2 * TRW];
     __shared__ int s_t0[TRW + 1];
-    for (int q = tid; q <= TRW; q += TPB) {
-        const int tr = tr0 + q;
-        if (q < TRW && tr < tr1) {
-            const int4 *h = reinterpret_cast<const int4 *>(A.L.trh + (size_t)tr * 4);
-            s_h[2 * q] = h[0];
-            s_h[2 * q + 1] = h[1];
-        } else if (q < TRW) {  // beyond the chunk: no slabs
-            s_h[2 * q] = s_h[2 * q + 1] = make_int4(0, 0, 0, 0);
-        }
-        s_t0[q] = A.L.tile_ptr[min(tr, A.tilem)];
-    }
+    // small chunks keep their slabs in LDS too
+    constexpr int LSB = lds_slab_bytes(RPT);
+    __shared__ __align__(16) uint8_t s_slab[LSB > 0 ? LSB : 16];
+    const bool lds_s = stage_chunk<TPB>(A, tr0, tr1, TRW, s_h, s_t0, s_slab, LSB);
     // one workgroup with the whole gathered vector in LDS (up to LDS_GATHER
     // rows): the SpMV reads published entries there, with no global stores
     // to drain and no cache round trips
@@ -357,10 +404,14 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
     int rbase = tr0 * B + tid;  // row of step i is rbase + i * TPB
     auto tile_row = [&](int i) { return tr0 + i * GPB + grp; };
     auto row_of = [&](int i) { return (size_t)(rbase + i * TPB); };
-    // the solve, on the gathered vector in LDS or in global memory (two
-    // inlined copies, each with its address space known to the compiler)
-    auto solve = [&](double *g64, float *g32, auto lds_tag) {
+    // the solve, with the gathered vector and the slabs each in LDS or in
+    // global memory (up to four inlined copies, each with its address spaces
+    // known to the compiler)
+    auto solve = [&](double *g64, float *g32, auto lds_tag, auto slab_tag) {
         constexpr bool lds = decltype(lds_tag)::value;
+        // the layout as the SpMV and the passes read it: slabs in LDS or HBM
+        DevL Lw = A.L;
+        if constexpr (decltype(slab_tag)::value) Lw.blob = s_slab;
         auto publish = [&](size_t row, double v) {
             if (lds) {
                 if (A.has64) g64[row] = v;
@@ -433,7 +484,7 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
                             tile_row_spmv_f64_c<B, IW, 2>(A.L64, row_ctx<B, IW>(A.L64, tr, lane), lane, lanebase, g, acc);
                         else
                             tile_row_spmv_f64_c<B, IW, 2>(
-                                A.L, row_ctx_from<B, IW>(A.L, tr, s_t0[q], s_t0[q + 1] - s_t0[q], s_h[2 * q], s_h[2 * q + 1], lane),
+                                Lw, row_ctx_from<B, IW>(Lw, tr, s_t0[q], s_t0[q + 1] - s_t0[q], s_h[2 * q], s_h[2 * q + 1], lane),
                                 lane, lanebase, g, acc);
                         const size_t row = (size_t)tr * B + lane;
                         const double rv = A.b[row] - acc[0];
@@ -472,7 +523,7 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
                 // thread (at 16 the registers spill; BiCGSTAB's allow four only
                 // for FP64)
                 tile_rows_spmv<B, IW, NR, GPB, RPT <= 8 ? 4 : 2, RPT <= 8 ? 4 : 1, CM>(
-                    A.L, s_h, s_t0, i0 * GPB + grp, tile_row(i0), lane, lanebase, g64, g32, w);
+                    Lw, s_h, s_t0, i0 * GPB + grp, tile_row(i0), lane, lanebase, g64, g32, w);
     #pragma unroll
                 for (int k = 0; k < NR; k++) {
                     const int i = i0 + k;
@@ -520,13 +571,20 @@ __global__ __launch_bounds__(TPB, 1) void k_cg(SolveArgs A)
             gamma = g1[0];
         }
     };
-    if constexpr (RPT <= 4) {
+    if constexpr (LSB > 0) {
         if (lds_g) {
-            solve(s_g64, s_g32, std::true_type{});
+            if (lds_s)
+                solve(s_g64, s_g32, std::true_type{}, std::true_type{});
+            else
+                solve(s_g64, s_g32, std::true_type{}, std::false_type{});
+            return;
+        }
+        if (lds_s) {
+            solve(A.g64, A.g32, std::false_type{}, std::true_type{});
             return;
         }
     }
-    solve(A.g64, A.g32, std::false_type{});
+    solve(A.g64, A.g32, std::false_type{}, std::false_type{});
 }
 
 // rho = (r^, r) and (r, r) of r = s - omega t from the reduced dots
@@ -565,17 +623,10 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
     constexpr int TRW = RPT * GPB;
     __shared__ int4 s_h[2 * TRW];
     __shared__ int s_t0[TRW + 1];
-    for (int q = tid; q <= TRW; q += TPB) {
-        const int tr = tr0 + q;
-        if (q < TRW && tr < tr1) {
-            const int4 *h = reinterpret_cast<const int4 *>(A.L.trh + (size_t)tr * 4);
-            s_h[2 * q] = h[0];
-            s_h[2 * q + 1] = h[1];
-        } else if (q < TRW) {  // beyond the chunk: no slabs
-            s_h[2 * q] = s_h[2 * q + 1] = make_int4(0, 0, 0, 0);
-        }
-        s_t0[q] = A.L.tile_ptr[min(tr, A.tilem)];
-    }
+    // small chunks keep their slabs in LDS too
+    constexpr int LSB = lds_slab_bytes(RPT);
+    __shared__ __align__(16) uint8_t s_slab[LSB > 0 ? LSB : 16];
+    const bool lds_s = stage_chunk<TPB>(A, tr0, tr1, TRW, s_h, s_t0, s_slab, LSB);
     // one workgroup with the whole gathered vector in LDS (up to LDS_GATHER
     // rows): the SpMV reads published entries there, with no global stores
     // to drain and no cache round trips
@@ -592,10 +643,14 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
     int rbase = tr0 * B + tid;  // row of step i is rbase + i * TPB
     auto tile_row = [&](int i) { return tr0 + i * GPB + grp; };
     auto row_of = [&](int i) { return (size_t)(rbase + i * TPB); };
-    // the solve, on the gathered vector in LDS or in global memory (two
-    // inlined copies, each with its address space known to the compiler)
-    auto solve = [&](double *g64, float *g32, auto lds_tag) {
+    // the solve, with the gathered vector and the slabs each in LDS or in
+    // global memory (up to four inlined copies, each with its address spaces
+    // known to the compiler)
+    auto solve = [&](double *g64, float *g32, auto lds_tag, auto slab_tag) {
         constexpr bool lds = decltype(lds_tag)::value;
+        // the layout as the SpMV and the passes read it: slabs in LDS or HBM
+        DevL Lw = A.L;
+        if constexpr (decltype(slab_tag)::value) Lw.blob = s_slab;
         auto publish = [&](size_t row, double v) {
             if (lds) {
                 if (A.has64) g64[row] = v;
@@ -621,7 +676,7 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
     #pragma unroll 1
             for (int i0 = 0; i0 < RPT; i0 += NR) {
                 double w[NR];
-                tile_rows_spmv<B, IW, NR, GPB, 2, 4, CM>(A.L, s_h, s_t0, i0 * GPB + grp, tile_row(i0), lane, lanebase,
+                tile_rows_spmv<B, IW, NR, GPB, 2, 4, CM>(Lw, s_h, s_t0, i0 * GPB + grp, tile_row(i0), lane, lanebase,
                                                          g64, g32, w);
     #pragma unroll
                 for (int k = 0; k < NR; k++)
@@ -715,7 +770,7 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
                             tile_row_spmv_f64_c<B, IW, 1>(A.L64, row_ctx<B, IW>(A.L64, tr, lane), lane, lanebase, g, acc);
                         else
                             tile_row_spmv_f64_c<B, IW, 1>(
-                                A.L, row_ctx_from<B, IW>(A.L, tr, s_t0[q], s_t0[q + 1] - s_t0[q], s_h[2 * q], s_h[2 * q + 1], lane),
+                                Lw, row_ctx_from<B, IW>(Lw, tr, s_t0[q], s_t0[q + 1] - s_t0[q], s_h[2 * q], s_h[2 * q + 1], lane),
                                 lane, lanebase, g, acc);
                         s_t[i * TPB + tid] = A.b[(size_t)tr * B + lane] - acc[0];
                     }
@@ -755,13 +810,20 @@ __global__ __launch_bounds__(TPB, 1) void k_bicgstab(SolveArgs A)
             clk.lap(3);
         }
     };
-    if constexpr (RPT <= 4) {
+    if constexpr (LSB > 0) {
         if (lds_g) {
-            solve(s_g64, s_g32, std::true_type{});
+            if (lds_s)
+                solve(s_g64, s_g32, std::true_type{}, std::true_type{});
+            else
+                solve(s_g64, s_g32, std::true_type{}, std::false_type{});
+            return;
+        }
+        if (lds_s) {
+            solve(A.g64, A.g32, std::false_type{}, std::true_type{});
             return;
         }
     }
-    solve(A.g64, A.g32, std::false_type{});
+    solve(A.g64, A.g32, std::false_type{}, std::false_type{});
 }
 
 // ------------------------------------------------- streamed (HBM) kernels

@@ -315,6 +315,7 @@ static int launch_persistent(mf_matrix m, const Layout &L, const Layout *L64, in
     hipStream_t st = stream();
     SolveArgs A;
     A.L = L.view(m->d_tile_ptr);
+    A.blob_bytes = L.blob_bytes;
     A.use64 = L64 && L64 != &L ? 1 : 0;
     A.L64 = A.use64 ? L64->view(m->d_tile_ptr) : A.L;
     if (A.use64 && L64->iw != L.iw) return fail(MF_E_FORMAT, "layouts with different index forms");

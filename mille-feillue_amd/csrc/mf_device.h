@@ -326,12 +326,18 @@ __device__ __forceinline__ void slab_spmv_rows(const DevL &L, const int4 *hd, co
 // 1 FP32, 2 FP16, 3 E4M3).  Kernels built for a mask compile only those
 // classes' slab loops: fewer live registers in the SpMV.
 constexpr int CM_ALL = 0xF, CM_F64 = 0x1, CM_LOW = 0xE;
+// With CM_ACC64 the FP32, FP16 and E4M3 tiles' products are summed in FP64
+// against the FP64 vector instead (kernels for small systems, whose FP64
+// vector stays in cache): the values keep their storage precision, and
+// lossless tiles then give the FP64 product.
+constexpr int CM_ACC64 = 0x10;
 
 // y of NR tile rows (local rows q0 + k * QS, global tile rows trq + k * QS,
 // headers in LDS).  Per row, slab by slab in class order: the FP64 tiles'
 // products summed in FP64, the FP32, FP16 and E4M3 tiles' in FP32 against
-// the FP32 copy of the vector; y = a64 + (double)a32.  Only the classes in
-// CM are read (the layout holds no others).
+// the FP32 copy of the vector (in FP64 against the FP64 vector under
+// CM_ACC64); y = a64 + (double)a32.  Only the classes in CM are read (the
+// layout holds no others).
 template <int B, int IW, int NR, int QS, int UQL4 = 2, int UQ64 = 4, int CM = CM_ALL>
 __device__ __forceinline__ void tile_rows_spmv(const DevL &L, const int4 *hd, const int *t0s, int q0, int trq,
                                                int r, int lanebase, const double *__restrict__ g64,
@@ -349,9 +355,15 @@ __device__ __forceinline__ void tile_rows_spmv(const DevL &L, const int4 *hd, co
     }
     const int row0 = trq * B + r;
     if constexpr (CM & 1) slab_spmv_rows<B, IW, 0, NR, QS, UQL4, UQ64>(L, hd, t0s, q0, tcr, row0, r, lanebase, g64, a64);
-    if constexpr (CM & 2) slab_spmv_rows<B, IW, 1, NR, QS, UQL4, UQ64>(L, hd, t0s, q0, tcr, row0, r, lanebase, g32, a32);
-    if constexpr (CM & 4) slab_spmv_rows<B, IW, 2, NR, QS, UQL4, UQ64>(L, hd, t0s, q0, tcr, row0, r, lanebase, g32, a32);
-    if constexpr (CM & 8) slab_spmv_rows<B, IW, 3, NR, QS, UQL4, UQ64>(L, hd, t0s, q0, tcr, row0, r, lanebase, g32, a32);
+    if constexpr ((CM & CM_ACC64) != 0) {
+        if constexpr (CM & 2) slab_spmv_rows<B, IW, 1, NR, QS, UQL4, UQ64>(L, hd, t0s, q0, tcr, row0, r, lanebase, g64, a64);
+        if constexpr (CM & 4) slab_spmv_rows<B, IW, 2, NR, QS, UQL4, UQ64>(L, hd, t0s, q0, tcr, row0, r, lanebase, g64, a64);
+        if constexpr (CM & 8) slab_spmv_rows<B, IW, 3, NR, QS, UQL4, UQ64>(L, hd, t0s, q0, tcr, row0, r, lanebase, g64, a64);
+    } else {
+        if constexpr (CM & 2) slab_spmv_rows<B, IW, 1, NR, QS, UQL4, UQ64>(L, hd, t0s, q0, tcr, row0, r, lanebase, g32, a32);
+        if constexpr (CM & 4) slab_spmv_rows<B, IW, 2, NR, QS, UQL4, UQ64>(L, hd, t0s, q0, tcr, row0, r, lanebase, g32, a32);
+        if constexpr (CM & 8) slab_spmv_rows<B, IW, 3, NR, QS, UQL4, UQ64>(L, hd, t0s, q0, tcr, row0, r, lanebase, g32, a32);
+    }
 #pragma unroll
     for (int k = 0; k < NR; k++) y[k] = a64[k] + (double)a32[k];
     if (L.split) {

@@ -1190,6 +1190,7 @@ struct KernelChoice {
     KernelFn fn = nullptr;
     int tpb = 0, rpt = 0;
     int streamed = 0;  // 1: vectors in HBM (k_cg_st / k_bicgstab_st), rpt = 0
+    int acc64 = 0;     // 1: built with CM_ACC64 (publishes the FP64 vector only)
 };
 
 constexpr int ST_TPB = 512;
@@ -1257,11 +1258,40 @@ template <int B, int CM> KernelChoice pick_tile_cm(int iw, int method, long long
                      : pick_kernel_cm<B, 4, CM>(method, rows_per_wg);
 }
 
+// Small systems (1 or 2 rows per thread) on layouts without FP64 tiles:
+// kernels whose low-precision tiles accumulate in FP64 (CM_ACC64).  At 4
+// rows per thread the FP64 gathers of the low-precision blocking spill
+// 90-309 registers, so those systems keep FP32 accumulation.
+template <int B, int IW>
+KernelChoice pick_kernel_acc64(int method, long long rows_per_wg)
+{
+    constexpr int CMA = CM_LOW | CM_ACC64;
+    KernelChoice k;
+    const int q = (int)((rows_per_wg + 511) / 512);
+    if (method == MF_CG) {
+        if (q <= 1) k = {k_cg<B, IW, 512, 1, CMA>, 512, 1};
+        else if (q <= 2) k = {k_cg<B, IW, 512, 2, CMA>, 512, 2};
+    } else {
+        if (q <= 1) k = {k_bicgstab<B, IW, 512, 1, CMA>, 512, 1};
+        else if (q <= 2) k = {k_bicgstab<B, IW, 512, 2, CMA>, 512, 2};
+    }
+    k.acc64 = k.fn ? 1 : 0;
+    return k;
+}
+
+template <int B> KernelChoice pick_tile_acc64(int iw, int method, long long rows_per_wg)
+{
+    return iw == 1   ? pick_kernel_acc64<B, 1>(method, rows_per_wg)
+           : iw == 2 ? pick_kernel_acc64<B, 2>(method, rows_per_wg)
+                     : pick_kernel_acc64<B, 4>(method, rows_per_wg);
+}
+
 // tile sizes and class masks instantiated in their own translation units
 KernelChoice pick_b8(int iw, int method, long long rows_per_wg, bool stream);   // solver_b8.hip
 KernelChoice pick_b32(int iw, int method, long long rows_per_wg, bool stream);  // solver_b32.hip
 KernelChoice pick_b16_f64(int iw, int method, long long rows_per_wg);           // solver_f64.hip
 KernelChoice pick_b16_low(int iw, int method, long long rows_per_wg);           // solver_low.hip
+KernelChoice pick_b16_acc64(int iw, int method, long long rows_per_wg);         // solver_acc64.hip
 
 template <int B> KernelChoice pick_tile(int iw, int method, long long rows_per_wg, bool stream)
 {

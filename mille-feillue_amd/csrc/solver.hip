@@ -120,6 +120,10 @@ static int layout_cm(const Layout &L)
 
 static KernelChoice pick(int B, int iw, int method, long long rows_per_wg, bool stream, int cm = CM_ALL)
 {
+    if (B == 16 && !stream && cm == CM_LOW) {
+        const KernelChoice k = pick_b16_acc64(iw, method, rows_per_wg);
+        if (k.fn) return k;
+    }
     if (B == 16 && !stream && cm != CM_ALL) {
         const KernelChoice k = cm == CM_F64 ? pick_b16_f64(iw, method, rows_per_wg)
                                             : pick_b16_low(iw, method, rows_per_wg);
@@ -334,8 +338,9 @@ static int launch_persistent(mf_matrix m, const Layout &L, const Layout *L64, in
     A.tol = tol;
     A.rr_factor = o.rr_factor;
     A.maxit = o.maxit;
-    A.has64 = L.has64 ? 1 : 0;
-    A.haslow = L.haslow ? 1 : 0;
+    // kernels that sum low-precision tiles in FP64 gather the FP64 vector only
+    A.has64 = L.has64 || kc.acc64 ? 1 : 0;
+    A.haslow = L.haslow && !kc.acc64 ? 1 : 0;
     A.spin_limit = o.spin_limit;
     A.dep_ptr = w.dep_ptr;
     A.dep_idx = w.dep_idx;

@@ -262,8 +262,11 @@ __device__ __forceinline__ void slab_spmv_rows(const DevL &L, const int4 *hd, co
                                                const int (&tcr)[NR], int row0, int r, int lanebase,
                                                const GT *__restrict__ g, AccT (&acc)[NR])
 {
-    constexpr int NB = CLS == 0 && NR > 1 ? NR / 2 : NR;
-    constexpr int UQ = CLS == 0 ? (NB == 1 && IW == 4 ? UQ64 : 1) : (IW == 4 ? UQL4 : 2);
+    // FP64 sums (FP64 tiles, or low-precision tiles under CM_ACC64) gather
+    // FP64 entries: half the rows and one quad per step
+    constexpr bool wide = CLS == 0 || sizeof(AccT) == 8;
+    constexpr int NB = wide && NR > 1 ? NR / 2 : NR;
+    constexpr int UQ = wide ? (NB == 1 && IW == 4 ? UQ64 : 1) : (IW == 4 ? UQL4 : 2);
 #pragma unroll
     for (int kb = 0; kb < NR; kb += NB) {
         int2 h[NB];

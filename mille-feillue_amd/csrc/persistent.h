@@ -1258,32 +1258,36 @@ template <int B, int CM> KernelChoice pick_tile_cm(int iw, int method, long long
                      : pick_kernel_cm<B, 4, CM>(method, rows_per_wg);
 }
 
-// Small systems (1 or 2 rows per thread) on layouts without FP64 tiles:
-// kernels whose low-precision tiles accumulate in FP64 (CM_ACC64).  At 4
-// rows per thread the FP64 gathers of the low-precision blocking spill
-// 90-309 registers, so those systems keep FP32 accumulation.
+// Sync-bound systems on layouts without FP64 tiles -- one row per thread, or
+// one workgroup (up to 4 rows per thread) -- get kernels whose low-precision
+// tiles accumulate in FP64 (CM_ACC64, with the FP64 tiles' blocking: no
+// spills).  From 2 rows per thread on many workgroups the FP64 gathers cost
+// more than the iterations they save.
 template <int B, int IW>
-KernelChoice pick_kernel_acc64(int method, long long rows_per_wg)
+KernelChoice pick_kernel_acc64(int method, long long rows_per_wg, int G)
 {
     constexpr int CMA = CM_LOW | CM_ACC64;
     KernelChoice k;
     const int q = (int)((rows_per_wg + 511) / 512);
+    if (q > 1 && G != 1) return k;
     if (method == MF_CG) {
         if (q <= 1) k = {k_cg<B, IW, 512, 1, CMA>, 512, 1};
         else if (q <= 2) k = {k_cg<B, IW, 512, 2, CMA>, 512, 2};
+        else if (q <= 4) k = {k_cg<B, IW, 512, 4, CMA>, 512, 4};
     } else {
         if (q <= 1) k = {k_bicgstab<B, IW, 512, 1, CMA>, 512, 1};
         else if (q <= 2) k = {k_bicgstab<B, IW, 512, 2, CMA>, 512, 2};
+        else if (q <= 4) k = {k_bicgstab<B, IW, 512, 4, CMA>, 512, 4};
     }
     k.acc64 = k.fn ? 1 : 0;
     return k;
 }
 
-template <int B> KernelChoice pick_tile_acc64(int iw, int method, long long rows_per_wg)
+template <int B> KernelChoice pick_tile_acc64(int iw, int method, long long rows_per_wg, int G)
 {
-    return iw == 1   ? pick_kernel_acc64<B, 1>(method, rows_per_wg)
-           : iw == 2 ? pick_kernel_acc64<B, 2>(method, rows_per_wg)
-                     : pick_kernel_acc64<B, 4>(method, rows_per_wg);
+    return iw == 1   ? pick_kernel_acc64<B, 1>(method, rows_per_wg, G)
+           : iw == 2 ? pick_kernel_acc64<B, 2>(method, rows_per_wg, G)
+                     : pick_kernel_acc64<B, 4>(method, rows_per_wg, G);
 }
 
 // tile sizes and class masks instantiated in their own translation units
@@ -1291,7 +1295,7 @@ KernelChoice pick_b8(int iw, int method, long long rows_per_wg, bool stream);   
 KernelChoice pick_b32(int iw, int method, long long rows_per_wg, bool stream);  // solver_b32.hip
 KernelChoice pick_b16_f64(int iw, int method, long long rows_per_wg);           // solver_f64.hip
 KernelChoice pick_b16_low(int iw, int method, long long rows_per_wg);           // solver_low.hip
-KernelChoice pick_b16_acc64(int iw, int method, long long rows_per_wg);         // solver_acc64.hip
+KernelChoice pick_b16_acc64(int iw, int method, long long rows_per_wg, int G);  // solver_acc64.hip
 
 template <int B> KernelChoice pick_tile(int iw, int method, long long rows_per_wg, bool stream)
 {

@@ -118,10 +118,10 @@ static int layout_cm(const Layout &L)
     return !low ? CM_F64 : !f64 ? CM_LOW : CM_ALL;
 }
 
-static KernelChoice pick(int B, int iw, int method, long long rows_per_wg, bool stream, int cm = CM_ALL)
+static KernelChoice pick(int B, int iw, int method, long long rows_per_wg, bool stream, int cm = CM_ALL, int G = 0)
 {
     if (B == 16 && !stream && cm == CM_LOW) {
-        const KernelChoice k = pick_b16_acc64(iw, method, rows_per_wg);
+        const KernelChoice k = pick_b16_acc64(iw, method, rows_per_wg, G);
         if (k.fn) return k;
     }
     if (B == 16 && !stream && cm != CM_ALL) {
@@ -455,7 +455,7 @@ static int launch_inner(mf_matrix m, const Layout &L, const Layout *L64, int met
 {
     int G = 1, trw = 1;
     solver_grid(m->tilem, m->B, o.grid, &G, &trw);
-    KernelChoice kc = pick(m->B, L.iw, method, (long long)trw * m->B, false, layout_cm(L));
+    KernelChoice kc = pick(m->B, L.iw, method, (long long)trw * m->B, false, layout_cm(L), G);
     if (!kc.fn) return fail(MF_E_UNSUPPORTED, "no persistent kernel for this tile size and index form");
     // streaming the vectors of a system whose rows fit on chip is switched
     // off: two runs that forced it on small grids ended with a GPU fault
@@ -721,7 +721,7 @@ static int noalloc_core(mf_matrix m, int dev, const double *d_b, double *d_x, co
     int G = 1, trw = 1;
     solver_grid(m->tilem, m->B, o.grid, &G, &trw);
     if (G > NA_MAX_G) return fail(MF_E_ARG, "allocation-free solve: more than 1024 workgroups");
-    const KernelChoice kc = pick(m->B, L.iw, o.method, (long long)trw * m->B, false, layout_cm(L));
+    const KernelChoice kc = pick(m->B, L.iw, o.method, (long long)trw * m->B, false, layout_cm(L), G);
     if (!kc.fn || kc.streamed)
         return fail(MF_E_UNSUPPORTED, "allocation-free solve: system beyond register/LDS residency at this grid");
     unsigned char *base = static_cast<unsigned char *>(g_scratch[dev]);
@@ -756,7 +756,7 @@ static int noalloc_core(mf_matrix m, int dev, const double *d_b, double *d_x, co
     // lossy tiles that failed: the same solve on the FP64 layout, as mf_solve_device
     if (!exact && (status == MF_E_DIVERGED || status == MF_E_BREAKDOWN || status == MF_E_NONFINITE ||
                    status == MF_E_MAXIT)) {
-        const KernelChoice k64 = pick(m->B, m->L64.iw, o.method, (long long)trw * m->B, false, layout_cm(m->L64));
+        const KernelChoice k64 = pick(m->B, m->L64.iw, o.method, (long long)trw * m->B, false, layout_cm(m->L64), G);
         if (!k64.fn || k64.streamed) return fail(MF_E_UNSUPPORTED, "allocation-free solve: no FP64 kernel");
         rc = run(m->L64, nullptr, k64);
         if (rc) return rc;

@@ -264,6 +264,40 @@ def test_gpu_bicgstab_fp64(gpu, orc):
         _bicg_case(gpu, orc, g["rowptr"], g["col"], g["val"], b)
 
 
+@pytest.mark.parametrize("method", ["cg", "bicgstab"])
+@pytest.mark.parametrize("size", ["one_row_per_thread", "one_workgroup"])
+def test_gpu_small_lossless_mixed_sums_in_fp64(gpu, orc, method, size):
+    """Sync-bound systems without FP64 tiles (one row per thread on several
+    workgroups, or one workgroup of up to 4 rows per thread) sum their
+    low-precision tiles in FP64 against the FP64 vector.  With lossless
+    tiles (values exact in E4M3) the product is then the FP64 one up to
+    summation order, so the mixed solve takes the FP64 solve's iterations
+    (within 2) -- under FP32 accumulation the 64^2 Laplacian took about
+    45 % more -- and reaches the FP64 answer."""
+    if size == "one_row_per_thread":
+        rp, ci, va = gpu.gen_laplace2d(64, 64) if method == "cg" else gpu.gen_poisson3d(16, 16, 16)
+    else:
+        rp, ci, va = gpu.gen_laplace2d(40, 40) if method == "cg" else gpu.gen_poisson3d(12, 12, 12)
+    n = len(rp) - 1
+    b = orc.csr_spmv(rp, ci, va, np.ones(n))
+    mth = gpu.MF_CG if method == "cg" else gpu.MF_BICGSTAB
+    codes = orc.tiles(16, n, n, rp, ci, va).classes(0.0)
+    assert np.bincount(codes, minlength=5)[0] == 0  # no FP64 tiles: the layout is all low precision
+    A = gpu.Matrix(rp, ci, va)
+    x64, r64 = A.solve(b, method=mth, tol=TOL)
+    xm, rm = A.solve(b, method=mth, tol=TOL, precision=gpu.MF_MIXED, tau=0.0, rr_factor=1e-2)
+    print(f"{method} n={n} fp64 it={r64['iterations']} mixed it={rm['iterations']} rpt={rm['rows_per_thread']} "
+          f"grid={rm['grid']}", flush=True)
+    assert r64["status"] == 0 and rm["status"] == 0 and rm["relres_true"] <= TOL
+    if size == "one_row_per_thread":
+        assert rm["rows_per_thread"] == 1 and rm["grid"] > 1
+    else:
+        assert rm["grid"] == 1 and rm["rows_per_thread"] > 1
+    assert abs(rm["iterations"] - r64["iterations"]) <= 2
+    assert np.linalg.norm(xm - x64) / np.linalg.norm(x64) < 1e-8
+    A.close()
+
+
 def _diag_csr(d):
     n = len(d)
     return np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32), np.asarray(d, np.float64)

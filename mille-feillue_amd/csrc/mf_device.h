@@ -330,8 +330,8 @@ __device__ __forceinline__ void slab_spmv_rows(const DevL &L, const int4 *hd, co
 // classes' slab loops: fewer live registers in the SpMV.
 constexpr int CM_ALL = 0xF, CM_F64 = 0x1, CM_LOW = 0xE;
 // With CM_ACC64 the FP32, FP16 and E4M3 tiles' products are summed in FP64
-// against the FP64 vector instead (kernels for small systems, whose FP64
-// vector stays in cache): the values keep their storage precision, and
+// against the FP64 vector instead (kernels for sync-bound systems, whose
+// FP64 vector stays in cache): the values keep their storage precision, and
 // lossless tiles then give the FP64 product.
 constexpr int CM_ACC64 = 0x10;
 

@@ -1258,15 +1258,15 @@ template <int B, int CM> KernelChoice pick_tile_cm(int iw, int method, long long
                      : pick_kernel_cm<B, 4, CM>(method, rows_per_wg);
 }
 
-// Sync-bound systems on layouts without FP64 tiles -- one row per thread, or
-// one workgroup (up to 4 rows per thread) -- get kernels whose low-precision
-// tiles accumulate in FP64 (CM_ACC64, with the FP64 tiles' blocking: no
-// spills).  From 2 rows per thread on many workgroups the FP64 gathers cost
-// more than the iterations they save.
+// Sync-bound systems on layouts with low-precision tiles -- one row per
+// thread, or one workgroup (up to 4 rows per thread) -- get kernels whose
+// low-precision tiles accumulate in FP64 (CM_ACC64, every class compiled,
+// with the FP64 tiles' blocking: no spills).  From 2 rows per thread on many
+// workgroups the FP64 gathers cost more than the iterations they save.
 template <int B, int IW>
 KernelChoice pick_kernel_acc64(int method, long long rows_per_wg, int G)
 {
-    constexpr int CMA = CM_LOW | CM_ACC64;
+    constexpr int CMA = CM_ALL | CM_ACC64;
     KernelChoice k;
     const int q = (int)((rows_per_wg + 511) / 512);
     if (q > 1 && G != 1) return k;

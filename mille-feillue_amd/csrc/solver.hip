@@ -120,7 +120,7 @@ static int layout_cm(const Layout &L)
 
 static KernelChoice pick(int B, int iw, int method, long long rows_per_wg, bool stream, int cm = CM_ALL, int G = 0)
 {
-    if (B == 16 && !stream && cm == CM_LOW) {
+    if (B == 16 && !stream && cm != CM_F64) {
         const KernelChoice k = pick_b16_acc64(iw, method, rows_per_wg, G);
         if (k.fn) return k;
     }

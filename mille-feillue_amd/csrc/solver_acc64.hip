@@ -1,7 +1,7 @@
 // Persistent CG / BiCGSTAB kernels for 16-row tiles on sync-bound systems
-// (one row per thread, or one workgroup) whose layouts hold no FP64 tiles:
-// their FP32, FP16 and E4M3 tiles accumulate in FP64 against the FP64
-// vector (instantiated here so they compile in parallel).
+// (one row per thread, or one workgroup) whose layouts hold low-precision
+// tiles: their FP32, FP16 and E4M3 tiles accumulate in FP64 against the
+// FP64 vector (instantiated here so they compile in parallel).
 #include "persistent.h"
 
 namespace mf {

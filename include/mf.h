@@ -210,7 +210,10 @@ enum { MF_SYNC_AUTO = 0, MF_SYNC_GLOBAL = 1 };
 enum { MF_VECTORS_AUTO = 0, MF_VECTORS_HBM = 1, MF_VECTORS_ONCHIP = 2 };
 typedef struct {
     int method;         /* MF_CG or MF_BICGSTAB */
-    int precision;      /* MF_FP64 or MF_MIXED */
+    int precision;      /* MF_FP64 or MF_MIXED (FP32/FP16/E4M3 tiles summed in FP32
+                           against an FP32 copy of the vector; on sync-bound
+                           systems -- one row per thread, or one workgroup --
+                           in FP64 against the FP64 vector) */
     double tol;         /* true relative residual target (default 1e-10) */
     int maxit;          /* per inner solve (default 10000) */
     double tau;         /* mixed: per-tile rounding tolerance (default 0 = lossless) */

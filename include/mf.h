@@ -340,8 +340,9 @@ int mf_spmv_bench(mf_matrix m, int mixed, int reps, double *ms);
  * MF_VECTORS_ONCHIP and 2 to 8 right-hand sides, in the LDS of gangs of
  * workgroups).  Tiles are built on the GPU per system and joined
  * into one block-diagonal layout; opts->block picks the tile size (8, 16 or
- * 32; 0 = 16), opts->grid the target number of workgroups (0 = 2 per CU),
- * opts->precision/tau/allow_mask the per-tile classes (every system reaches
+ * 32; 0 = 16), opts->grid the target number of workgroups (0 = as many as
+ * can be resident: one per CU with 8 columns, where the kernel needs every
+ * register), opts->precision/tau/allow_mask the per-tile classes (every system reaches
  * opts->tol in true relative residual through FP64 true-residual passes). */
 typedef struct mf_batch_s *mf_batch;
 typedef struct {

@@ -328,7 +328,7 @@ __device__ __forceinline__ void slab_spmv_rows(const DevL &L, const int4 *hd, co
 // Class masks of a layout: bit c set when it holds class-c tiles (0 FP64,
 // 1 FP32, 2 FP16, 3 E4M3).  Kernels built for a mask compile only those
 // classes' slab loops: fewer live registers in the SpMV.
-constexpr int CM_ALL = 0xF, CM_F64 = 0x1, CM_LOW = 0xE;
+constexpr int CM_ALL = 0xF, CM_F64 = 0x1, CM_LOW = 0xE, CM_E4M3 = 0x8;
 // With CM_ACC64 the FP32, FP16 and E4M3 tiles' products are summed in FP64
 // against the FP64 vector instead (kernels for sync-bound systems, whose
 // FP64 vector stays in cache): the values keep their storage precision, and

@@ -1295,6 +1295,7 @@ KernelChoice pick_b8(int iw, int method, long long rows_per_wg, bool stream);   
 KernelChoice pick_b32(int iw, int method, long long rows_per_wg, bool stream);  // solver_b32.hip
 KernelChoice pick_b16_f64(int iw, int method, long long rows_per_wg);           // solver_f64.hip
 KernelChoice pick_b16_low(int iw, int method, long long rows_per_wg);           // solver_low.hip
+KernelChoice pick_b16_e4m3(int iw, int method, long long rows_per_wg);          // solver_e4m3.hip
 KernelChoice pick_b16_acc64(int iw, int method, long long rows_per_wg, int G);  // solver_acc64.hip
 
 template <int B> KernelChoice pick_tile(int iw, int method, long long rows_per_wg, bool stream)

@@ -110,12 +110,14 @@ __global__ void k_axpy(double *__restrict__ x, const double *__restrict__ d, lon
     if (i < n) x[i] += d[i];
 }
 
-// Class mask of a layout's tiles (CM_F64, CM_LOW or CM_ALL).
+// Class mask of a layout's tiles (CM_F64, CM_E4M3, CM_LOW or CM_ALL; codes
+// 2, 3 and 4 are E4M3, FP16 and FP32 tiles).
 static int layout_cm(const Layout &L)
 {
     const bool f64 = L.tiles_per_code[0] > 0;
     const bool low = L.tiles_per_code[2] + L.tiles_per_code[3] + L.tiles_per_code[4] > 0;
-    return !low ? CM_F64 : !f64 ? CM_LOW : CM_ALL;
+    const bool e4m3_only = !f64 && L.tiles_per_code[3] + L.tiles_per_code[4] == 0;
+    return !low ? CM_F64 : e4m3_only ? CM_E4M3 : !f64 ? CM_LOW : CM_ALL;
 }
 
 static KernelChoice pick(int B, int iw, int method, long long rows_per_wg, bool stream, int cm = CM_ALL, int G = 0)
@@ -125,8 +127,9 @@ static KernelChoice pick(int B, int iw, int method, long long rows_per_wg, bool 
         if (k.fn) return k;
     }
     if (B == 16 && !stream && cm != CM_ALL) {
-        const KernelChoice k = cm == CM_F64 ? pick_b16_f64(iw, method, rows_per_wg)
-                                            : pick_b16_low(iw, method, rows_per_wg);
+        const KernelChoice k = cm == CM_F64    ? pick_b16_f64(iw, method, rows_per_wg)
+                               : cm == CM_E4M3 ? pick_b16_e4m3(iw, method, rows_per_wg)
+                                               : pick_b16_low(iw, method, rows_per_wg);
         if (k.fn) return k;
     }
     switch (B) {

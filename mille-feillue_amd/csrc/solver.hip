@@ -42,9 +42,10 @@ namespace mf {
 // covers NR * 256/B consecutive tile rows; lane group q handles tile rows
 // base + k * 256/B + q (k < NR) together.  NR = 2 for layouts with narrow
 // tiles and one- or two-byte indices, 1 otherwise (measured fastest at 128^3
-// Poisson and the 96^3 27-point stencil).
+// Poisson and the 96^3 27-point stencil).  With 16-row tiles the kernel is
+// built per class mask of the layout (CM; see layout_cm), as the solvers are.
 constexpr int SPMV_TPB = 256;  // 128, 512 and 1024 measured within 1 % (or slower)
-template <int B, int IW, int NR>
+template <int B, int IW, int NR, int CM = CM_ALL>
 __global__ __launch_bounds__(SPMV_TPB) void k_spmv(DevL L, int tilem, const double *__restrict__ g64,
                                                    const float *__restrict__ g32, double *__restrict__ y)
 {
@@ -65,7 +66,7 @@ __global__ __launch_bounds__(SPMV_TPB) void k_spmv(DevL L, int tilem, const doub
     const int grp = threadIdx.x / B, r = threadIdx.x % B;
     const int lanebase = (threadIdx.x & 63) & ~(B - 1);
     double v[NR];
-    tile_rows_spmv<B, IW, NR, GPB>(L, s_h, s_t0, grp, tb + grp, r, lanebase, g64, g32, v);
+    tile_rows_spmv<B, IW, NR, GPB, 2, 4, CM>(L, s_h, s_t0, grp, tb + grp, r, lanebase, g64, g32, v);
 #pragma unroll
     for (int k = 0; k < NR; k++) {
         const int tr = tb + k * GPB + grp;
@@ -797,12 +798,18 @@ int spmv_device(mf_matrix m, const Layout &L, const double *d_x, double *d_y, bo
     const int nr = L.haslow && L.iw < 4 ? 2 : 1;
     const int nb = (int)(((long long)m->tilem * B + SPMV_TPB * nr - 1) / (SPMV_TPB * nr));
     DevL V = L.view(m->d_tile_ptr);
-#define MF_SPMV(BB, W, NR) k_spmv<BB, W, NR><<<nb, SPMV_TPB, 0, st>>>(V, m->tilem, d_x, m->ws.g32, d_y)
-#define MF_SPMV_W(BB, NR) if (L.iw == 1) MF_SPMV(BB, 1, NR); else if (L.iw == 2) MF_SPMV(BB, 2, NR); else MF_SPMV(BB, 4, NR)
-#define MF_SPMV_B(BB) if (nr == 2) { MF_SPMV_W(BB, 2); } else { MF_SPMV_W(BB, 1); }
-    if (B == 16) { MF_SPMV_B(16); }
-    else if (B == 8) { MF_SPMV_B(8); }
-    else { MF_SPMV_B(32); }
+#define MF_SPMV(BB, W, NR, C) k_spmv<BB, W, NR, C><<<nb, SPMV_TPB, 0, st>>>(V, m->tilem, d_x, m->ws.g32, d_y)
+#define MF_SPMV_W(BB, NR, C) if (L.iw == 1) MF_SPMV(BB, 1, NR, C); else if (L.iw == 2) MF_SPMV(BB, 2, NR, C); else MF_SPMV(BB, 4, NR, C)
+#define MF_SPMV_B(BB, C) if (nr == 2) { MF_SPMV_W(BB, 2, C); } else { MF_SPMV_W(BB, 1, C); }
+    if (B == 16) {
+        const int cm = layout_cm(L);
+        if (cm == CM_F64) { MF_SPMV_B(16, CM_F64); }
+        else if (cm == CM_E4M3) { MF_SPMV_B(16, CM_E4M3); }
+        else if (cm == CM_LOW) { MF_SPMV_B(16, CM_LOW); }
+        else { MF_SPMV_B(16, CM_ALL); }
+    }
+    else if (B == 8) { MF_SPMV_B(8, CM_ALL); }
+    else { MF_SPMV_B(32, CM_ALL); }
 #undef MF_SPMV_B
 #undef MF_SPMV_W
 #undef MF_SPMV
